@@ -1,0 +1,205 @@
+/*
+ * rgs_amd.h - C ABI of the MI355X (gfx950) graph-state Born sampler.
+ *
+ * Plain pointers, sizes and a HIP stream (passed as void*); no PyTorch or C++
+ * types.  Every function returns RGS_OK or an error code; rgs_last_error()
+ * gives the message of the last failure on the calling thread.  Device
+ * pointers are marked "dev"; everything else is host memory.  All device work
+ * is enqueued on `stream` (NULL = the default stream) and is asynchronous
+ * unless stated otherwise.
+ *
+ * Complex numbers are interleaved (re, im) pairs: complex64 = 2 floats,
+ * complex128 = 2 doubles.  Qubit k is bit k of a state index (qubit 0 = LSB),
+ * as in the reference (work/rgs_attack/graph_sampler.py:1).
+ *
+ * Each declaration cites the reference interface it replaces.  The reference
+ * launches CUDA kernels from Python strings through CuPy RawModule
+ * (work/rgs_attack/distributed_sampler.py:80,88-90); this library is the
+ * native boundary those Python engines would call instead.
+ */
+#ifndef RGS_AMD_H
+#define RGS_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define RGS_OK 0
+#define RGS_EINVAL 1 /* ValueError in the Python mirror      */
+#define RGS_EARITH 2 /* ArithmeticError                      */
+#define RGS_ENOMEM 3 /* MemoryError                          */
+#define RGS_EHIP 4   /* RuntimeError (HIP runtime failure)   */
+
+#define RGS_ABI_VERSION 1
+
+/* Thread-local message of the last failure ("" if none). */
+const char *rgs_last_error(void);
+int rgs_abi_version(void);
+
+/* ------------------------------------------------------------------ *
+ * Host planning: GF(2) algebra and chunked lookup tables (8-bit chunks).
+ * ------------------------------------------------------------------ */
+
+/* GF(2) inverse of an n x n 0/1 matrix (row-major uint8), n <= 64.
+ * Replaces gf2_inverse (work/rgs_attack/fourier_covariant_sampler.py:23-36).
+ * RGS_EINVAL when singular. */
+int rgs_gf2_inverse(const uint8_t *m, int n, uint8_t *inv);
+
+/* GF(2) rank of a rows x cols 0/1 matrix (cols <= 128).
+ * Replaces gf2rank (work/rgs_attack/graph_sampler.py:59-69). */
+int rgs_gf2_rank(const uint8_t *m, int rows, int cols, int *rank);
+
+/* Phase words of the graph phase (-1)^{q(x)}, q(x) = sum_{a<b} G_ab x_a x_b,
+ * for n <= 63 vertices: lut[c*256 + k] = (cross mask of chunk c code k onto
+ * higher bits) | (in-chunk parity << 63); the parity of
+ * XOR_c (lut[c*256+byte_c(x)] & (x | 1<<63)) is q(x).  lut has
+ * ceil(n/8)*256 words.  Replaces the per-element popcount loop of
+ * quadratic_phase / qmasks (work/rgs_attack/graph_sampler.py:76-85,
+ * distributed_sampler.py:57) and the phase half of build_specialization
+ * (work/rgs_attack/init_specialization.py:160-189). */
+int rgs_graph_phase_lut(const uint8_t *g, int n, uint64_t *lut);
+
+/* Fourier-covariant init tables for a balanced cut (|A| = |B| = m <= 63):
+ * map_lut[(c*256 + k)*2 + 0] = XOR of the columns of N = (G[B,A])^{-T}
+ * selected by code k of chunk c (z -> b = N z), map_lut[...*2 + 1] = phase
+ * word of the transformed quadratic q_B(N z) (same convention as above);
+ * aphase_lut = rgs_graph_phase_lut of G[A,A].  Replaces build_specialization
+ * (work/rgs_attack/init_specialization.py:160-189) without its identical-U
+ * restriction.  RGS_EINVAL if G[B,A] is singular. */
+int rgs_cov_luts(const uint8_t *g, int nv, const int32_t *A, const int32_t *B, int m,
+                 uint64_t *map_lut, uint64_t *aphase_lut);
+
+/* Per-chunk products for arbitrary per-site unitaries: wlut[c*256 + k] =
+ * prod_{t<8, c*8+t<m} u[B[c*8+t]][outB[c*8+t]][bit t of k] (complex64,
+ * computed in complex128), `scale` folded into chunk 0.  u is complex128
+ * [nv][2][2].  Replaces weight_table (init_specialization.py:98-115), which
+ * needs identical B unitaries. */
+int rgs_cov_weight_lut(const double *u, int nv, const int32_t *B, const uint8_t *outB, int m,
+                       double scale, float *wlut);
+
+/* ------------------------------------------------------------------ *
+ * Slab pipeline: init -> [Walsh] -> [diagonal phase] -> [2x2 gates] -> masses
+ * on one 2^nbits complex64 state, fused into few HBM passes.
+ * ------------------------------------------------------------------ */
+
+#define RGS_INIT_LOAD 0     /* the state already holds the input          */
+#define RGS_INIT_GRAPH 1    /* scale * (-1)^{q(x)} from graph_lut         */
+#define RGS_INIT_COV 2      /* (-1)^{q_B(Nz)} prod_c wlut[c][byte_c(Nz)] */
+#define RGS_INIT_DELTA 3    /* scale at index 0, zero elsewhere           */
+
+typedef struct rgs_slab_desc {
+  void *state;                 /* dev complex64[2^nbits], overwritten       */
+  int nbits;                   /* 6..40                                     */
+  int init_kind;               /* RGS_INIT_*                                */
+  float init_scale;            /* GRAPH / DELTA                             */
+  const uint64_t *init_lut;    /* dev GRAPH: phase words; COV: map_lut      */
+  const float *init_wlut;      /* dev COV weights (complex64)               */
+  int do_walsh;                /* unnormalised Walsh transform on all bits  */
+  const uint64_t *phase_lut;   /* dev phase words applied after the Walsh,
+                                  or NULL                                    */
+  const float *gates;          /* dev complex64 [nbits][2][2] or NULL       */
+  int gate_limit;              /* gates on qubits >= gate_limit are skipped */
+  double *mass;                /* dev float64[2^(nbits-mass_bits)] or NULL  */
+  int mass_bits;               /* granularity of the mass level (4..10)     */
+  int low_bits;                /* tile bits of the contiguous pass (6..14)  */
+  int tile_bits;               /* tile bits of strided passes (6..14)       */
+  int lane_bits;               /* contiguous bits per strided row (>= 1)    */
+} rgs_slab_desc;
+
+/* Enqueue the fused pipeline.  Replaces SpecializedDeferredSampler.compute
+ * and its kernels initcov_specialized_global, lowfwht, fusedfwht,
+ * lowprodphase, fusedprod (work/rgs_attack/specialized_deferred_sampler.py:58-74,
+ * fourier_fused_sampler.py:100-112, distributed_sampler.py:14-49,
+ * fourier_covariant_sampler.py:78-93).  When mass != NULL the last pass also
+ * writes the pairwise-tree FP64 masses of every 2^mass_bits block. */
+int rgs_slab_pipeline(const rgs_slab_desc *d, void *stream);
+
+/* Number of HBM passes and their text description ("<kind>:<first bit>+<count>
+ * ..."), for logs and the autotuner. */
+int rgs_slab_describe(const rgs_slab_desc *d, int *passes, char *text, int text_len);
+
+/* Build the upper levels of the FP64 mass pyramid in place.  pyr holds level 0
+ * (2^base_bits masses) followed by levels of 2^(base_bits-1), ..., 1 entries:
+ * level[k][i] = level[k-1][2i] + level[k-1][2i+1].  Replaces the host
+ * np.concatenate + sum of blockmass output
+ * (work/rgs_attack/distributed_sampler.py:145-152). */
+int rgs_mass_pyramid(double *pyr, int base_bits, void *stream);
+
+/* Qubit-by-qubit draws: for each uniform u, descend the pyramid from the root
+ * with t = u * root (go to child 1 iff t >= mass(child 0) and mass(child 1) > 0,
+ * subtracting mass(child 0)), then inside the 2^mass_bits leaf block with masses
+ * recomputed by the same pairwise tree from the complex64 amplitudes.
+ * Outputs the state index and p = re^2 + im^2 (float64) of each draw.
+ * Replaces DistributedSampler.draw's host searchsorted / tile copy
+ * (work/rgs_attack/distributed_sampler.py:153-157). */
+int rgs_draw(const void *state, int nbits, const double *pyr, int mass_bits,
+             const double *uniforms, int count, int64_t *out_index, double *out_p,
+             void *stream);
+
+/* float64 p = re^2 + im^2 at `count` indices (probability_at,
+ * distributed_sampler.py:158-161). */
+int rgs_gather_prob(const void *state, const int64_t *index, int count, double *out_p,
+                    void *stream);
+
+
+/* ------------------------------------------------------------------ *
+ * Linear rank-width scan: p(x) of bitstrings by a fixed vertex order.
+ * ------------------------------------------------------------------ */
+
+#define RGS_SCAN_MAX_R 62
+
+/* One compiled step: sum the vertex v.  p-vectors pack (c, y) as
+ * c | y << r_in.  out[c'] = sum_{p in L c' + span(kern)} E[p & (2^r_in - 1)]
+ * * f_v(y) * (-1)^{y * parity(c & beta)}, with L the columns lcols. */
+typedef struct rgs_scan_step {
+  int32_t v;
+  int32_t r_in;
+  int32_t r_out;
+  int32_t nker;
+  uint64_t beta;
+  uint64_t lcols[64];
+  uint64_t kern[64];
+} rgs_scan_step;
+
+/* Compile the scan for an ordering of all n <= 128 vertices; profile[j] =
+ * rank after step j.  Replaces make_plan and future_masks
+ * (work/rgs_attack/upstream_scan/scan_plan.py:45-105). */
+int rgs_scan_compile(const uint8_t *g, int n, const int32_t *order, rgs_scan_step *steps,
+                     int32_t *profile);
+
+/* Cut-rank profile by direct GF(2) elimination of every prefix cut.
+ * Replaces search_width.profile (work/rgs_attack/search_width.py:5-15) and
+ * rank_cut (order_optimization/search.cpp:22-29). */
+int rgs_scan_profile(const uint8_t *g, int n, const int32_t *order, int32_t *profile);
+
+/* Annealed order search minimising sum_j 2^{r_j} with max r <= rmax_cap
+ * (swap / insertion / reversal moves, geometric cooling over `seconds`).
+ * Replaces order_optimization/search.cpp:240-274 and search_width.py:27-45. */
+int rgs_scan_order_search(const uint8_t *g, int n, const int32_t *start, double seconds, uint64_t seed,
+                          int rmax_cap, int32_t *best_order, uint64_t *best_cost);
+
+/* Pack compiled steps into the device plan (uint64 words).  Call with
+ * blob == NULL to get the size in *words.  Ranks must be <= 28.  Replaces the
+ * per-step byte_tables / kernel_combos uploads of GPUMixin.alloc
+ * (upstream_scan/scan_plan.py:133-152, gpu_kernel.py:111-123). */
+int rgs_scan_pack(const rgs_scan_step *steps, int n, uint64_t *blob, uint64_t *words);
+
+/* p(x) of `count` proposals in one persistent launch.  plan_dev: packed plan;
+ * u_dev: complex128 [n][2][2] measurement unitaries (f_v = u[v][x_v][:]);
+ * bits_dev: uint64 [count][2] bitstrings (qubit q = bit q).  precision 64
+ * stores the tables as complex64 (rmax <= 12), 128 as complex128
+ * (rmax <= 11); sums are FP64 in both.  Outputs: out_val complex128
+ * [count] and out_exp int32 [count] with amplitude
+ * = out_val * 2^out_exp * 2^{-n/2}.  Replaces ScanOracle.amplitude /
+ * GPUMixin._sweep / scan_step_scaled (work/rgs_attack/scan_oracle.py:25-29,
+ * upstream_scan/gpu_kernel.py:59-102,125-170, gpu_kernel_fp64.py:24-67,90-135),
+ * which run one proposal per 70 launches with a host read per step. */
+int rgs_scan_batch(const uint64_t *plan_dev, int rmax, const double *u_dev, const uint64_t *bits_dev,
+                   int count, int precision, double *out_val, int32_t *out_exp, void *stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* RGS_AMD_H */

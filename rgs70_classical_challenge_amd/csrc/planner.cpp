@@ -1,0 +1,223 @@
+// Host planning: error plumbing, GF(2) algebra and chunked lookup tables.
+//
+// All arithmetic here is exact integer work on bit masks (uint64 rows for
+// matrices up to 64 columns, two words up to 128).  The tables follow the
+// conventions documented in include/rgs_amd.h.
+#include <stdint.h>
+
+#include <algorithm>
+#include <complex>
+#include <cstring>
+#include <string>
+#include <vector>
+
+#include "rgs_amd.h"
+#include "rgs_status.h"
+
+namespace rgs {
+
+static thread_local std::string g_last_error;
+
+void set_error(const std::string& msg) { g_last_error = msg; }
+int fail(int code, const std::string& msg) {
+  g_last_error = msg;
+  return code;
+}
+
+static inline int parity(uint64_t x) { return __builtin_popcountll(x) & 1; }
+
+// Rows of an n x n 0/1 matrix as masks (bit j = column j).
+static std::vector<uint64_t> pack_rows(const uint8_t* m, int rows, int cols, int stride) {
+  std::vector<uint64_t> r(rows, 0);
+  for (int i = 0; i < rows; ++i)
+    for (int j = 0; j < cols; ++j)
+      if (m[i * stride + j] & 1) r[i] |= 1ull << j;
+  return r;
+}
+
+// Gauss-Jordan over GF(2); returns false if singular.
+static bool inverse_rows(std::vector<uint64_t> a, int n, std::vector<uint64_t>& inv) {
+  inv.assign(n, 0);
+  for (int i = 0; i < n; ++i) inv[i] = 1ull << i;
+  for (int col = 0; col < n; ++col) {
+    int piv = -1;
+    for (int r = col; r < n; ++r)
+      if ((a[r] >> col) & 1) { piv = r; break; }
+    if (piv < 0) return false;
+    std::swap(a[col], a[piv]);
+    std::swap(inv[col], inv[piv]);
+    for (int r = 0; r < n; ++r)
+      if (r != col && ((a[r] >> col) & 1)) {
+        a[r] ^= a[col];
+        inv[r] ^= inv[col];
+      }
+  }
+  return true;
+}
+
+// Phase words for a quadratic form given by strict-upper rows (bit j of
+// rows[i] = Q_ij, j > i) and a linear mask: one table of 256 words per chunk.
+static void phase_words(const std::vector<uint64_t>& rows, uint64_t linear, int n, uint64_t* out) {
+  const int chunks = (n + 7) / 8;
+  const uint64_t full = n == 64 ? ~0ull : ((1ull << n) - 1);
+  for (int c = 0; c < chunks; ++c) {
+    const int lo = 8 * c, hi = std::min(n, 8 * c + 8);
+    const uint64_t higher = full & ~((hi == 64) ? ~0ull : ((1ull << hi) - 1));
+    for (int code = 0; code < 256; ++code) {
+      uint64_t word = 0;
+      if (code < (1 << (hi - lo))) {
+        const uint64_t z = (uint64_t)code << lo;
+        uint64_t cross = 0;
+        int inner = parity(z & linear);
+        for (int j = lo; j < hi; ++j)
+          if ((z >> j) & 1) {
+            cross ^= rows[j] & higher;
+            inner ^= parity(z & rows[j]);
+          }
+        word = cross | ((uint64_t)inner << 63);
+      }
+      out[c * 256 + code] = word;
+    }
+  }
+}
+
+}  // namespace rgs
+
+using namespace rgs;
+
+extern "C" const char* rgs_last_error(void) { return g_last_error.c_str(); }
+extern "C" int rgs_abi_version(void) { return RGS_ABI_VERSION; }
+
+extern "C" int rgs_gf2_inverse(const uint8_t* m, int n, uint8_t* inv) {
+  if (!m || !inv || n < 1 || n > 64) return fail(kInvalid, "gf2_inverse: need 1 <= n <= 64");
+  std::vector<uint64_t> r;
+  if (!inverse_rows(pack_rows(m, n, n, n), n, r)) return fail(kInvalid, "singular GF(2) matrix");
+  for (int i = 0; i < n; ++i)
+    for (int j = 0; j < n; ++j) inv[i * n + j] = (r[i] >> j) & 1;
+  return kOk;
+}
+
+extern "C" int rgs_gf2_rank(const uint8_t* m, int rows, int cols, int* rank) {
+  if (!m || !rank || rows < 0 || cols < 0 || cols > 128) return fail(kInvalid, "gf2_rank: cols <= 128");
+  std::vector<std::pair<uint64_t, uint64_t>> a(rows);
+  for (int i = 0; i < rows; ++i) {
+    uint64_t lo = 0, hi = 0;
+    for (int j = 0; j < cols; ++j)
+      if (m[i * cols + j] & 1) (j < 64 ? lo : hi) |= 1ull << (j & 63);
+    a[i] = {lo, hi};
+  }
+  int r = 0;
+  for (int col = 0; col < cols && r < rows; ++col) {
+    auto bit = [&](int i) { return col < 64 ? (a[i].first >> col) & 1 : (a[i].second >> (col - 64)) & 1; };
+    int piv = -1;
+    for (int i = r; i < rows; ++i)
+      if (bit(i)) { piv = i; break; }
+    if (piv < 0) continue;
+    std::swap(a[r], a[piv]);
+    for (int i = r + 1; i < rows; ++i)
+      if (bit(i)) {
+        a[i].first ^= a[r].first;
+        a[i].second ^= a[r].second;
+      }
+    ++r;
+  }
+  *rank = r;
+  return kOk;
+}
+
+extern "C" int rgs_graph_phase_lut(const uint8_t* g, int n, uint64_t* lut) {
+  if (!g || !lut || n < 1 || n > 63) return fail(kInvalid, "graph_phase_lut: need 1 <= n <= 63");
+  std::vector<uint64_t> rows(n, 0);
+  for (int i = 0; i < n; ++i)
+    for (int j = i + 1; j < n; ++j)
+      if (g[i * n + j] & 1) rows[i] |= 1ull << j;
+  for (int i = 0; i < n; ++i)
+    for (int j = 0; j < n; ++j)
+      if ((g[i * n + j] & 1) != (g[j * n + i] & 1) || (i == j && (g[i * n + j] & 1)))
+        return fail(kInvalid, "graph must be symmetric with zero diagonal");
+  phase_words(rows, 0, n, lut);
+  return kOk;
+}
+
+extern "C" int rgs_cov_luts(const uint8_t* g, int nv, const int32_t* A, const int32_t* B, int m,
+                            uint64_t* map_lut, uint64_t* aphase_lut) {
+  if (!g || !A || !B || !map_lut || m < 1 || m > 63) return fail(kInvalid, "cov_luts: need 1 <= m <= 63");
+  for (int i = 0; i < m; ++i)
+    if (A[i] < 0 || A[i] >= nv || B[i] < 0 || B[i] >= nv) return fail(kInvalid, "vertex out of range");
+  // M = G[B, A]; N = M^{-T}; b = N z.
+  std::vector<uint64_t> M(m, 0);
+  for (int i = 0; i < m; ++i)
+    for (int j = 0; j < m; ++j)
+      if (g[B[i] * nv + A[j]] & 1) M[i] |= 1ull << j;
+  std::vector<uint64_t> Minv;
+  if (!inverse_rows(M, m, Minv)) return fail(kInvalid, "cut matrix G[B,A] is singular (B marginal not uniform)");
+  // N = (M^{-1})^T: column j of N = row j of M^{-1}.
+  const std::vector<uint64_t>& colN = Minv;
+  // Transformed quadratic: P = N^T U N with U = strict upper of G[B,B].
+  std::vector<uint64_t> U(m, 0);
+  for (int i = 0; i < m; ++i)
+    for (int j = i + 1; j < m; ++j)
+      if (g[B[i] * nv + B[j]] & 1) U[i] |= 1ull << j;
+  // P_ab = sum_{i<j} N_ia U_ij N_jb = colN[a]^T U colN[b]
+  std::vector<uint64_t> Ucol(m);  // U times column b of N: (U n_b)_i = parity(U[i] & colN[b])
+  for (int b = 0; b < m; ++b) {
+    uint64_t v = 0;
+    for (int i = 0; i < m; ++i)
+      if (parity(U[i] & colN[b])) v |= 1ull << i;
+    Ucol[b] = v;
+  }
+  std::vector<uint64_t> Qrows(m, 0);
+  uint64_t linear = 0;
+  for (int a = 0; a < m; ++a)
+    for (int b = 0; b < m; ++b) {
+      int pab = parity(colN[a] & Ucol[b]);
+      int pba = parity(colN[b] & Ucol[a]);
+      if (a == b && pab) linear |= 1ull << a;
+      if (b > a && (pab ^ pba)) Qrows[a] |= 1ull << b;
+    }
+  std::vector<uint64_t> words(((m + 7) / 8) * 256);
+  phase_words(Qrows, linear, m, words.data());
+  const int chunks = (m + 7) / 8;
+  for (int c = 0; c < chunks; ++c) {
+    const int lo = 8 * c, hi = std::min(m, 8 * c + 8);
+    for (int code = 0; code < 256; ++code) {
+      uint64_t mapped = 0;
+      if (code < (1 << (hi - lo)))
+        for (int j = lo; j < hi; ++j)
+          if ((code >> (j - lo)) & 1) mapped ^= colN[j];
+      map_lut[2 * (c * 256 + code)] = mapped;
+      map_lut[2 * (c * 256 + code) + 1] = words[c * 256 + code];
+    }
+  }
+  if (aphase_lut) {
+    std::vector<uint64_t> rowsA(m, 0);
+    for (int i = 0; i < m; ++i)
+      for (int j = i + 1; j < m; ++j)
+        if (g[A[i] * nv + A[j]] & 1) rowsA[i] |= 1ull << j;
+    phase_words(rowsA, 0, m, aphase_lut);
+  }
+  return kOk;
+}
+
+extern "C" int rgs_cov_weight_lut(const double* u, int nv, const int32_t* B, const uint8_t* outB, int m,
+                                  double scale, float* wlut) {
+  if (!u || !B || !outB || !wlut || m < 1 || m > 63) return fail(kInvalid, "cov_weight_lut: need 1 <= m <= 63");
+  const int chunks = (m + 7) / 8;
+  for (int c = 0; c < chunks; ++c) {
+    for (int code = 0; code < 256; ++code) {
+      std::complex<double> w = c == 0 ? std::complex<double>(scale, 0.0) : std::complex<double>(1.0, 0.0);
+      for (int t = 0; t < 8; ++t) {
+        int j = 8 * c + t;
+        if (j >= m) break;
+        int site = B[j];
+        if (site < 0 || site >= nv || outB[j] > 1) return fail(kInvalid, "bad site or output bit");
+        int col = (code >> t) & 1;
+        const double* e = u + 2 * (site * 4 + outB[j] * 2 + col);
+        w *= std::complex<double>(e[0], e[1]);
+      }
+      wlut[2 * (c * 256 + code)] = (float)w.real();
+      wlut[2 * (c * 256 + code) + 1] = (float)w.imag();
+    }
+  }
+  return kOk;
+}

@@ -1,0 +1,157 @@
+// Batched linear rank-width scan: p(x) of many proposals in one persistent
+// launch.  One wavefront owns one proposal at a time and sweeps all n steps
+// with the environment table resident in LDS (2^rmax entries); outputs of a
+// step are gathered into registers (K per lane), the wave computes their
+// maximum modulus, and they are written back scaled by an exact power of two,
+// so there is no host round trip between steps.  Sums and weights are FP64;
+// the table is stored as complex64 (S = float2) or complex128 (S = double2,
+// the audit precision).
+#include "rgs_common.h"
+#include "rgs_amd.h"
+
+#include <algorithm>
+
+namespace rgs {
+
+template <typename S>
+__device__ __forceinline__ double2 load_c(const S& s) {
+  return make_double2((double)s.x, (double)s.y);
+}
+
+template <typename S>
+__device__ __forceinline__ S make_s(double re, double im);
+template <>
+__device__ __forceinline__ float2 make_s<float2>(double re, double im) {
+  return make_float2((float)re, (float)im);
+}
+template <>
+__device__ __forceinline__ double2 make_s<double2>(double re, double im) {
+  return make_double2(re, im);
+}
+
+template <typename S, int K>
+__global__ void __launch_bounds__(64) k_scan_batch(const uint64_t* __restrict__ plan, const double2* __restrict__ u,
+                                                   const uint64_t* __restrict__ bits, int count,
+                                                   double2* __restrict__ out_val, int32_t* __restrict__ out_exp) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  S* E = reinterpret_cast<S*>(smem);
+  const int lane = threadIdx.x;
+  const int n = (int)plan[0];
+  for (int pidx = blockIdx.x; pidx < count; pidx += gridDim.x) {
+    const uint64_t x0 = bits[2 * pidx], x1 = bits[2 * pidx + 1];
+    if (lane == 0) E[0] = make_s<S>(1.0, 0.0);
+    __syncthreads();
+    int esum = 0;
+    for (int j = 0; j < n; ++j) {
+      const uint64_t* st = plan + plan[2 + j];
+      const uint64_t hdr = st[0];
+      const int v = (int)(hdr & 255u), r_in = (int)((hdr >> 8) & 255u), r_out = (int)((hdr >> 16) & 255u);
+      const int nker = (int)((hdr >> 24) & 255u);
+      const uint64_t beta = st[1];
+      const int xv = (int)((v < 64 ? (x0 >> v) : (x1 >> (v - 64))) & 1u);
+      const double2 f0 = u[(v * 2 + xv) * 2 + 0], f1 = u[(v * 2 + xv) * 2 + 1];
+      const uint64_t lo = st[2 + lane];
+      const uint64_t* hi = st + 66;
+      const uint64_t* combos = st + 66 + (1ull << (r_out > 6 ? r_out - 6 : 0));
+      const int ncomb = 1 << nker;
+      const uint64_t mask = (1ull << r_in) - 1ull;
+      const int nk = r_out > 6 ? 1 << (r_out - 6) : 1;
+      const bool lane_ok = r_out >= 6 || lane < (1 << r_out);
+      S outv[K];
+      double mx = 0.0;
+#pragma unroll
+      for (int k = 0; k < K; ++k) {
+        if (k < nk && lane_ok) {
+          const uint64_t p0 = lo ^ hi[k];
+          double ar = 0.0, ai = 0.0;
+          for (int t = 0; t < ncomb; ++t) {
+            const uint64_t p = p0 ^ combos[t];
+            const uint64_t c = p & mask;
+            const uint32_t y = (uint32_t)(p >> r_in) & 1u;
+            const uint32_t neg = y & ((uint32_t)__popcll(c & beta) & 1u);
+            const double2 w = y ? f1 : f0;
+            const double2 e = load_c(E[c]);
+            const double pr = e.x * w.x - e.y * w.y;
+            const double pi = e.x * w.y + e.y * w.x;
+            if (neg) {
+              ar -= pr;
+              ai -= pi;
+            } else {
+              ar += pr;
+              ai += pi;
+            }
+          }
+          mx = fmax(mx, ar * ar + ai * ai);
+          outv[k] = make_s<S>(ar, ai);
+        }
+      }
+#pragma unroll
+      for (int off = 32; off > 0; off >>= 1) mx = fmax(mx, __shfl_xor(mx, off));
+      int e = 0;
+      if (mx > 0.0) frexp(sqrt(mx), &e);
+      __syncthreads();
+#pragma unroll
+      for (int k = 0; k < K; ++k) {
+        if (k < nk && lane_ok) {
+          S val = outv[k];
+          val.x = ldexp(val.x, -e);
+          val.y = ldexp(val.y, -e);
+          E[lane + 64 * k] = val;
+        }
+      }
+      esum += e;
+      __syncthreads();
+    }
+    if (lane == 0) {
+      out_val[pidx] = load_c(E[0]);
+      out_exp[pidx] = esum;
+    }
+    __syncthreads();
+  }
+}
+
+template <typename S, int K>
+static hipError_t launch_scan(const uint64_t* plan, const double2* u, const uint64_t* bits, int count,
+                              double2* val, int32_t* ex, int rmax, hipStream_t st) {
+  size_t shmem = ((size_t)1 << rmax) * sizeof(S);
+  int grid = std::min(count, 256 * 16);
+  static bool attr = false;
+  if (!attr) {
+    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_scan_batch<S, K>),
+                              hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+    attr = true;
+  }
+  hipLaunchKernelGGL((k_scan_batch<S, K>), dim3(grid), dim3(64), shmem, st, plan, u, bits, count, val, ex);
+  return hipGetLastError();
+}
+
+}  // namespace rgs
+
+using namespace rgs;
+
+extern "C" int rgs_scan_batch(const uint64_t* plan_dev, int rmax, const double* u_dev, const uint64_t* bits_dev,
+                              int count, int precision, double* out_val, int32_t* out_exp, void* stream) {
+  if (!plan_dev || !u_dev || !bits_dev || !out_val || !out_exp) return fail(kInvalid, "scan_batch: null pointer");
+  if (count <= 0) return kOk;
+  hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+  const double2* u = reinterpret_cast<const double2*>(u_dev);
+  double2* val = reinterpret_cast<double2*>(out_val);
+  hipError_t e;
+  if (precision == 64) {
+    if (rmax <= 6) e = launch_scan<float2, 1>(plan_dev, u, bits_dev, count, val, out_exp, 6, st);
+    else if (rmax <= 8) e = launch_scan<float2, 4>(plan_dev, u, bits_dev, count, val, out_exp, 8, st);
+    else if (rmax <= 10) e = launch_scan<float2, 16>(plan_dev, u, bits_dev, count, val, out_exp, 10, st);
+    else if (rmax <= 12) e = launch_scan<float2, 64>(plan_dev, u, bits_dev, count, val, out_exp, 12, st);
+    else return fail(kInvalid, "scan_batch: rmax above 12 needs the large-table scan");
+  } else if (precision == 128) {
+    if (rmax <= 6) e = launch_scan<double2, 1>(plan_dev, u, bits_dev, count, val, out_exp, 6, st);
+    else if (rmax <= 8) e = launch_scan<double2, 4>(plan_dev, u, bits_dev, count, val, out_exp, 8, st);
+    else if (rmax <= 10) e = launch_scan<double2, 16>(plan_dev, u, bits_dev, count, val, out_exp, 10, st);
+    else if (rmax <= 11) e = launch_scan<double2, 32>(plan_dev, u, bits_dev, count, val, out_exp, 11, st);
+    else return fail(kInvalid, "scan_batch: complex128 batch limited to rmax <= 11");
+  } else {
+    return fail(kInvalid, "precision must be 64 (complex64 tables) or 128 (complex128)");
+  }
+  if (e != hipSuccess) return fail(kHip, std::string("k_scan_batch: ") + hipGetErrorString(e));
+  return kOk;
+}

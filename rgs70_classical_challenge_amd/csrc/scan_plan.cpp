@@ -1,0 +1,308 @@
+// Host side of the linear rank-width scan: plan compiler, cut-rank profiles,
+// the order annealer, and the packed device plan.
+//
+// Positions are bits of 128-bit masks (n <= 128).  Step j sums the vertex at
+// position j; the environment is indexed by coordinates over a basis of the
+// syndrome space (parities of processed neighbours of each later position).
+// Generators of the next space are the current basis with bit j cleared and
+// the vertex's later-neighbour mask; echelon reduction with highest-bit
+// pivots gives the new basis, a right inverse (lcols) and a kernel basis of
+// the map p = (c, y) -> c'.  The packed device plan stores, per step,
+// p0 = L c' split into a 64-entry table for the low six bits of c' and a
+// table for the remaining bits, plus all kernel combinations.
+#include <stdint.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <random>
+#include <string>
+#include <vector>
+
+#include "rgs_amd.h"
+#include "rgs_status.h"
+
+namespace rgs {
+
+using u128 = unsigned __int128;
+
+static inline int msb128(u128 x) {
+  uint64_t hi = (uint64_t)(x >> 64);
+  return hi ? 127 - __builtin_clzll(hi) : 63 - __builtin_clzll((uint64_t)x);
+}
+
+static bool check_graph(const uint8_t* g, int n, std::string& err) {
+  for (int i = 0; i < n; ++i)
+    for (int j = 0; j < n; ++j)
+      if ((g[i * n + j] & 1) != (g[j * n + i] & 1) || (i == j && (g[i * n + j] & 1))) {
+        err = "graph must be symmetric with zero diagonal";
+        return false;
+      }
+  return true;
+}
+
+static bool check_order(const int32_t* order, int n, std::vector<int>& pos, std::string& err) {
+  pos.assign(n, -1);
+  for (int j = 0; j < n; ++j) {
+    if (order[j] < 0 || order[j] >= n || pos[order[j]] >= 0) {
+      err = "order must be a permutation of 0..n-1";
+      return false;
+    }
+    pos[order[j]] = j;
+  }
+  return true;
+}
+
+// fut[j]: later neighbours of the vertex at position j, as position bits.
+static std::vector<u128> future_masks(const uint8_t* g, int n, const int32_t* order, const std::vector<int>& pos) {
+  std::vector<u128> fut(n, 0);
+  for (int j = 0; j < n; ++j)
+    for (int w = 0; w < n; ++w)
+      if ((g[order[j] * n + w] & 1) && pos[w] > j) fut[j] |= (u128)1 << pos[w];
+  return fut;
+}
+
+int compile(const uint8_t* g, int n, const int32_t* order, rgs_scan_step* steps, int32_t* profile,
+            std::string& err) {
+  std::vector<int> pos;
+  if (!check_graph(g, n, err) || !check_order(order, n, pos, err)) return kInvalid;
+  std::vector<u128> fut = future_masks(g, n, order, pos);
+  std::vector<u128> basis;
+  for (int j = 0; j < n; ++j) {
+    rgs_scan_step& st = steps[j];
+    std::memset(&st, 0, sizeof st);
+    st.v = order[j];
+    st.r_in = (int32_t)basis.size();
+    if (st.r_in > RGS_SCAN_MAX_R) {
+      err = "cut rank exceeds the scan limit";
+      return kInvalid;
+    }
+    for (size_t i = 0; i < basis.size(); ++i) st.beta |= (uint64_t)((basis[i] >> j) & 1) << i;
+    std::vector<u128> gens;
+    const u128 clear = ~((u128)1 << j);
+    for (auto b : basis) gens.push_back(b & clear);
+    gens.push_back(fut[j]);
+    std::vector<u128> nb;
+    std::vector<int> piv;
+    std::vector<uint64_t> coords;
+    std::vector<int> creator;
+    for (size_t i = 0; i < gens.size(); ++i) {
+      u128 w = gens[i];
+      uint64_t coord = 0;
+      for (size_t k = 0; k < nb.size(); ++k)
+        if ((w >> piv[k]) & 1) {
+          w ^= nb[k];
+          coord |= 1ull << k;
+        }
+      if (w) {
+        piv.push_back(msb128(w));
+        nb.push_back(w);
+        coord |= 1ull << (nb.size() - 1);
+        creator.push_back((int)i);
+      }
+      coords.push_back(coord);
+    }
+    const int r_out = (int)nb.size();
+    st.r_out = r_out;
+    for (int k = 0; k < r_out; ++k) {
+      int i = creator[k];
+      uint64_t p = 1ull << i;
+      uint64_t rest = coords[i] & ~(1ull << k);
+      for (int l = 0; l < k; ++l)
+        if ((rest >> l) & 1) p ^= st.lcols[l];
+      st.lcols[k] = p;
+    }
+    std::vector<char> is_creator(gens.size(), 0);
+    for (int i : creator) is_creator[i] = 1;
+    int nk = 0;
+    for (size_t i = 0; i < gens.size(); ++i) {
+      if (is_creator[i]) continue;
+      uint64_t kv = 1ull << i;
+      for (int k = 0; k < r_out; ++k)
+        if ((coords[i] >> k) & 1) kv ^= st.lcols[k];
+      st.kern[nk++] = kv;
+    }
+    st.nker = nk;
+    if (profile) profile[j] = r_out;
+    basis.swap(nb);
+  }
+  return kOk;
+}
+
+// Rank over GF(2) of the cut (prefix | rest) by elimination on 128-bit rows.
+static int cut_rank(const std::vector<u128>& adj, u128 left, int n) {
+  u128 full = n == 128 ? ~(u128)0 : (((u128)1 << n) - 1);
+  u128 right = full & ~left;
+  std::vector<u128> piv(128, 0);
+  int rank = 0;
+  u128 todo = left;
+  while (todo) {
+    int v = msb128(todo);
+    todo &= ~((u128)1 << v);
+    u128 x = adj[v] & right;
+    while (x) {
+      int p = msb128(x);
+      if (piv[p]) x ^= piv[p];
+      else {
+        piv[p] = x;
+        ++rank;
+        break;
+      }
+    }
+  }
+  return rank;
+}
+
+static std::vector<u128> adjacency(const uint8_t* g, int n) {
+  std::vector<u128> adj(n, 0);
+  for (int i = 0; i < n; ++i)
+    for (int j = 0; j < n; ++j)
+      if (g[i * n + j] & 1) adj[i] |= (u128)1 << j;
+  return adj;
+}
+
+static void profile_of(const std::vector<u128>& adj, const std::vector<int>& order, int n, std::vector<int>& prof,
+                       int lo = 0) {
+  prof.resize(n);
+  u128 prefix = 0;
+  for (int k = 0; k < lo; ++k) prefix |= (u128)1 << order[k];
+  for (int k = lo; k < n; ++k) {
+    prefix |= (u128)1 << order[k];
+    prof[k] = (k == n - 1) ? 0 : cut_rank(adj, prefix, n);
+  }
+}
+
+static uint64_t cost_of(const std::vector<int>& prof) {
+  uint64_t c = 0;
+  for (int r : prof) c += 1ull << r;
+  return c;
+}
+
+}  // namespace rgs
+
+using namespace rgs;
+
+extern "C" int rgs_scan_compile(const uint8_t* g, int n, const int32_t* order, rgs_scan_step* steps,
+                                int32_t* profile) {
+  if (!g || !order || !steps || n < 1 || n > 128) return fail(kInvalid, "scan_compile: need 1 <= n <= 128");
+  std::string err;
+  int rc = compile(g, n, order, steps, profile, err);
+  return rc ? fail(rc, err) : kOk;
+}
+
+extern "C" int rgs_scan_profile(const uint8_t* g, int n, const int32_t* order, int32_t* profile) {
+  if (!g || !order || !profile || n < 1 || n > 128) return fail(kInvalid, "scan_profile: need 1 <= n <= 128");
+  std::string err;
+  std::vector<int> pos;
+  if (!check_graph(g, n, err) || !check_order(order, n, pos, err)) return fail(kInvalid, err);
+  std::vector<int> ord(order, order + n), prof;
+  profile_of(adjacency(g, n), ord, n, prof);
+  for (int k = 0; k < n; ++k) profile[k] = prof[k];
+  return kOk;
+}
+
+extern "C" int rgs_scan_order_search(const uint8_t* g, int n, const int32_t* start, double seconds, uint64_t seed,
+                                     int rmax_cap, int32_t* best_order, uint64_t* best_cost) {
+  if (!g || !start || !best_order || n < 2 || n > 128) return fail(kInvalid, "order_search: need 2 <= n <= 128");
+  std::string err;
+  std::vector<int> pos;
+  if (!check_graph(g, n, err) || !check_order(start, n, pos, err)) return fail(kInvalid, err);
+  const std::vector<u128> adj = adjacency(g, n);
+  std::vector<int> cur(start, start + n), best, prof, cprof;
+  profile_of(adj, cur, n, prof);
+  uint64_t cur_cost = cost_of(prof);
+  int cur_peak = *std::max_element(prof.begin(), prof.end());
+  best = cur;
+  uint64_t best_c = cur_peak <= rmax_cap ? cur_cost : ~0ull;
+  std::mt19937_64 rng(seed);
+  std::uniform_real_distribution<double> unif(0.0, 1.0);
+  auto t0 = std::chrono::steady_clock::now();
+  auto elapsed = [&]() { return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count(); };
+  // Annealing: moves are swap, insertion and segment reversal; the
+  // temperature decays geometrically over the budget, relative to the cost.
+  uint64_t it = 0;
+  while (true) {
+    if ((it & 63) == 0 && elapsed() >= seconds) break;
+    ++it;
+    double frac = std::min(1.0, elapsed() / std::max(seconds, 1e-9));
+    double temp = 0.05 * std::pow(1e-3, frac);
+    std::vector<int> cand = cur;
+    int i = (int)(rng() % n), j = (int)(rng() % n);
+    if (i == j) continue;
+    if (i > j) std::swap(i, j);
+    int move = (int)(rng() % 3);
+    if (move == 0) std::swap(cand[i], cand[j]);
+    else if (move == 1) std::rotate(cand.begin() + i, cand.begin() + j, cand.begin() + j + 1);
+    else std::reverse(cand.begin() + i, cand.begin() + j + 1);
+    cprof = prof;
+    profile_of(adj, cand, n, cprof, i);
+    int peak = *std::max_element(cprof.begin(), cprof.end());
+    uint64_t c = cost_of(cprof);
+    double diff = (double)c - (double)cur_cost;
+    bool take = diff <= 0 || unif(rng) < std::exp(-diff / (temp * (double)cur_cost));
+    if (peak > rmax_cap && cur_peak <= rmax_cap) take = false;
+    if (take) {
+      cur.swap(cand);
+      prof.swap(cprof);
+      cur_cost = c;
+      cur_peak = peak;
+      if (peak <= rmax_cap && c < best_c) {
+        best_c = c;
+        best = cur;
+      }
+    }
+  }
+  if (best_c == ~0ull) return fail(kInvalid, "no order within the rank cap found");
+  for (int k = 0; k < n; ++k) best_order[k] = best[k];
+  if (best_cost) *best_cost = best_c;
+  return kOk;
+}
+
+// Packed device plan (uint64 words):
+//   [0] n  [1] rmax  [2 .. 2+n) offset of each step
+//   step: hdr = v | r_in << 8 | r_out << 16 | nker << 24, beta,
+//         lo[64]  (p0 of the low six bits of c'),
+//         hi[2^max(0, r_out-6)] (p0 of the remaining bits),
+//         combos[2^nker]
+extern "C" int rgs_scan_pack(const rgs_scan_step* steps, int n, uint64_t* blob, uint64_t* words) {
+  if (!steps || n < 1 || n > 128 || !words) return fail(kInvalid, "scan_pack: bad arguments");
+  uint64_t w = 2 + (uint64_t)n;
+  int rmax = 0;
+  for (int j = 0; j < n; ++j) {
+    const rgs_scan_step& st = steps[j];
+    if (st.r_out > 28 || st.nker > 16) return fail(kInvalid, "scan_pack: rank above 28 or kernel above 16");
+    rmax = std::max(rmax, st.r_out);
+    uint64_t hi = 1ull << std::max(0, st.r_out - 6);
+    uint64_t off = w;
+    w += 2 + 64 + hi + (1ull << st.nker);
+    if (blob) {
+      blob[2 + j] = off;
+      uint64_t* p = blob + off;
+      p[0] = (uint64_t)st.v | ((uint64_t)st.r_in << 8) | ((uint64_t)st.r_out << 16) | ((uint64_t)st.nker << 24);
+      p[1] = st.beta;
+      for (int c = 0; c < 64; ++c) {
+        uint64_t x = 0;
+        for (int i = 0; i < 6 && i < st.r_out; ++i)
+          if ((c >> i) & 1) x ^= st.lcols[i];
+        p[2 + c] = x;
+      }
+      for (uint64_t k = 0; k < hi; ++k) {
+        uint64_t x = 0;
+        for (int i = 6; i < st.r_out; ++i)
+          if ((k >> (i - 6)) & 1) x ^= st.lcols[i];
+        p[66 + k] = x;
+      }
+      uint64_t* cb = p + 66 + hi;
+      cb[0] = 0;
+      for (int k = 0; k < st.nker; ++k)
+        for (uint64_t t = 0; t < (1ull << k); ++t) cb[(1ull << k) + t] = cb[t] ^ st.kern[k];
+    }
+  }
+  if (blob) {
+    blob[0] = (uint64_t)n;
+    blob[1] = (uint64_t)rmax;
+  }
+  *words = w;
+  return kOk;
+}

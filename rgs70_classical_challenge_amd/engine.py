@@ -1,0 +1,389 @@
+"""Sampler engines: the drop-in Python boundary over the HIP slab pipeline.
+
+Two engines share the kernels:
+
+* ``GraphStateSampler`` - the plain method on a full n-qubit state
+  (x)_k u[k] |G> (graph_sampler.graph_vector + apply_one,
+  work/rgs_attack/graph_sampler.py:87-88,150-151), computed by the slab
+  pipeline init -> Walsh -> gates (``pipeline='slab'``: g(y) = 2^{-n}(-1)^{q(y)},
+  unnormalised Walsh, gates u_k H), the literal circuit
+  (``'literal'``: delta -> Walsh -> graph phase -> u), or with H^n|0> folded
+  into the init as the reference's graph_vector does (``'folded'``).
+  Draws descend the FP64 mass pyramid qubit by qubit on the device.
+
+* ``ConditionalSampler`` - the reference's production algorithm for a
+  balanced full-rank cut A|B (SpecializedDeferredSampler,
+  work/rgs_attack/specialized_deferred_sampler.py:40-74, with
+  DeferredGlobalMixin, deferred_global_sampler.py:40-88): the Fourier-covariant
+  init from chunk tables (any per-site unitaries), Walsh, the A-side graph
+  phase and the A gates, with the terminal gate on bit m-1 deferred to the
+  draw.  Method names, return tuples, exceptions and the 3-uniform draw
+  schedule follow the reference (compute / masses / draw / probability_at /
+  host / sync).
+
+Device memory and streams come from PyTorch (plumbing only); every kernel is
+in librgs_amd.so.  There is no CPU fallback.
+"""
+import time
+
+import numpy as np
+
+from . import _lib
+from ._lib import call, ptr
+
+HADAMARD = np.array([[1, 1], [1, -1]], np.complex128) / np.sqrt(2.0)
+
+
+def _torch():
+    import torch
+    return torch
+
+
+def default_tuning(nbits):
+    """Tile choices used when no tuned plan is given."""
+    return dict(low_bits=min(14, nbits), tile_bits=14, lane_bits=4)
+
+
+class _Slab:
+    """A 2^nbits complex64 state, its lookup tables and its mass pyramid on one device."""
+
+    def __init__(self, nbits, device, mass_bits=8, tuning=None):
+        torch = _torch()
+        if not torch.cuda.is_available():
+            raise RuntimeError('no HIP device visible')
+        _lib.lib()
+        self.torch = torch
+        self.nbits = nbits
+        self.device = torch.device('cuda', device)
+        free, _total = torch.cuda.mem_get_info(self.device)
+        need = (8 << nbits) + (16 << max(0, nbits - mass_bits)) + (64 << 20)
+        if free < need:
+            raise MemoryError(f'device {device}: need {need} bytes, free {free}')
+        self.tuning = dict(default_tuning(nbits), **(tuning or {}))
+        self.state = torch.empty(1 << nbits, dtype=torch.complex64, device=self.device)
+        self.mass_bits = max(4, min(mass_bits, 8, nbits, self.tuning['low_bits']))
+        self.base_bits = nbits - self.mass_bits
+        self.pyramid = torch.empty((2 << self.base_bits) - 1, dtype=torch.float64, device=self.device)
+        self.stream = torch.cuda.current_stream(self.device)
+
+    def to_device(self, array, dtype=None):
+        t = self.torch.as_tensor(np.ascontiguousarray(array))
+        if dtype is not None:
+            t = t.to(dtype)
+        return t.to(self.device, non_blocking=False)
+
+    def desc(self, init_kind, init_scale=0.0, init_lut=None, init_wlut=None, walsh=False, phase_lut=None,
+             gates=None, gate_limit=None, mass=True):
+        d = _lib.SlabDesc()
+        d.state = ptr(self.state)
+        d.nbits = self.nbits
+        d.init_kind = init_kind
+        d.init_scale = init_scale
+        d.init_lut = ptr(init_lut)
+        d.init_wlut = ptr(init_wlut)
+        d.do_walsh = int(walsh)
+        d.phase_lut = ptr(phase_lut)
+        d.gates = ptr(gates)
+        d.gate_limit = self.nbits if gate_limit is None else gate_limit
+        d.mass = ptr(self.pyramid) if mass else None
+        d.mass_bits = self.mass_bits
+        d.low_bits = self.tuning['low_bits']
+        d.tile_bits = self.tuning['tile_bits']
+        d.lane_bits = self.tuning['lane_bits']
+        return d
+
+    def run(self, d, mass=True):
+        s = self.stream.cuda_stream
+        call('rgs_slab_pipeline', ctypes_ref(d), s)
+        if mass:
+            call('rgs_mass_pyramid', ptr(self.pyramid), self.base_bits, s)
+
+    def describe(self, d):
+        import ctypes
+        n = ctypes.c_int()
+        buf = ctypes.create_string_buffer(2048)
+        call('rgs_slab_describe', ctypes_ref(d), ctypes.byref(n), buf, 2048)
+        return n.value, buf.value.decode()
+
+    def level(self, bits):
+        """Masses of aligned 2^bits blocks (bits >= mass_bits) as a host float64 array."""
+        lvl = bits - self.mass_bits
+        if lvl < 0:
+            raise ValueError('mass level finer than the pyramid base')
+        B = self.base_bits
+        off = (2 << B) - (2 << (B - lvl))
+        return self.pyramid[off:off + (1 << (B - lvl))].cpu().numpy()
+
+    def root(self):
+        return float(self.pyramid[-1].item())
+
+    def draw_many(self, uniforms):
+        torch = self.torch
+        u = torch.as_tensor(np.ascontiguousarray(uniforms, np.float64)).to(self.device)
+        idx = torch.empty(len(u), dtype=torch.int64, device=self.device)
+        p = torch.empty(len(u), dtype=torch.float64, device=self.device)
+        call('rgs_draw', ptr(self.state), self.nbits, ptr(self.pyramid), self.mass_bits, ptr(u), len(u),
+             ptr(idx), ptr(p), self.stream.cuda_stream)
+        return idx.cpu().numpy(), p.cpu().numpy()
+
+    def gather_prob(self, indices):
+        torch = self.torch
+        ix = torch.as_tensor(np.ascontiguousarray(indices, np.int64)).to(self.device)
+        out = torch.empty(len(ix), dtype=torch.float64, device=self.device)
+        call('rgs_gather_prob', ptr(self.state), ptr(ix), len(ix), ptr(out), self.stream.cuda_stream)
+        return out.cpu().numpy()
+
+    def sync(self):
+        self.torch.cuda.synchronize(self.device)
+
+
+def ctypes_ref(d):
+    import ctypes
+    return ctypes.byref(d)
+
+
+def _gates_array(mats):
+    """complex64 [n][2][2] -> float32 [n*8] interleaved (u00, u01, u10, u11)."""
+    m = np.ascontiguousarray(np.asarray(mats, np.complex128).reshape(-1, 4).astype(np.complex64))
+    return m.view(np.float32).reshape(-1)
+
+
+def graph_phase_lut(g):
+    g = np.ascontiguousarray(np.asarray(g, np.uint8))
+    n = len(g)
+    lut = np.zeros(((n + 7) // 8) * 256, np.uint64)
+    call('rgs_graph_phase_lut', ptr(g), n, ptr(lut))
+    return lut
+
+
+def gf2_rank(m):
+    import ctypes
+    m = np.ascontiguousarray(np.asarray(m, np.uint8))
+    r = ctypes.c_int()
+    call('rgs_gf2_rank', ptr(m), m.shape[0], m.shape[1], ctypes.byref(r))
+    return r.value
+
+
+def gf2_inverse(m):
+    m = np.ascontiguousarray(np.asarray(m, np.uint8))
+    out = np.zeros_like(m)
+    call('rgs_gf2_inverse', ptr(m), len(m), ptr(out))
+    return out
+
+
+class GraphStateSampler:
+    """Plain exact sampler on the full state (x)_k u[k] |G> of one graph."""
+
+    PIPELINES = ('slab', 'literal', 'folded')
+
+    def __init__(self, g, u, device=0, pipeline='slab', mass_bits=8, tuning=None):
+        g = np.asarray(g, np.uint8)
+        u = np.asarray(u, np.complex128)
+        n = len(g)
+        if g.shape != (n, n) or u.shape != (n, 2, 2):
+            raise ValueError('graph must be n x n and unitaries n x 2 x 2')
+        if np.max(np.abs(u @ np.conj(np.transpose(u, (0, 2, 1))) - np.eye(2))) > 1e-12:
+            raise ValueError('measurement matrices are not unitary')
+        if pipeline not in self.PIPELINES:
+            raise ValueError(f'pipeline must be one of {self.PIPELINES}')
+        self.g, self.u, self.n, self.pipeline = g, u, n, pipeline
+        self.slab = _Slab(n, device, mass_bits, tuning)
+        sl = self.slab
+        self._lut = sl.to_device(graph_phase_lut(g))
+        if pipeline == 'slab':
+            mats = u @ HADAMARD
+        else:
+            mats = u
+        self._gates = sl.to_device(_gates_array(mats))
+        if pipeline == 'slab':
+            self._desc = sl.desc(_lib.INIT_GRAPH, 2.0 ** (-n), init_lut=self._lut, walsh=True, gates=self._gates)
+        elif pipeline == 'literal':
+            self._desc = sl.desc(_lib.INIT_DELTA, 1.0, walsh=True, phase_lut=self._lut, gates=self._gates)
+        else:
+            self._desc = sl.desc(_lib.INIT_GRAPH, 2.0 ** (-n / 2), init_lut=self._lut, gates=self._gates)
+        self.passes, self.plan_text = sl.describe(self._desc)
+        self._norm = None
+
+    def compute(self):
+        """Build the state and its mass pyramid; returns stage seconds."""
+        t = time.perf_counter()
+        self.slab.run(self._desc)
+        self.slab.sync()
+        self._norm = None
+        return {'state_and_masses': time.perf_counter() - t}
+
+    def masses(self, tilebits=None):
+        """(mass float64[nmass], norm, tilebits, nmass) as DistributedSampler.masses
+        (distributed_sampler.py:145-152), from the device pyramid."""
+        tb = min(20, self.n) if tilebits is None else tilebits
+        tb = max(tb, self.slab.mass_bits)
+        mass = self.slab.level(tb)
+        norm = float(mass.sum())
+        if not np.isfinite(norm) or abs(norm - 1) > 1e-3:
+            raise ArithmeticError(f'Conditional norm check failed: {norm}')
+        return mass, norm, tb, len(mass)
+
+    def norm(self):
+        r = self.slab.root()
+        if not np.isfinite(r) or abs(r - 1) > 1e-3:
+            raise ArithmeticError(f'norm check failed: {r}')
+        return r
+
+    def sample(self, uniforms):
+        """Device qubit-by-qubit draws, one uniform each: (indices, p)."""
+        return self.slab.draw_many(uniforms)
+
+    def draw(self, rng):
+        """One draw consuming one uniform: (index, p, norm)."""
+        norm = self.norm()
+        idx, p = self.slab.draw_many([rng.random()])
+        return int(idx[0]), float(p[0]), norm
+
+    def probability_at(self, index):
+        if not 0 <= index < (1 << self.n):
+            raise ValueError('index outside the state')
+        return float(self.slab.gather_prob([index])[0])
+
+    def host(self):
+        return self.slab.state.cpu().numpy()
+
+    def sync(self):
+        self.slab.sync()
+
+
+def _check_unitary(m):
+    m = np.asarray(m, np.complex128)
+    if np.max(np.abs(m.conj().T @ m - np.eye(2))) > 1e-12:
+        raise ValueError('Last product gate is not unitary')
+    return m
+
+
+class ConditionalSampler:
+    """Exact conditional Born sampler for a balanced full-rank cut (one device).
+
+    Drop-in for SpecializedDeferredSampler / the deferred fused engines: same
+    constructor arguments (chunk_mib, fusion_bits, lane_bits, lookup_bits are
+    accepted; tiles come from ``tuning``), same methods.  ``defer_top`` keeps
+    the terminal gate on bit m-1 out of the state and applies it in complex128
+    at draw time with a third uniform, as DeferredGlobalMixin does.
+    """
+
+    def __init__(self, g, u, A, B, devices=(0,), chunk_mib=1, fusion_bits=5, lane_bits=5, lookup_bits=8,
+                 defer_top=True, mass_bits=8, tuning=None):
+        g = np.ascontiguousarray(np.asarray(g, np.uint8))
+        u = np.ascontiguousarray(np.asarray(u, np.complex128))
+        A, B = [int(x) for x in A], [int(x) for x in B]
+        if len(A) != len(B):
+            raise ValueError('balanced partition required')
+        if lookup_bits != 8:
+            raise ValueError('device lookup tables use 8-bit chunks')
+        if len(list(devices)) != 1:
+            raise ValueError('one device per engine; shard by running one engine per GPU')
+        m = len(A)
+        self.g, self.u, self.A, self.B, self.m = g, u, A, B, m
+        self.ng = 1
+        self.defer_top = defer_top
+        self.slab = _Slab(m, list(devices)[0], mass_bits, tuning)
+        sl = self.slab
+        chunks = (m + 7) // 8
+        map_lut = np.zeros(chunks * 256 * 2, np.uint64)
+        aphase = np.zeros(chunks * 256, np.uint64)
+        Aa = np.asarray(A, np.int32)
+        Ba = np.asarray(B, np.int32)
+        call('rgs_cov_luts', ptr(g), len(g), ptr(Aa), ptr(Ba), m, ptr(map_lut), ptr(aphase))
+        self._map = sl.to_device(map_lut)
+        self._aphase = sl.to_device(aphase)
+        self._Ba = Ba
+        self._u = np.ascontiguousarray(u)
+        self._gates = sl.to_device(_gates_array(u[A]))
+        self._wlut_host = np.zeros(chunks * 256 * 2, np.float32)
+        self._wlut = sl.to_device(self._wlut_host)
+        self.deferred_global_unitary = _check_unitary(u[A[-1]]) if defer_top else None
+        self.n = 1 << (m - 1)          # shard size of the reference's two-device layout
+        self._desc = sl.desc(_lib.INIT_COV, 0.0, init_lut=self._map, init_wlut=self._wlut, walsh=True,
+                             phase_lut=self._aphase, gates=self._gates, gate_limit=m - 1 if defer_top else m)
+        self.passes, self.plan_text = sl.describe(self._desc)
+
+    def compute(self, outB):
+        if len(outB) != self.m:
+            raise ValueError('wrong output condition length')
+        ob = np.asarray(outB, np.uint8)
+        if np.any(ob > 1):
+            raise ValueError('outB must contain bits')
+        stage = {}
+        t = time.perf_counter()
+        call('rgs_cov_weight_lut', ptr(self._u), len(self.g), ptr(self._Ba), ptr(ob), self.m,
+             2.0 ** (-self.m / 2), ptr(self._wlut_host))
+        self._wlut.copy_(self.slab.torch.as_tensor(self._wlut_host))
+        stage['weights'] = time.perf_counter() - t
+        t = time.perf_counter()
+        self.slab.run(self._desc)
+        self.slab.sync()
+        stage['state_and_masses'] = time.perf_counter() - t
+        stage['linear_permutation_phase'] = 0.0
+        return stage
+
+    def masses(self):
+        """Tile masses (tiles of 2^min(20, m-1) inside each half), norm, tilebits, nmass per half."""
+        tb = min(20, self.m - 1)
+        tb = max(tb, self.slab.mass_bits)
+        mass = self.slab.level(tb)
+        norm = float(mass.sum())
+        if not np.isfinite(norm) or abs(norm - 1) > 1e-3:
+            raise ArithmeticError(f'Conditional norm check failed: {norm}')
+        return mass, norm, tb, len(mass) // 2
+
+    def _slice(self, start, stop):
+        return self.slab.state[start:stop].cpu().numpy()
+
+    def transformed_pair(self, local_index):
+        if not 0 <= local_index < self.n:
+            raise ValueError('Local index outside shard')
+        raw = np.array([complex(self._slice(local_index, local_index + 1)[0]),
+                        complex(self._slice(self.n + local_index, self.n + local_index + 1)[0])], np.complex128)
+        return self.deferred_global_unitary @ raw
+
+    def draw(self, rng):
+        """Reference draw: tile by searchsorted over FP64 tile masses, index in
+        the tile by searchsorted over float64 p, then (deferred) the terminal
+        bit; exactly 3 uniforms when deferred (distributed_sampler.py:153-157,
+        deferred_global_sampler.py:72-78)."""
+        mass, norm, tb, nmass = self.masses()
+        block = int(np.searchsorted(np.cumsum(mass), rng.random() * norm, side='right'))
+        half, chunk = block // nmass, block % nmass
+        start = half * self.n + (chunk << tb)
+        tile = self._slice(start, min(start + (1 << tb), (half + 1) * self.n))
+        p = tile.real.astype(np.float64) ** 2 + tile.imag.astype(np.float64) ** 2
+        within = int(np.searchsorted(np.cumsum(p), rng.random() * p.sum(), side='right'))
+        index = start + within
+        if not self.defer_top:
+            return index, float(p[within]), norm
+        local = index % self.n
+        pair = self.transformed_pair(local)
+        pp = np.abs(pair) ** 2
+        total = float(pp.sum())
+        if not np.isfinite(total) or total <= 0:
+            raise ArithmeticError('Invalid transformed local mass')
+        bit = int(rng.random() * total >= pp[0])
+        return bit * self.n + local, float(pp[bit]), norm
+
+    def probability_at(self, index):
+        if self.defer_top:
+            bit = int(index // self.n)
+            if bit not in (0, 1):
+                raise ValueError('Bad global index')
+            return float(abs(self.transformed_pair(index % self.n)[bit]) ** 2)
+        return float(self.slab.gather_prob([index])[0])
+
+    def host(self):
+        raw = self.slab.state.cpu().numpy().astype(np.complex128)
+        if self.defer_top:
+            blk = raw.reshape(2, -1)
+            a, b = blk[0].copy(), blk[1].copy()
+            U = self.deferred_global_unitary
+            blk[0] = U[0, 0] * a + U[0, 1] * b
+            blk[1] = U[1, 0] * a + U[1, 1] * b
+        return raw
+
+    def sync(self):
+        self.slab.sync()

@@ -1,0 +1,151 @@
+"""Batched rank-scan scorer: the drop-in for the reference's ScanOracle.
+
+``ScanOracle(g, u, order, device=0, fp64=False)`` keeps the reference's
+constructor and ``amplitude(bits)`` (work/rgs_attack/scan_oracle.py:11-29)
+and adds ``amplitudes(bits_batch)`` / ``probabilities(bits_batch)``, which
+score thousands of proposals in one persistent launch (librgs_amd
+rgs_scan_batch).  The plan is compiled by the library's C++ planner;
+``order_search`` is its annealer (the reference's order_optimization/search.cpp).
+"""
+import ctypes
+
+import numpy as np
+
+from . import _lib
+from ._lib import call, ptr
+
+
+class ScanStep(ctypes.Structure):
+    _fields_ = [('v', ctypes.c_int32), ('r_in', ctypes.c_int32), ('r_out', ctypes.c_int32),
+                ('nker', ctypes.c_int32), ('beta', ctypes.c_uint64),
+                ('lcols', ctypes.c_uint64 * 64), ('kern', ctypes.c_uint64 * 64)]
+
+
+_V = ctypes.c_void_p
+_lib.register('rgs_scan_compile', ctypes.c_int, [_V, ctypes.c_int, _V, _V, _V])
+_lib.register('rgs_scan_profile', ctypes.c_int, [_V, ctypes.c_int, _V, _V])
+_lib.register('rgs_scan_order_search', ctypes.c_int,
+              [_V, ctypes.c_int, _V, ctypes.c_double, ctypes.c_uint64, ctypes.c_int, _V, ctypes.POINTER(ctypes.c_uint64)])
+_lib.register('rgs_scan_pack', ctypes.c_int, [_V, ctypes.c_int, _V, ctypes.POINTER(ctypes.c_uint64)])
+_lib.register('rgs_scan_batch', ctypes.c_int,
+              [_V, ctypes.c_int, _V, _V, ctypes.c_int, ctypes.c_int, _V, _V, _V])
+
+
+def _graph(g):
+    return np.ascontiguousarray(np.asarray(g, np.uint8))
+
+
+def profile(g, order):
+    """Cut-rank profile of an order (rank after each prefix)."""
+    g = _graph(g)
+    o = np.ascontiguousarray(np.asarray(order, np.int32))
+    out = np.zeros(len(g), np.int32)
+    call('rgs_scan_profile', ptr(g), len(g), ptr(o), ptr(out))
+    return out.tolist()
+
+
+def order_search(g, start=None, seconds=5.0, seed=20261030, rmax_cap=62):
+    """Annealed order minimising sum 2^r (max r <= rmax_cap).  Returns (order, cost)."""
+    g = _graph(g)
+    n = len(g)
+    s = np.ascontiguousarray(np.arange(n, dtype=np.int32) if start is None else np.asarray(start, np.int32))
+    best = np.zeros(n, np.int32)
+    cost = ctypes.c_uint64()
+    call('rgs_scan_order_search', ptr(g), n, ptr(s), float(seconds), int(seed), int(rmax_cap), ptr(best),
+         ctypes.byref(cost))
+    return best.tolist(), int(cost.value)
+
+
+def bits_to_words(bits):
+    """uint8 [count][n] (qubit q in column q) -> uint64 [count][2]."""
+    b = np.asarray(bits, np.uint8)
+    if b.ndim == 1:
+        b = b[None, :]
+    count, n = b.shape
+    if n > 128:
+        raise ValueError('at most 128 qubits')
+    words = np.zeros((count, 2), np.uint64)
+    for q in range(n):
+        words[:, q >> 6] |= b[:, q].astype(np.uint64) << np.uint64(q & 63)
+    return words
+
+
+class ScanPlan:
+    def __init__(self, g, order):
+        g = _graph(g)
+        n = len(g)
+        self.n = n
+        self.order = [int(x) for x in order]
+        o = np.ascontiguousarray(np.asarray(self.order, np.int32))
+        self.steps = (ScanStep * n)()
+        prof = np.zeros(n, np.int32)
+        call('rgs_scan_compile', ptr(g), n, ptr(o), ctypes.addressof(self.steps), ptr(prof))
+        self.ranks = prof.tolist()
+        self.rmax = max(self.ranks)
+        self.sum_states = sum(1 << r for r in self.ranks)
+
+    def packed(self):
+        words = ctypes.c_uint64()
+        call('rgs_scan_pack', ctypes.addressof(self.steps), self.n, None, ctypes.byref(words))
+        blob = np.zeros(words.value, np.uint64)
+        call('rgs_scan_pack', ctypes.addressof(self.steps), self.n, ptr(blob), ctypes.byref(words))
+        return blob
+
+    def table_bytes(self, precision=64):
+        """Algorithmic table traffic of one proposal: every step reads its input
+        table once and writes its output table once."""
+        size = 8 if precision == 64 else 16
+        r_in = [0] + self.ranks[:-1]
+        return sum(size * ((1 << a) + (1 << b)) for a, b in zip(r_in, self.ranks))
+
+
+class ScanOracle:
+    """p(x) by the linear rank-width scan on one device (drop-in for ScanOracle)."""
+
+    def __init__(self, g, u, order, source=None, device=0, gpu=True, fp64=False):
+        import torch
+        if not gpu:
+            raise ValueError('this scorer runs on the device only')
+        if not torch.cuda.is_available():
+            raise RuntimeError('no HIP device visible')
+        self.torch = torch
+        self.n = len(g)
+        self.u = np.ascontiguousarray(np.asarray(u, np.complex128))
+        self.order = list(order)
+        self.plan = ScanPlan(g, order)
+        self.ranks = self.plan.ranks
+        self.rmax = self.plan.rmax
+        self.steps = self.plan.steps
+        self.precision = 128 if fp64 else 64
+        self.device = torch.device('cuda', device)
+        self._plan = torch.as_tensor(self.plan.packed().view(np.int64)).to(self.device)
+        self._u = torch.as_tensor(self.u.view(np.float64).reshape(-1)).to(self.device)
+        self.stream = torch.cuda.current_stream(self.device)
+
+    def raw(self, bits, precision=None):
+        """Enqueue a batch; returns device tensors (value complex128 pairs, exponent)."""
+        torch = self.torch
+        words = bits_to_words(bits)
+        count = len(words)
+        dev_bits = torch.as_tensor(words.view(np.int64)).to(self.device)
+        val = torch.empty((count, 2), dtype=torch.float64, device=self.device)
+        ex = torch.empty(count, dtype=torch.int32, device=self.device)
+        call('rgs_scan_batch', ptr(self._plan), self.rmax, ptr(self._u), ptr(dev_bits), count,
+             precision or self.precision, ptr(val), ptr(ex), self.stream.cuda_stream)
+        return val, ex, dev_bits
+
+    def amplitudes(self, bits, precision=None):
+        val, ex, _ = self.raw(bits, precision)
+        v = val.cpu().numpy()
+        e = ex.cpu().numpy().astype(np.float64)
+        scale = np.exp2(e - self.n / 2)
+        return (v[:, 0] + 1j * v[:, 1]) * scale
+
+    def probabilities(self, bits, precision=None):
+        val, ex, _ = self.raw(bits, precision)
+        v = val.cpu().numpy()
+        e = ex.cpu().numpy().astype(np.float64)
+        return (v[:, 0] ** 2 + v[:, 1] ** 2) * np.exp2(2 * e - self.n)
+
+    def amplitude(self, bits):
+        return complex(self.amplitudes(np.asarray(bits, np.uint8)[None, :])[0])

@@ -1,0 +1,134 @@
+"""The C ABI library on the CPU: it loads, exports every declared symbol, and
+its host planning functions (GF(2) algebra, lookup tables, pass schedules)
+agree with the oracle.  No device calls."""
+import ctypes
+import os
+import re
+
+import numpy as np
+import pytest
+
+from oracle import graph_state as gs
+from rgs70_classical_challenge_amd import _lib
+from rgs70_classical_challenge_amd.instances import instance
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def declared_symbols():
+    names = set()
+    inc = os.path.join(ROOT, 'include')
+    for f in os.listdir(inc):
+        if f.endswith('.h'):
+            text = open(os.path.join(inc, f)).read()
+            text = re.sub(r'/\*.*?\*/', '', text, flags=re.S)
+            names |= set(re.findall(r'\b(rgs_[a-z0-9_]+)\s*\(', text))
+    return names
+
+
+def test_library_exports_every_declared_symbol():
+    lib = ctypes.CDLL(_lib.LIB_PATH)
+    names = declared_symbols()
+    assert len(names) >= 10
+    missing = [n for n in sorted(names) if not hasattr(lib, n)]
+    assert not missing, missing
+    assert lib.rgs_abi_version() == 1
+
+
+def test_binding_covers_declarations():
+    missing = declared_symbols() - set(_lib.SIGNATURES)
+    assert not missing, missing
+
+
+def test_gf2_inverse_and_rank(golden_dir):
+    from rgs70_classical_challenge_amd.engine import gf2_inverse, gf2_rank
+    d = np.load(f'{golden_dir}/rgs_graph.npz')
+    g, A, B = d['graph'], d['A'].tolist(), d['B'].tolist()
+    M = g[np.ix_(B, A)]
+    assert np.array_equal(gf2_inverse(M), gs.gf2_inverse(M))
+    assert gf2_rank(M) == gs.gf2_rank(M) == 35
+    rng = np.random.default_rng(0)
+    for _ in range(5):
+        m = rng.integers(0, 2, (30, 70), dtype=np.uint8)
+        assert gf2_rank(m) == gs.gf2_rank(m)
+    with pytest.raises(ValueError):
+        gf2_inverse(np.ones((4, 4), np.uint8))
+
+
+def test_graph_phase_lut_matches_oracle():
+    from rgs70_classical_challenge_amd.engine import graph_phase_lut
+    for n in (6, 20, 34, 63):
+        g, _ = instance(n if n % 2 == 0 else n, 5)
+        lut = graph_phase_lut(g)
+        ref = gs.graph_phase_tables(g)[:, :, 1].reshape(-1)
+        assert np.array_equal(lut, ref)
+
+
+def test_cov_luts_match_oracle(golden_dir):
+    for name in ('induced_12q.npz', 'induced_40q.npz', 'rgs_graph.npz'):
+        d = np.load(f'{golden_dir}/{name}')
+        g, u, A, B = d['graph'], d['measurement_unitaries'], d['A'].tolist(), d['B'].tolist()
+        m = len(A)
+        chunks = (m + 7) // 8
+        map_lut = np.zeros(chunks * 256 * 2, np.uint64)
+        aph = np.zeros(chunks * 256, np.uint64)
+        gc, Aa, Ba = np.ascontiguousarray(g), np.asarray(A, np.int32), np.asarray(B, np.int32)
+        _lib.call('rgs_cov_luts', _lib.ptr(gc), len(g), _lib.ptr(Aa), _lib.ptr(Ba), m, _lib.ptr(map_lut),
+                  _lib.ptr(aph))
+        N = gs.gf2_inverse(g[np.ix_(B, A)]).T
+        Q, ell = gs.transformed_quadratic(g[np.ix_(B, B)], N)
+        ref = gs.phase_map_tables(m, N, Q, ell).reshape(-1)
+        assert np.array_equal(map_lut, ref), name
+        assert np.array_equal(aph, gs.graph_phase_tables(g[np.ix_(A, A)])[:, :, 1].reshape(-1))
+        ob = np.random.default_rng(1).integers(0, 2, m).astype(np.uint8)
+        w = np.zeros(chunks * 256 * 2, np.float32)
+        uc = np.ascontiguousarray(u)
+        _lib.call('rgs_cov_weight_lut', _lib.ptr(uc), len(g), _lib.ptr(Ba), _lib.ptr(ob), m, 2.0 ** (-m / 2),
+                  _lib.ptr(w))
+        wc = w.view(np.complex64).reshape(chunks, 256)
+        wref = gs.weight_tables(u, B, ob.tolist(), 2.0 ** (-m / 2))
+        for c in range(chunks):
+            valid = 1 << min(8, m - 8 * c)
+            assert np.allclose(wc[c, :valid], wref[c, :valid], rtol=2e-7, atol=0)
+
+
+def test_cov_luts_reject_singular_cut():
+    g = np.zeros((8, 8), np.uint8)
+    A, B, out = np.arange(4, dtype=np.int32), np.arange(4, 8, dtype=np.int32), np.zeros(512, np.uint64)
+    with pytest.raises(ValueError, match='singular'):
+        _lib.call('rgs_cov_luts', _lib.ptr(g), 8, _lib.ptr(A), _lib.ptr(B), 4, _lib.ptr(out), None)
+
+
+def describe(nbits, walsh, gates=True, phase=False, mass=True, init=_lib.INIT_GRAPH, low=14, tile=14, lane=4):
+    d = _lib.SlabDesc()
+    d.state = 1
+    d.nbits = nbits
+    d.init_kind = init
+    d.init_lut = 1
+    d.do_walsh = int(walsh)
+    d.phase_lut = 1 if phase else None
+    d.gates = 1 if gates else None
+    d.gate_limit = nbits
+    d.mass = 1 if mass else None
+    d.mass_bits = 8
+    d.low_bits, d.tile_bits, d.lane_bits = low, tile, lane
+    n = ctypes.c_int()
+    buf = ctypes.create_string_buffer(4096)
+    _lib.call('rgs_slab_describe', ctypes.byref(d), ctypes.byref(n), buf, 4096)
+    return n.value, buf.value.decode()
+
+
+def test_pass_schedules():
+    # n=34: 14 contiguous + two strided groups of 10 bits
+    n, text = describe(34, walsh=True)
+    assert n == 5, text
+    assert text.split()[0].startswith('low[init][wht]')
+    assert '[wht][gates]:24+10' in text and text.split()[-1].startswith('low[gates][mass]')
+    n, text = describe(34, walsh=False)
+    assert n == 3, text
+    n, text = describe(12, walsh=True)
+    assert n == 1 and '[wht]' in text and '[mass]' in text
+    n, text = describe(20, walsh=True, low=8, tile=8, lane=3)
+    assert n == 1 + 2 + 1 + 2 + 1, text
+    with pytest.raises(ValueError):
+        describe(34, walsh=True, tile=15)
