@@ -97,16 +97,31 @@ typedef struct rgs_slab_desc {
   const uint64_t *init_lut;    /* dev GRAPH: phase words; COV: map_lut      */
   const float *init_wlut;      /* dev COV weights (complex64)               */
   int do_walsh;                /* unnormalised Walsh transform on all bits  */
-  const uint64_t *phase_lut;   /* dev phase words applied after the Walsh,
+  const uint64_t *phase_lut;   /* dev sign words applied after the Walsh,
                                   or NULL                                    */
-  const float *gates;          /* dev complex64 [nbits][2][2] or NULL       */
-  int gate_limit;              /* gates on qubits >= gate_limit are skipped */
+  const float *dphase_lut;     /* dev complex64 [ceil(nbits/8)][256] diagonal
+                                  phase applied with phase_lut, or NULL      */
+  const float *rot;            /* HOST float32 [nbits][2] (cos, sin) of the
+                                  real rotation of each qubit, or NULL; read
+                                  when a pass is enqueued (the values travel
+                                  in the kernel arguments)                   */
+  int gate_limit;              /* rotations on qubits >= gate_limit skipped */
   double *mass;                /* dev float64[2^(nbits-mass_bits)] or NULL  */
   int mass_bits;               /* granularity of the mass level (4..10)     */
   int low_bits;                /* tile bits of the contiguous pass (6..14)  */
   int tile_bits;               /* tile bits of strided passes (6..14)       */
   int lane_bits;               /* contiguous bits per strided row (>= 1)    */
 } rgs_slab_desc;
+
+/* Factor 2x2 unitaries u (complex128 [n][2][2]) as u = D1 R D2 with
+ * R = [[c, -s], [s, c]] real, D1 = diag(e^{i a0}, e^{i a1}),
+ * D2 = diag(1, e^{i b}).  Outputs (each optional): rot float32 [n][2] (c, s);
+ * dphase_lut complex64 [ceil(n/8)][256], the product of the D2 phases of the
+ * qubits < gate_limit selected by each byte; d1 complex128 [n][2].  The slab
+ * pipeline applies D2 with the pointwise phase and R per qubit; D1 changes no
+ * probability and is applied to amplitudes read back (graph_sampler.apply_one
+ * semantics, work/rgs_attack/graph_sampler.py:71-74). */
+int rgs_gate_zyz(const double *u, int n, int gate_limit, float *rot, float *dphase_lut, double *d1);
 
 /* Enqueue the fused pipeline.  Replaces SpecializedDeferredSampler.compute
  * and its kernels initcov_specialized_global, lowfwht, fusedfwht,
@@ -115,6 +130,12 @@ typedef struct rgs_slab_desc {
  * fourier_covariant_sampler.py:78-93).  When mass != NULL the last pass also
  * writes the pairwise-tree FP64 masses of every 2^mass_bits block. */
 int rgs_slab_pipeline(const rgs_slab_desc *d, void *stream);
+
+/* Enqueue only pass `pass` (0-based) of the same schedule, so a caller can
+ * time each HBM pass; running every pass in order equals rgs_slab_pipeline.
+ * No reference counterpart beyond the per-stage timers of compute
+ * (work/rgs_attack/distributed_sampler.py:127-143). */
+int rgs_slab_pass(const rgs_slab_desc *d, int pass, void *stream);
 
 /* Number of HBM passes and their text description ("<kind>:<first bit>+<count>
  * ..."), for logs and the autotuner. */
@@ -189,8 +210,8 @@ int rgs_scan_pack(const rgs_scan_step *steps, int n, uint64_t *blob, uint64_t *w
 /* p(x) of `count` proposals in one persistent launch.  plan_dev: packed plan;
  * u_dev: complex128 [n][2][2] measurement unitaries (f_v = u[v][x_v][:]);
  * bits_dev: uint64 [count][2] bitstrings (qubit q = bit q).  precision 64
- * stores the tables as complex64 (rmax <= 12), 128 as complex128
- * (rmax <= 11); sums are FP64 in both.  Outputs: out_val complex128
+ * stores the tables as complex64 (rmax <= 13), 128 as complex128
+ * (rmax <= 12); sums are FP64 in both.  Outputs: out_val complex128
  * [count] and out_exp int32 [count] with amplitude
  * = out_val * 2^out_exp * 2^{-n/2}.  Replaces ScanOracle.amplitude /
  * GPUMixin._sweep / scan_step_scaled (work/rgs_attack/scan_oracle.py:25-29,
@@ -198,6 +219,45 @@ int rgs_scan_pack(const rgs_scan_step *steps, int n, uint64_t *blob, uint64_t *w
  * which run one proposal per 70 launches with a host read per step. */
 int rgs_scan_batch(const uint64_t *plan_dev, int rmax, const double *u_dev, const uint64_t *bits_dev,
                    int count, int precision, double *out_val, int32_t *out_exp, void *stream);
+
+/* ---- learned proposal (MADE) -------------------------------------------
+ * The reference has no counterpart for any entry point below: the learned
+ * proposal is new work.  All matrices are row-major; bf16 values are raw
+ * uint16 bit patterns (round-to-nearest-even from FP32). */
+
+/* C[M][N] (+)= alpha * op(A) op(B) with op = transpose when ta / tb is 1;
+ * A is [M][K] (or [K][M]), B is [K][N] (or [N][K]), bf16, FP32 MFMA
+ * accumulation.  Epilogue in order: + bias[N] (may be NULL), ReLU if relu,
+ * multiply by (dmask[i][j] != 0) if dmask is non-NULL (ReLU backward), then
+ * store FP32 to C (may be NULL; accumulate adds to C) and/or bf16 to Cb
+ * (may be NULL). */
+int rgs_gemm_bf16(int ta, int tb, int M, int N, int K, const uint16_t *A, int lda, const uint16_t *B, int ldb,
+                  float alpha, float *C, int ldc, uint16_t *Cb, int ldcb, const float *bias, int relu,
+                  const uint16_t *dmask, int ldm, int accumulate, void *stream);
+
+/* Bernoulli negative log-likelihood of bits x[B][n] (bf16 0/1) under
+ * logits[B][n]: nll[b] = sum_i softplus(l) - x l.  If dlogits is non-NULL it
+ * receives scale * (sigmoid(l) - x) in FP32, and dlogits_bf16 (may be NULL)
+ * the same in bf16. */
+int rgs_made_bce(const float *logits, const uint16_t *x, int B, int n, float scale, float *nll, float *dlogits,
+                 uint16_t *dlogits_bf16, void *stream);
+
+/* out[j] (+)= sum_r in[r][j] over rows (bias gradients), FP32 / bf16 input. */
+int rgs_colsum(const float *in, int rows, int cols, int ld, float *out, int accumulate, void *stream);
+int rgs_colsum_bf16(const uint16_t *in, int rows, int cols, int ld, float *out, void *stream);
+
+/* One masked Adam step on FP32 master weights w[count] (bias-corrected with
+ * the device step counter *t_dev, which this call increments first);
+ * entries with mask 0 stay untouched; wb receives the bf16 copy. */
+int rgs_adam(float *w, const float *grad, const float *mask, float *m, float *v, uint16_t *wb, long long count,
+             int *t_dev, float lr, float b1, float b2, float eps, void *stream);
+
+/* Ancestral sampling of column o: x[b][o] = (u[b][o] < sigmoid(logits[b][o]))
+ * with u an FP64 [B][n] array of uniforms; x holds bf16 1.0 / 0.0. */
+int rgs_sample_col(const float *logits, const double *u, int B, int n, int o, uint16_t *x, void *stream);
+
+/* bf16 copy (round to nearest even) of an FP32 array. */
+int rgs_f32_to_bf16(const float *in, uint16_t *out, long long count, void *stream);
 
 #ifdef __cplusplus
 }

@@ -14,9 +14,10 @@ Two draws are restated:
   binary tree of FP64 pairwise sums over p(x) = re^2 + im^2 (level 0 = leaves),
   descended from the root with one uniform per sample, qubit n-1 first.  With
   the same leaf values and the same pairwise association, host and device
-  masses are bit-identical, so draws are identical whenever the uniform does
-  not fall within rounding of a node boundary; ``tree_draw`` reports that
-  margin.
+  masses are bit-identical, so the draws are identical; against another
+  state (the complex128 oracle) they agree wherever the uniform is farther
+  from a node boundary than the L1 distance of the two distributions allows
+  (``tree_draw`` reports the margin).
 """
 import numpy as np
 
@@ -90,19 +91,20 @@ def tree_draw(levels, u):
     """Descend the pyramid with target t = u * root.
 
     At a node with children (L, R): go right iff t >= L and R > 0, then t -= L.
-    Returns (index, relative margin) where the margin is the smallest
-    |t - L| / (L + R) met on the path: the distance to a decision flip.
+    Returns (index, margin) where the margin is the smallest |t - L| met on
+    the path, in units of total mass (absolute).  If two mass pyramids differ
+    in their leaves by delta in L1 norm, every decision agrees whenever the
+    margin exceeds 3 * delta / root (t moves by at most 2 delta, L by delta).
     """
     top = len(levels) - 1
-    t = u * levels[top][0]
+    root = levels[top][0]
+    t = u * root
     idx = 0
     margin = np.inf
     for lev in range(top - 1, -1, -1):
         left = levels[lev][2 * idx]
         right = levels[lev][2 * idx + 1]
-        node = left + right
-        if node > 0:
-            margin = min(margin, abs(t - left) / node)
+        margin = min(margin, abs(t - left) / root)
         if t >= left and right > 0:
             t -= left
             idx = 2 * idx + 1

@@ -34,7 +34,8 @@ class SlabDesc(ctypes.Structure):
         ('init_wlut', c_void_p),
         ('do_walsh', c_int),
         ('phase_lut', c_void_p),
-        ('gates', c_void_p),
+        ('dphase_lut', c_void_p),
+        ('rot', c_void_p),
         ('gate_limit', c_int),
         ('mass', c_void_p),
         ('mass_bits', c_int),
@@ -52,8 +53,10 @@ SIGNATURES = {
     'rgs_gf2_rank': (c_int, [c_void_p, c_int, c_int, P(c_int)]),
     'rgs_graph_phase_lut': (c_int, [c_void_p, c_int, c_void_p]),
     'rgs_cov_luts': (c_int, [c_void_p, c_int, c_void_p, c_void_p, c_int, c_void_p, c_void_p]),
+    'rgs_gate_zyz': (c_int, [c_void_p, c_int, c_int, c_void_p, c_void_p, c_void_p]),
     'rgs_cov_weight_lut': (c_int, [c_void_p, c_int, c_void_p, c_void_p, c_int, c_double, c_void_p]),
     'rgs_slab_pipeline': (c_int, [P(SlabDesc), c_void_p]),
+    'rgs_slab_pass': (c_int, [P(SlabDesc), c_int, c_void_p]),
     'rgs_slab_describe': (c_int, [P(SlabDesc), P(c_int), c_char_p, c_int]),
     'rgs_mass_pyramid': (c_int, [c_void_p, c_int, c_void_p]),
     'rgs_draw': (c_int, [c_void_p, c_int, c_void_p, c_int, c_void_p, c_int, c_void_p, c_void_p, c_void_p]),

@@ -1,0 +1,112 @@
+"""Tile autotuner for the slab pipeline.
+
+For a state size n it times every admissible (low_bits, tile_bits,
+lane_bits) choice of the fused pass schedule on the device - each candidate
+runs the whole pipeline a few times and keeps its best time - and records the
+fastest.  Results are cached per device name in ``tuned.json`` next to this
+file; engines built with ``tuning='auto'`` read the table and fall back to the
+default shape for sizes it does not hold.
+
+    python -m rgs70_classical_challenge_amd.autotune 16 30   # tune n = 16..30
+"""
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+TABLE = os.path.join(HERE, 'tuned.json')
+
+
+def candidates(n):
+    out = []
+    for low in (12, 13, 14):
+        if low > n:
+            continue
+        for tile in (12, 13, 14):
+            for lane in (2, 3, 4, 5, 6):
+                if tile - lane < 4:
+                    continue
+                out.append(dict(low_bits=low, tile_bits=tile, lane_bits=lane))
+    if n < 12:
+        out.append(dict(low_bits=n, tile_bits=14, lane_bits=4))
+    return out
+
+
+def device_key(torch):
+    return torch.cuda.get_device_name(0).replace(' ', '_') or 'gpu'
+
+
+def load_table(torch=None):
+    if not os.path.exists(TABLE):
+        return {}
+    table = json.load(open(TABLE))
+    if torch is None:
+        return table
+    return table.get(device_key(torch), {})
+
+
+def lookup(n, torch):
+    t = load_table(torch)
+    return t.get(str(n))
+
+
+def tune(n, reps=3, seed=7):
+    """Time every candidate for size n; returns (best tuning, report list)."""
+    import torch
+    from . import _lib
+    from .engine import GraphStateSampler, ctypes_ref
+    from .instances import instance
+    g, u = instance(n, seed)
+    s = GraphStateSampler(g, u, pipeline='slab')
+    sl = s.slab
+    st = sl.stream
+    report = []
+    best = None
+    for cand in candidates(n):
+        sl.tuning = dict(cand)
+        try:
+            d = sl.desc(_lib.INIT_GRAPH, 2.0 ** (-n), init_lut=s._lut, walsh=True, gates=s._gates)
+            passes, text = sl.describe(d)
+        except ValueError:
+            continue
+        times = []
+        for _ in range(reps + 1):
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record(st)
+            _lib.call('rgs_slab_pipeline', ctypes_ref(d), st.cuda_stream)
+            e1.record(st)
+            torch.cuda.synchronize()
+            times.append(e0.elapsed_time(e1))
+        ms = min(times[1:])
+        report.append(dict(cand, passes=passes, ms=ms))
+        if best is None or ms < best[1]:
+            best = (dict(cand), ms)
+    del s
+    torch.cuda.empty_cache()
+    return best, report
+
+
+def main(argv):
+    """argv: lo [hi [output path]] - the table is merged into the output file
+    (default: the package's tuned.json)."""
+    import torch
+    lo, hi = int(argv[0]), int(argv[1]) if len(argv) > 1 else int(argv[0])
+    out = argv[2] if len(argv) > 2 else TABLE
+    table = json.load(open(out)) if os.path.exists(out) else load_table()
+    key = device_key(torch)
+    dev = table.setdefault(key, {})
+    for n in range(lo, hi + 1):
+        t0 = time.time()
+        (best, ms), report = tune(n)
+        dev[str(n)] = dict(best, ms=round(ms, 4), candidates=len(report))
+        print(json.dumps({'n': n, 'best': best, 'ms': round(ms, 4), 'tuned_in_s': round(time.time() - t0, 2)}),
+              flush=True)
+    with open(out, 'w') as f:
+        json.dump(table, f, indent=1, sort_keys=True)
+
+
+if __name__ == '__main__':
+    main(sys.argv[1:])

@@ -1,0 +1,134 @@
+"""Many uneven graph-state instances in one captured launch per memory wave.
+
+Each instance (n qubits, its graph and unitaries) gets a state slab, mass
+pyramid, uniforms and outputs; instances are packed into waves whose total
+device footprint stays under a budget, and each wave is captured once as a
+HIP graph that forks its instances over several streams (default priority)
+and joins them - so the whole wave is one ``graphLaunch`` from the host.
+Tile shapes per n come from the autotuner table (``autotune.py``) when
+present.  ``run_eager`` is the one-launch-sequence-per-instance baseline.
+The reference runs one state per process invocation
+(work/rgs_attack/born_production.py:44-58); no batched counterpart exists.
+"""
+import numpy as np
+
+from . import _lib
+from ._lib import call, ptr
+from .engine import GraphStateSampler
+from .instances import instance
+
+
+def uneven_instances(count=256, seed=20261204, nmin=16, nmax=30):
+    """n ~ U{nmin..nmax} per instance (seeded); 3-regular graphs for even n,
+    Erdos-Renyi G(n, 3/n) for odd n; QR unitaries."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    ns = rng.integers(nmin, nmax + 1, count)
+    seeds = rng.integers(0, 2 ** 62, count)
+    return [(int(n), int(s)) for n, s in zip(ns, seeds)]
+
+
+def footprint(n, samples, mass_bits=8):
+    return (8 << n) + (16 << max(0, n - mass_bits)) + samples * 24 + (1 << 16)
+
+
+class BatchSampler:
+    def __init__(self, specs, samples=1000, budget_bytes=200 << 30, streams=4, tuning='auto', seed=1):
+        import torch
+        self.torch = torch
+        self.specs = list(specs)
+        self.samples = samples
+        self.streams = [torch.cuda.Stream() for _ in range(streams)]
+        self.tuning = tuning
+        self.rng = np.random.Generator(np.random.PCG64(seed))
+        order = sorted(range(len(self.specs)), key=lambda i: -self.specs[i][0])
+        self.waves = []
+        cur, used = [], 0
+        for i in order:
+            need = footprint(self.specs[i][0], samples)
+            if cur and used + need > budget_bytes:
+                self.waves.append(cur)
+                cur, used = [], 0
+            cur.append(i)
+            used += need
+        if cur:
+            self.waves.append(cur)
+        self.graphs = {}
+
+    def _tuning_for(self, n):
+        if self.tuning != 'auto':
+            return self.tuning
+        from .autotune import lookup
+        t = lookup(n, self.torch)
+        return None if t is None else {k: t[k] for k in ('low_bits', 'tile_bits', 'lane_bits')}
+
+    def _build(self, wave):
+        torch = self.torch
+        items = []
+        for i in wave:
+            n, seed = self.specs[i]
+            g, u = instance(n, seed)
+            s = GraphStateSampler(g, u, pipeline='slab', tuning=self._tuning_for(n))
+            uni = torch.as_tensor(self.rng.random(self.samples)).to(s.slab.device)
+            idx = torch.empty(self.samples, dtype=torch.int64, device=s.slab.device)
+            pr = torch.empty(self.samples, dtype=torch.float64, device=s.slab.device)
+            items.append(dict(i=i, s=s, uni=uni, idx=idx, p=pr))
+        return items
+
+    def _launch(self, item, stream):
+        s = item['s']
+        sl = s.slab
+        h = stream.cuda_stream
+        from .engine import ctypes_ref
+        call('rgs_slab_pipeline', ctypes_ref(s._desc), h)
+        call('rgs_mass_pyramid', ptr(sl.pyramid), sl.base_bits, h)
+        call('rgs_draw', ptr(sl.state), s.n, ptr(sl.pyramid), sl.mass_bits, ptr(item['uni']), self.samples,
+             ptr(item['idx']), ptr(item['p']), h)
+
+    def run_eager(self):
+        """One launch sequence per instance on the current stream, wave by wave."""
+        torch = self.torch
+        out = {}
+        for wave in self.waves:
+            items = self._build(wave)
+            cur = torch.cuda.current_stream()
+            for it in items:
+                self._launch(it, cur)
+            torch.cuda.synchronize()
+            for it in items:
+                out[it['i']] = it['idx'].cpu().numpy()
+            del items
+            torch.cuda.empty_cache()
+        return out
+
+    def run_graph(self, replays=1, keep=False):
+        """Capture each wave as one multi-stream graph and replay it."""
+        torch = self.torch
+        out = {}
+        for wi, wave in enumerate(self.waves):
+            items = self._build(wave)
+            main = torch.cuda.Stream()
+            graph = torch.cuda.CUDAGraph()
+            # warm the launch path once outside capture (one-time attribute setup)
+            for k, it in enumerate(items):
+                self._launch(it, self.streams[k % len(self.streams)])
+            torch.cuda.synchronize()
+            with torch.cuda.stream(main):
+                graph.capture_begin()
+                for st in self.streams:
+                    st.wait_stream(main)
+                for k, it in enumerate(items):
+                    self._launch(it, self.streams[k % len(self.streams)])
+                for st in self.streams:
+                    main.wait_stream(st)
+                graph.capture_end()
+            for _ in range(replays):
+                graph.replay()
+            torch.cuda.synchronize()
+            for it in items:
+                out[it['i']] = it['idx'].cpu().numpy()
+            if keep:
+                self.graphs[wi] = (graph, items)
+            else:
+                del graph, items
+                torch.cuda.empty_cache()
+        return out

@@ -1,0 +1,159 @@
+// FP64 mass pyramid, device draws and probability gathers on a complex64 state.
+#include "rgs_common.h"
+#include "rgs_amd.h"
+
+#include <algorithm>
+
+namespace rgs {
+
+// ------------------------------------------------------------------ pyramid
+// One workgroup reduces 2^k consecutive entries of level L into levels L+1..L+k.
+__global__ void __launch_bounds__(256) k_pyramid(double* pyr, uint64_t level_off, int level_bits, int k,
+                                                 uint64_t out_off_first) {
+  __shared__ double s[512];
+  const uint32_t tid = threadIdx.x;
+  const uint64_t group = blockIdx.x;
+  const uint32_t width = 1u << k;  // entries per group, <= 512
+  const double* in = pyr + level_off + (group << k);
+  for (uint32_t i = tid; i < width; i += 256) s[i] = in[i];
+  __syncthreads();
+  uint64_t out_off = out_off_first;
+  int bits = level_bits;
+  for (int l = 1; l <= k; ++l) {
+    uint32_t cnt = width >> l;
+    double val = 0.0;
+    if (tid < cnt) val = __dadd_rn(s[2 * tid], s[2 * tid + 1]);
+    __syncthreads();
+    if (tid < cnt) {
+      s[tid] = val;
+      pyr[out_off + (group << (k - l)) + tid] = val;
+    }
+    __syncthreads();
+    bits -= 1;
+    out_off += 1ull << bits;
+  }
+}
+
+// ------------------------------------------------------------------ draw
+// One wavefront per draw.  Lanes hold 4 consecutive amplitudes of the leaf
+// block (mass_bits <= 8 -> 256 amplitudes).  Level l of the pyramid (0 = base
+// with 2^B entries) starts at 2^(B+1) - 2^(B+1-l).
+__device__ __forceinline__ uint64_t level_off(int B, int l) {
+  return (1ull << (B + 1)) - (1ull << (B + 1 - l));
+}
+
+__global__ void __launch_bounds__(64) k_draw(const float2* state, int nbits, const double* pyr, int mass_bits,
+                                             const double* uniforms, int count, int64_t* out_index,
+                                             double* out_p) {
+  const int d = blockIdx.x;
+  if (d >= count) return;
+  const int lane = threadIdx.x;
+  const int B = nbits - mass_bits;
+  double t = uniforms[d] * pyr[level_off(B, B)];
+  uint64_t idx = 0;
+  for (int l = B - 1; l >= 0; --l) {
+    const double* lev = pyr + level_off(B, l);
+    double left = lev[2 * idx], right = lev[2 * idx + 1];
+    if (t >= left && right > 0.0) {
+      t = __dsub_rn(t, left);
+      idx = 2 * idx + 1;
+    } else {
+      idx = 2 * idx;
+    }
+  }
+  // Leaf block: recompute the pairwise tree with the same association.
+  const int bsize = 1 << mass_bits;
+  const uint64_t bbase = idx << mass_bits;
+  double p[4];
+#pragma unroll
+  for (int j = 0; j < 4; ++j) {
+    int l = lane * 4 + j;
+    p[j] = l < bsize ? prob64(state[bbase + l]) : 0.0;
+  }
+  double lv[7];
+  lv[0] = __dadd_rn(__dadd_rn(p[0], p[1]), __dadd_rn(p[2], p[3]));
+#pragma unroll
+  for (int b = 0; b < 6; ++b) lv[b + 1] = __dadd_rn(lv[b], __shfl_xor(lv[b], 1 << b));
+  // Descend over local bits mass_bits-1 .. 2 (lane bit b = bit - 2).
+  int node = 0;
+#pragma unroll
+  for (int b = 5; b >= 0; --b) {
+    if (b + 2 <= mass_bits - 1) {
+      int left_lane = node << (b + 1);
+      double left = __shfl(lv[b], left_lane);
+      double right = __shfl(lv[b], left_lane | (1 << b));
+      if (t >= left && right > 0.0) {
+        t = __dsub_rn(t, left);
+        node = 2 * node + 1;
+      } else {
+        node = 2 * node;
+      }
+    }
+  }
+  double q0 = __shfl(p[0], node), q1 = __shfl(p[1], node), q2 = __shfl(p[2], node), q3 = __shfl(p[3], node);
+  double a01 = __dadd_rn(q0, q1), a23 = __dadd_rn(q2, q3);
+  int sub;
+  double ps;
+  if (t >= a01 && a23 > 0.0) {
+    t = __dsub_rn(t, a01);
+    if (t >= q2 && q3 > 0.0) { sub = 3; ps = q3; } else { sub = 2; ps = q2; }
+  } else {
+    if (t >= q0 && q1 > 0.0) { sub = 1; ps = q1; } else { sub = 0; ps = q0; }
+  }
+  if (lane == 0) {
+    out_index[d] = (int64_t)(bbase + (uint64_t)(node * 4 + sub));
+    out_p[d] = ps;
+  }
+}
+
+__global__ void k_gather_prob(const float2* state, const int64_t* index, int count, double* out) {
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < count) out[i] = prob64(state[index[i]]);
+}
+
+}  // namespace rgs
+
+using namespace rgs;
+
+extern "C" int rgs_mass_pyramid(double* pyr, int base_bits, void* stream) {
+  if (!pyr || base_bits < 0 || base_bits > 40) return fail(kInvalid, "bad pyramid");
+  hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+  uint64_t off = 0;
+  int bits = base_bits;
+  while (bits > 0) {
+    int k = std::min(bits, 9);
+    uint64_t groups = 1ull << (bits - k);
+    uint64_t out_off = off + (1ull << bits);
+    hipLaunchKernelGGL(k_pyramid, dim3((unsigned)groups), dim3(256), 0, st, pyr, off, bits, k, out_off);
+    RGS_LAUNCH_CHECK("k_pyramid");
+    // advance past k levels
+    for (int l = 0; l < k; ++l) {
+      off += 1ull << bits;
+      bits -= 1;
+    }
+  }
+  return kOk;
+}
+
+extern "C" int rgs_draw(const void* state, int nbits, const double* pyr, int mass_bits, const double* uniforms,
+                        int count, int64_t* out_index, double* out_p, void* stream) {
+  if (!state || !pyr || !uniforms || !out_index || !out_p) return fail(kInvalid, "null pointer");
+  if (mass_bits < 2 || mass_bits > 8 || nbits - mass_bits < 0 || nbits - mass_bits > 47)
+    return fail(kInvalid, "draw needs 2 <= mass_bits <= 8");
+  if (count <= 0) return kOk;
+  hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+  hipLaunchKernelGGL(k_draw, dim3(count), dim3(64), 0, st, reinterpret_cast<const float2*>(state), nbits, pyr,
+                     mass_bits, uniforms, count, out_index, out_p);
+  RGS_LAUNCH_CHECK("k_draw");
+  return kOk;
+}
+
+extern "C" int rgs_gather_prob(const void* state, const int64_t* index, int count, double* out_p, void* stream) {
+  if (!state || !index || !out_p) return fail(kInvalid, "null pointer");
+  if (count <= 0) return kOk;
+  hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+  hipLaunchKernelGGL(k_gather_prob, dim3((count + 255) / 256), dim3(256), 0, st,
+                     reinterpret_cast<const float2*>(state), index, count, out_p);
+  RGS_LAUNCH_CHECK("k_gather_prob");
+  return kOk;
+}

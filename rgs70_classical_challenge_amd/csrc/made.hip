@@ -1,0 +1,261 @@
+// Kernels of the learned proposal: a masked autoregressive MLP (MADE) with
+// BF16 weights and activations, FP32 accumulation on the MFMA units, and the
+// elementwise pieces of training (loss, bias gradients, masked Adam) and of
+// autoregressive sampling.
+//
+// GEMM: C[M x N] = op(A)[M x K] * op(B)[K x N], op = identity or transpose of
+// a row-major BF16 matrix.  A 256-thread workgroup computes a 64 x 64 tile
+// with v_mfma_f32_16x16x16_bf16 (each wavefront a 32 x 32 quarter, 2 x 2
+// MFMA blocks), staging 64 x 32 slices of A and B through LDS.  Fragment
+// layout of the 16x16x16 instruction: lane l supplies A[l % 16][4 (l / 16) + i]
+// and B[4 (l / 16) + i][l % 16], i = 0..3, and receives
+// D[4 (l / 16) + i][l % 16].  Epilogues: bias, ReLU, a ReLU-derivative mask
+// from a saved activation, FP32 and/or BF16 stores, accumulate-into-C.
+#include "rgs_common.h"
+#include "rgs_amd.h"
+
+#include <algorithm>
+
+namespace rgs {
+
+typedef short bf16x4 __attribute__((ext_vector_type(4)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ float bf16_to_f32(uint16_t h) { return __uint_as_float((uint32_t)h << 16); }
+// Round to nearest even (inputs here are finite).
+__device__ __forceinline__ uint16_t f32_to_bf16(float f) {
+  uint32_t u = __float_as_uint(f);
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return (uint16_t)(u >> 16);
+}
+
+struct GemmArgs {
+  const uint16_t* A;
+  const uint16_t* B;
+  float* C;
+  uint16_t* Cb;
+  const float* bias;
+  const uint16_t* dmask;  // multiply by (dmask > 0) (ReLU derivative), or NULL
+  int M, N, K, lda, ldb, ldc, ldcb, ldm;
+  int ta, tb, relu, accumulate;
+  float alpha;
+};
+
+constexpr int TM = 64, TN = 64, TK = 32;
+
+__global__ void __launch_bounds__(256) k_gemm(GemmArgs g) {
+  // As[m][k], Bs[n][k]: k contiguous so a lane's four k values are adjacent.
+  __shared__ uint16_t As[TM][TK + 4];
+  __shared__ uint16_t Bs[TN][TK + 4];
+  const int tid = threadIdx.x;
+  const int wave = tid >> 6, lane = tid & 63;
+  const int m0 = blockIdx.y * TM, n0 = blockIdx.x * TN;
+  const int wm = (wave >> 1) * 32, wn = (wave & 1) * 32;
+  f32x4 acc[2][2];
+  for (int i = 0; i < 2; ++i)
+    for (int j = 0; j < 2; ++j) acc[i][j] = (f32x4){0.f, 0.f, 0.f, 0.f};
+
+  for (int k0 = 0; k0 < g.K; k0 += TK) {
+    // stage A: 64 x 32 elements, 8 per thread
+    for (int e = tid; e < TM * TK; e += 256) {
+      int mm = e / TK, kk = e % TK;
+      int gm = m0 + mm, gk = k0 + kk;
+      uint16_t v = 0;
+      if (gm < g.M && gk < g.K) v = g.ta ? g.A[(size_t)gk * g.lda + gm] : g.A[(size_t)gm * g.lda + gk];
+      As[mm][kk] = v;
+    }
+    for (int e = tid; e < TN * TK; e += 256) {
+      int nn = e / TK, kk = e % TK;
+      int gn = n0 + nn, gk = k0 + kk;
+      uint16_t v = 0;
+      if (gn < g.N && gk < g.K) v = g.tb ? g.B[(size_t)gn * g.ldb + gk] : g.B[(size_t)gk * g.ldb + gn];
+      Bs[nn][kk] = v;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int ks = 0; ks < TK; ks += 16) {
+      const int kq = ks + 4 * (lane >> 4);
+#pragma unroll
+      for (int i = 0; i < 2; ++i) {
+        bf16x4 a;
+        const int r = wm + 16 * i + (lane & 15);
+        a[0] = (short)As[r][kq + 0];
+        a[1] = (short)As[r][kq + 1];
+        a[2] = (short)As[r][kq + 2];
+        a[3] = (short)As[r][kq + 3];
+#pragma unroll
+        for (int j = 0; j < 2; ++j) {
+          bf16x4 b;
+          const int cidx = wn + 16 * j + (lane & 15);
+          b[0] = (short)Bs[cidx][kq + 0];
+          b[1] = (short)Bs[cidx][kq + 1];
+          b[2] = (short)Bs[cidx][kq + 2];
+          b[3] = (short)Bs[cidx][kq + 3];
+          acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc[i][j], 0, 0, 0);
+        }
+      }
+    }
+    __syncthreads();
+  }
+  // epilogue
+#pragma unroll
+  for (int i = 0; i < 2; ++i)
+#pragma unroll
+    for (int j = 0; j < 2; ++j)
+#pragma unroll
+      for (int t = 0; t < 4; ++t) {
+        const int row = m0 + wm + 16 * i + 4 * (lane >> 4) + t;
+        const int col = n0 + wn + 16 * j + (lane & 15);
+        if (row < g.M && col < g.N) {
+          float v = g.alpha * acc[i][j][t];
+          if (g.bias) v += g.bias[col];
+          if (g.relu) v = fmaxf(v, 0.f);
+          if (g.dmask && !(bf16_to_f32(g.dmask[(size_t)row * g.ldm + col]) > 0.f)) v = 0.f;
+          if (g.C) {
+            float* cp = g.C + (size_t)row * g.ldc + col;
+            *cp = g.accumulate ? *cp + v : v;
+          }
+          if (g.Cb) g.Cb[(size_t)row * g.ldcb + col] = f32_to_bf16(v);
+        }
+      }
+}
+
+// Loss and its gradient for sigmoid outputs: per row, NLL = sum_o softplus(l)
+// - x l (binary cross-entropy in logits), dlogit = (sigmoid(l) - x) * scale.
+__global__ void k_bce(const float* logits, const uint16_t* x, int B, int n, float scale, float* nll,
+                      float* dlog, uint16_t* dlog_b) {
+  int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= B) return;
+  float acc = 0.f;
+  for (int o = 0; o < n; ++o) {
+    float l = logits[(size_t)r * n + o];
+    float xv = bf16_to_f32(x[(size_t)r * n + o]);
+    float sp = l > 0.f ? l + log1pf(expf(-l)) : log1pf(expf(l));
+    acc += sp - xv * l;
+    float s = 1.f / (1.f + expf(-l));
+    float d = (s - xv) * scale;
+    if (dlog) dlog[(size_t)r * n + o] = d;
+    if (dlog_b) dlog_b[(size_t)r * n + o] = f32_to_bf16(d);
+  }
+  if (nll) nll[r] = acc;
+}
+
+// out[c] (+)= sum_r in[r][c]  (bias gradients)
+__global__ void k_colsum(const float* in, int rows, int cols, int ld, float* out, int accumulate) {
+  int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= cols) return;
+  float s = 0.f;
+  for (int r = 0; r < rows; ++r) s += in[(size_t)r * ld + c];
+  out[c] = accumulate ? out[c] + s : s;
+}
+
+__global__ void k_colsum_bf16(const uint16_t* in, int rows, int cols, int ld, float* out) {
+  int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= cols) return;
+  float s = 0.f;
+  for (int r = 0; r < rows; ++r) s += bf16_to_f32(in[(size_t)r * ld + c]);
+  out[c] = s;
+}
+
+__global__ void k_step_counter(int* t) { t[0] += 1; }
+
+// Masked Adam on a flat parameter vector; writes the BF16 working copy.
+__global__ void k_adam(float* w, const float* grad, const float* mask, float* m, float* v, uint16_t* wb,
+                       long long count, const int* t, float lr, float b1, float b2, float eps) {
+  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= count) return;
+  const float step = (float)t[0];
+  float g = grad[i] * mask[i];
+  float mi = b1 * m[i] + (1.f - b1) * g;
+  float vi = b2 * v[i] + (1.f - b2) * g * g;
+  m[i] = mi;
+  v[i] = vi;
+  float mh = mi / (1.f - powf(b1, step));
+  float vh = vi / (1.f - powf(b2, step));
+  float wi = (w[i] - lr * mh / (sqrtf(vh) + eps)) * mask[i];
+  w[i] = wi;
+  wb[i] = f32_to_bf16(wi);
+}
+
+// Autoregressive sampling step: x[r][o] = (u[r][o] * (1 + e^{-l}) < 1), i.e.
+// u < sigmoid(l), for the column o just computed.
+__global__ void k_sample_col(const float* logits, const double* u, int B, int n, int o, uint16_t* x) {
+  int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= B) return;
+  float l = logits[(size_t)r * n + o];
+  double p = 1.0 / (1.0 + exp(-(double)l));
+  x[(size_t)r * n + o] = u[(size_t)r * n + o] < p ? (uint16_t)0x3f80 : (uint16_t)0;  // bf16 1.0 / 0.0
+}
+
+__global__ void k_f32_to_bf16(const float* in, uint16_t* out, long long count) {
+  long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < count) out[i] = f32_to_bf16(in[i]);
+}
+
+}  // namespace rgs
+
+using namespace rgs;
+
+extern "C" int rgs_gemm_bf16(int ta, int tb, int M, int N, int K, const uint16_t* A, int lda, const uint16_t* B,
+                             int ldb, float alpha, float* C, int ldc, uint16_t* Cb, int ldcb, const float* bias,
+                             int relu, const uint16_t* dmask, int ldm, int accumulate, void* stream) {
+  if (!A || !B || (!C && !Cb) || M <= 0 || N <= 0 || K <= 0) return fail(kInvalid, "gemm: bad arguments");
+  GemmArgs g{A, B, C, Cb, bias, dmask, M, N, K, lda, ldb, ldc, ldcb, ldm, ta, tb, relu, accumulate, alpha};
+  dim3 grid((N + TN - 1) / TN, (M + TM - 1) / TM);
+  hipLaunchKernelGGL(k_gemm, grid, dim3(256), 0, reinterpret_cast<hipStream_t>(stream), g);
+  RGS_LAUNCH_CHECK("k_gemm");
+  return kOk;
+}
+
+extern "C" int rgs_made_bce(const float* logits, const uint16_t* x, int B, int n, float scale, float* nll,
+                            float* dlogits, uint16_t* dlogits_bf16, void* stream) {
+  if (!logits || !x || B <= 0 || n <= 0) return fail(kInvalid, "bce: bad arguments");
+  hipLaunchKernelGGL(k_bce, dim3((B + 255) / 256), dim3(256), 0, reinterpret_cast<hipStream_t>(stream), logits, x,
+                     B, n, scale, nll, dlogits, dlogits_bf16);
+  RGS_LAUNCH_CHECK("k_bce");
+  return kOk;
+}
+
+extern "C" int rgs_colsum(const float* in, int rows, int cols, int ld, float* out, int accumulate, void* stream) {
+  if (!in || !out || rows <= 0 || cols <= 0) return fail(kInvalid, "colsum: bad arguments");
+  hipLaunchKernelGGL(k_colsum, dim3((cols + 255) / 256), dim3(256), 0, reinterpret_cast<hipStream_t>(stream), in,
+                     rows, cols, ld, out, accumulate);
+  RGS_LAUNCH_CHECK("k_colsum");
+  return kOk;
+}
+
+extern "C" int rgs_colsum_bf16(const uint16_t* in, int rows, int cols, int ld, float* out, void* stream) {
+  if (!in || !out || rows <= 0 || cols <= 0) return fail(kInvalid, "colsum: bad arguments");
+  hipLaunchKernelGGL(k_colsum_bf16, dim3((cols + 255) / 256), dim3(256), 0, reinterpret_cast<hipStream_t>(stream),
+                     in, rows, cols, ld, out);
+  RGS_LAUNCH_CHECK("k_colsum_bf16");
+  return kOk;
+}
+
+extern "C" int rgs_adam(float* w, const float* grad, const float* mask, float* m, float* v, uint16_t* wb,
+                        long long count, int* t_dev, float lr, float b1, float b2, float eps, void* stream) {
+  if (!w || !grad || !mask || !m || !v || !wb || !t_dev || count <= 0) return fail(kInvalid, "adam: bad arguments");
+  hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+  hipLaunchKernelGGL(k_step_counter, dim3(1), dim3(1), 0, st, t_dev);
+  RGS_LAUNCH_CHECK("k_step_counter");
+  hipLaunchKernelGGL(k_adam, dim3((unsigned)((count + 255) / 256)), dim3(256), 0, st, w, grad, mask, m, v, wb, count,
+                     t_dev, lr, b1, b2, eps);
+  RGS_LAUNCH_CHECK("k_adam");
+  return kOk;
+}
+
+extern "C" int rgs_sample_col(const float* logits, const double* u, int B, int n, int o, uint16_t* x, void* stream) {
+  if (!logits || !u || !x || o < 0 || o >= n) return fail(kInvalid, "sample_col: bad arguments");
+  hipLaunchKernelGGL(k_sample_col, dim3((B + 255) / 256), dim3(256), 0, reinterpret_cast<hipStream_t>(stream),
+                     logits, u, B, n, o, x);
+  RGS_LAUNCH_CHECK("k_sample_col");
+  return kOk;
+}
+
+extern "C" int rgs_f32_to_bf16(const float* in, uint16_t* out, long long count, void* stream) {
+  if (!in || !out || count <= 0) return fail(kInvalid, "f32_to_bf16: bad arguments");
+  hipLaunchKernelGGL(k_f32_to_bf16, dim3((unsigned)((count + 255) / 256)), dim3(256), 0,
+                     reinterpret_cast<hipStream_t>(stream), in, out, count);
+  RGS_LAUNCH_CHECK("k_f32_to_bf16");
+  return kOk;
+}

@@ -199,6 +199,69 @@ extern "C" int rgs_cov_luts(const uint8_t* g, int nv, const int32_t* A, const in
   return kOk;
 }
 
+// u = D1 R D2 with D1 = diag(e^{i a0}, e^{i a1}), R = [[c, -s], [s, c]] real,
+// D2 = diag(1, e^{i b}): |u00| = c, |u10| = s, a0 = arg u00, a1 = arg u10,
+// b = arg u11 - a1 (= arg(-u01) - a0 by unitarity).  Degenerate columns
+// (c = 0 or s = 0) put the free phase into D1.
+extern "C" int rgs_gate_zyz(const double* u, int n, int gate_limit, float* rot, float* dphase_lut, double* d1) {
+  if (!u || n < 1 || n > 63) return fail(kInvalid, "gate_zyz: need 1 <= n <= 63");
+  using cd = std::complex<double>;
+  std::vector<double> b(n, 0.0);
+  for (int q = 0; q < n; ++q) {
+    const double* e = u + 8 * q;
+    cd u00(e[0], e[1]), u01(e[2], e[3]), u10(e[4], e[5]), u11(e[6], e[7]);
+    double c = std::abs(u00), s = std::abs(u10);
+    double nrm = std::hypot(c, s);
+    if (std::abs(nrm - 1.0) > 1e-9 || std::abs(std::abs(u01) - s) > 1e-9 || std::abs(std::abs(u11) - c) > 1e-9)
+      return fail(kInvalid, "gate_zyz: matrix is not unitary");
+    c /= nrm;
+    s /= nrm;
+    double a0, a1, bb;
+    if (s < 1e-300) {           // diagonal
+      a0 = std::arg(u00);
+      a1 = std::arg(u11);
+      bb = 0.0;
+      c = 1.0;
+      s = 0.0;
+    } else if (c < 1e-300) {    // anti-diagonal: u01 = -e^{i(a0 + b)}, u10 = e^{i a1}
+      a0 = 0.0;
+      a1 = std::arg(u10);
+      bb = std::arg(-u01);
+      c = 0.0;
+      s = 1.0;
+    } else {
+      a0 = std::arg(u00);
+      a1 = std::arg(u10);
+      bb = std::arg(u11) - a1;
+    }
+    if (rot) {
+      rot[2 * q] = (float)c;
+      rot[2 * q + 1] = (float)s;
+    }
+    if (d1) {
+      d1[4 * q + 0] = std::cos(a0);
+      d1[4 * q + 1] = std::sin(a0);
+      d1[4 * q + 2] = std::cos(a1);
+      d1[4 * q + 3] = std::sin(a1);
+    }
+    b[q] = q < gate_limit ? bb : 0.0;
+  }
+  if (dphase_lut) {
+    const int chunks = (n + 7) / 8;
+    for (int ch = 0; ch < chunks; ++ch)
+      for (int code = 0; code < 256; ++code) {
+        double ang = 0.0;
+        for (int t = 0; t < 8; ++t) {
+          int q = 8 * ch + t;
+          if (q < n && ((code >> t) & 1)) ang += b[q];
+        }
+        dphase_lut[2 * (ch * 256 + code)] = (float)std::cos(ang);
+        dphase_lut[2 * (ch * 256 + code) + 1] = (float)std::sin(ang);
+      }
+  }
+  return kOk;
+}
+
 extern "C" int rgs_cov_weight_lut(const double* u, int nv, const int32_t* B, const uint8_t* outB, int m,
                                   double scale, float* wlut) {
   if (!u || !B || !outB || !wlut || m < 1 || m > 63) return fail(kInvalid, "cov_weight_lut: need 1 <= m <= 63");

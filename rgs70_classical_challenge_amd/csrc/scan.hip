@@ -110,6 +110,106 @@ __global__ void __launch_bounds__(64) k_scan_batch(const uint64_t* __restrict__ 
   }
 }
 
+// Ping-pong variant: input and output tables are separate LDS buffers, so no
+// outputs are held in registers; the power-of-two scale of step j is applied
+// while step j+1 reads its input.  Used where K outputs per lane would not fit
+// in registers (complex128 tables at rmax 12).
+template <typename S>
+__global__ void __launch_bounds__(64) k_scan_batch_pp(const uint64_t* __restrict__ plan, const double2* __restrict__ u,
+                                                      const uint64_t* __restrict__ bits, int count,
+                                                      double2* __restrict__ out_val, int32_t* __restrict__ out_exp,
+                                                      int rmax) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  S* buf0 = reinterpret_cast<S*>(smem);
+  S* buf1 = buf0 + ((size_t)1 << rmax);
+  const int lane = threadIdx.x;
+  const int n = (int)plan[0];
+  for (int pidx = blockIdx.x; pidx < count; pidx += gridDim.x) {
+    const uint64_t x0 = bits[2 * pidx], x1 = bits[2 * pidx + 1];
+    S* Ein = buf0;
+    S* Eout = buf1;
+    if (lane == 0) Ein[0] = make_s<S>(1.0, 0.0);
+    __syncthreads();
+    int esum = 0;
+    double inscale = 1.0;  // 2^-e of the previous step, applied on read
+    for (int j = 0; j < n; ++j) {
+      const uint64_t* st = plan + plan[2 + j];
+      const uint64_t hdr = st[0];
+      const int v = (int)(hdr & 255u), r_in = (int)((hdr >> 8) & 255u), r_out = (int)((hdr >> 16) & 255u);
+      const int nker = (int)((hdr >> 24) & 255u);
+      const uint64_t beta = st[1];
+      const int xv = (int)((v < 64 ? (x0 >> v) : (x1 >> (v - 64))) & 1u);
+      const double2 f0 = u[(v * 2 + xv) * 2 + 0], f1 = u[(v * 2 + xv) * 2 + 1];
+      const uint64_t lo = st[2 + lane];
+      const uint64_t* hi = st + 66;
+      const uint64_t* combos = st + 66 + (1ull << (r_out > 6 ? r_out - 6 : 0));
+      const int ncomb = 1 << nker;
+      const uint64_t mask = (1ull << r_in) - 1ull;
+      const int nk = r_out > 6 ? 1 << (r_out - 6) : 1;
+      const bool lane_ok = r_out >= 6 || lane < (1 << r_out);
+      double mx = 0.0;
+      for (int k = 0; k < nk; ++k) {
+        if (!lane_ok) break;
+        const uint64_t p0 = lo ^ hi[k];
+        double ar = 0.0, ai = 0.0;
+        for (int t = 0; t < ncomb; ++t) {
+          const uint64_t p = p0 ^ combos[t];
+          const uint64_t c = p & mask;
+          const uint32_t y = (uint32_t)(p >> r_in) & 1u;
+          const uint32_t neg = y & ((uint32_t)__popcll(c & beta) & 1u);
+          const double2 w = y ? f1 : f0;
+          const double2 e = load_c(Ein[c]);
+          const double pr = e.x * w.x - e.y * w.y;
+          const double pi = e.x * w.y + e.y * w.x;
+          if (neg) {
+            ar -= pr;
+            ai -= pi;
+          } else {
+            ar += pr;
+            ai += pi;
+          }
+        }
+        ar *= inscale;
+        ai *= inscale;
+        mx = fmax(mx, ar * ar + ai * ai);
+        Eout[lane + 64 * k] = make_s<S>(ar, ai);
+      }
+#pragma unroll
+      for (int off = 32; off > 0; off >>= 1) mx = fmax(mx, __shfl_xor(mx, off));
+      int e = 0;
+      if (mx > 0.0) frexp(sqrt(mx), &e);
+      inscale = ldexp(1.0, -e);
+      esum += e;
+      S* tmp = Ein;
+      Ein = Eout;
+      Eout = tmp;
+      __syncthreads();
+    }
+    // Ein holds the last table unscaled by its own exponent, which esum counts.
+    if (lane == 0) {
+      double2 r = load_c(Ein[0]);
+      out_val[pidx] = make_double2(r.x * inscale, r.y * inscale);
+      out_exp[pidx] = esum;
+    }
+    __syncthreads();
+  }
+}
+
+template <typename S>
+static hipError_t launch_scan_pp(const uint64_t* plan, const double2* u, const uint64_t* bits, int count,
+                                 double2* val, int32_t* ex, int rmax, hipStream_t st) {
+  size_t shmem = ((size_t)2 << rmax) * sizeof(S);
+  int grid = std::min(count, 256 * 8);
+  static bool attr = false;
+  if (!attr) {
+    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_scan_batch_pp<S>),
+                              hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+    attr = true;
+  }
+  hipLaunchKernelGGL((k_scan_batch_pp<S>), dim3(grid), dim3(64), shmem, st, plan, u, bits, count, val, ex, rmax);
+  return hipGetLastError();
+}
+
 template <typename S, int K>
 static hipError_t launch_scan(const uint64_t* plan, const double2* u, const uint64_t* bits, int count,
                               double2* val, int32_t* ex, int rmax, hipStream_t st) {
@@ -142,13 +242,15 @@ extern "C" int rgs_scan_batch(const uint64_t* plan_dev, int rmax, const double* 
     else if (rmax <= 8) e = launch_scan<float2, 4>(plan_dev, u, bits_dev, count, val, out_exp, 8, st);
     else if (rmax <= 10) e = launch_scan<float2, 16>(plan_dev, u, bits_dev, count, val, out_exp, 10, st);
     else if (rmax <= 12) e = launch_scan<float2, 64>(plan_dev, u, bits_dev, count, val, out_exp, 12, st);
-    else return fail(kInvalid, "scan_batch: rmax above 12 needs the large-table scan");
+    else if (rmax <= 13) e = launch_scan_pp<float2>(plan_dev, u, bits_dev, count, val, out_exp, 13, st);
+    else return fail(kInvalid, "scan_batch: complex64 batch limited to rmax <= 13");
   } else if (precision == 128) {
     if (rmax <= 6) e = launch_scan<double2, 1>(plan_dev, u, bits_dev, count, val, out_exp, 6, st);
     else if (rmax <= 8) e = launch_scan<double2, 4>(plan_dev, u, bits_dev, count, val, out_exp, 8, st);
     else if (rmax <= 10) e = launch_scan<double2, 16>(plan_dev, u, bits_dev, count, val, out_exp, 10, st);
     else if (rmax <= 11) e = launch_scan<double2, 32>(plan_dev, u, bits_dev, count, val, out_exp, 11, st);
-    else return fail(kInvalid, "scan_batch: complex128 batch limited to rmax <= 11");
+    else if (rmax <= 12) e = launch_scan_pp<double2>(plan_dev, u, bits_dev, count, val, out_exp, 12, st);
+    else return fail(kInvalid, "scan_batch: complex128 batch limited to rmax <= 12");
   } else {
     return fail(kInvalid, "precision must be 64 (complex64 tables) or 128 (complex128)");
   }

@@ -74,6 +74,7 @@ class _Slab:
 
     def desc(self, init_kind, init_scale=0.0, init_lut=None, init_wlut=None, walsh=False, phase_lut=None,
              gates=None, gate_limit=None, mass=True):
+        """gates: a Gates object (rotations + diagonal phase) or None."""
         d = _lib.SlabDesc()
         d.state = ptr(self.state)
         d.nbits = self.nbits
@@ -83,13 +84,16 @@ class _Slab:
         d.init_wlut = ptr(init_wlut)
         d.do_walsh = int(walsh)
         d.phase_lut = ptr(phase_lut)
-        d.gates = ptr(gates)
+        d.dphase_lut = ptr(gates.dphase) if gates is not None else None
+        d.rot = ptr(gates.rot) if gates is not None else None
         d.gate_limit = self.nbits if gate_limit is None else gate_limit
         d.mass = ptr(self.pyramid) if mass else None
         d.mass_bits = self.mass_bits
         d.low_bits = self.tuning['low_bits']
         d.tile_bits = self.tuning['tile_bits']
         d.lane_bits = self.tuning['lane_bits']
+        # the descriptor holds raw pointers: keep their owners alive with it
+        d._owners = (self.state, self.pyramid, init_lut, init_wlut, phase_lut, gates)
         return d
 
     def run(self, d, mass=True):
@@ -142,10 +146,30 @@ def ctypes_ref(d):
     return ctypes.byref(d)
 
 
-def _gates_array(mats):
-    """complex64 [n][2][2] -> float32 [n*8] interleaved (u00, u01, u10, u11)."""
-    m = np.ascontiguousarray(np.asarray(mats, np.complex128).reshape(-1, 4).astype(np.complex64))
-    return m.view(np.float32).reshape(-1)
+class Gates:
+    """Per-qubit unitaries u_q = D1 R D2 (rgs_gate_zyz): device rotations
+    (cos, sin), the device chunk table of the D2 layer (qubits < gate_limit)
+    and the host D1 phases that read-back applies."""
+
+    def __init__(self, slab, mats, gate_limit=None):
+        u = np.ascontiguousarray(np.asarray(mats, np.complex128))
+        n = len(u)
+        self.gate_limit = n if gate_limit is None else gate_limit
+        rot = np.zeros(2 * n, np.float32)
+        dph = np.zeros(((n + 7) // 8) * 256 * 2, np.float32)
+        d1 = np.zeros(4 * n, np.float64)
+        call('rgs_gate_zyz', ptr(u), n, self.gate_limit, ptr(rot), ptr(dph), ptr(d1))
+        self.rot = rot                      # host: read when passes are enqueued
+        self.dphase = slab.to_device(dph)
+        self.d1 = d1.view(np.complex128).reshape(n, 2)
+
+    def apply_d1(self, v):
+        """In place: multiply amplitude x by prod_{q < gate_limit} D1_q[x_q]."""
+        ids = np.arange(len(v), dtype=np.uint64)
+        for q in range(self.gate_limit):
+            bit = ((ids >> np.uint64(q)) & np.uint64(1)).astype(np.int64)
+            v *= self.d1[q][bit]
+        return v
 
 
 def graph_phase_lut(g):
@@ -190,15 +214,12 @@ class GraphStateSampler:
         self.slab = _Slab(n, device, mass_bits, tuning)
         sl = self.slab
         self._lut = sl.to_device(graph_phase_lut(g))
-        if pipeline == 'slab':
-            mats = u @ HADAMARD
-        else:
-            mats = u
-        self._gates = sl.to_device(_gates_array(mats))
+        self._gates = Gates(sl, u @ HADAMARD if pipeline == 'slab' else u)
         if pipeline == 'slab':
             self._desc = sl.desc(_lib.INIT_GRAPH, 2.0 ** (-n), init_lut=self._lut, walsh=True, gates=self._gates)
         elif pipeline == 'literal':
-            self._desc = sl.desc(_lib.INIT_DELTA, 1.0, walsh=True, phase_lut=self._lut, gates=self._gates)
+            # H^n = WHT / 2^{n/2}: the delta carries the normalisation
+            self._desc = sl.desc(_lib.INIT_DELTA, 2.0 ** (-n / 2), walsh=True, phase_lut=self._lut, gates=self._gates)
         else:
             self._desc = sl.desc(_lib.INIT_GRAPH, 2.0 ** (-n / 2), init_lut=self._lut, gates=self._gates)
         self.passes, self.plan_text = sl.describe(self._desc)
@@ -244,8 +265,14 @@ class GraphStateSampler:
             raise ValueError('index outside the state')
         return float(self.slab.gather_prob([index])[0])
 
+    def device_amplitudes(self):
+        """The device complex64 values before the D1 phases, as complex128:
+        the numbers the masses, the pyramid and the draws are computed from."""
+        return self.slab.state.cpu().numpy().astype(np.complex128)
+
     def host(self):
-        return self.slab.state.cpu().numpy()
+        """The state as complex128 (device complex64 values times the D1 phases)."""
+        return self._gates.apply_d1(self.device_amplitudes())
 
     def sync(self):
         self.slab.sync()
@@ -295,7 +322,7 @@ class ConditionalSampler:
         self._aphase = sl.to_device(aphase)
         self._Ba = Ba
         self._u = np.ascontiguousarray(u)
-        self._gates = sl.to_device(_gates_array(u[A]))
+        self._gates = Gates(sl, u[A], m - 1 if defer_top else m)
         self._wlut_host = np.zeros(chunks * 256 * 2, np.float32)
         self._wlut = sl.to_device(self._wlut_host)
         self.deferred_global_unitary = _check_unitary(u[A[-1]]) if defer_top else None
@@ -376,7 +403,7 @@ class ConditionalSampler:
         return float(self.slab.gather_prob([index])[0])
 
     def host(self):
-        raw = self.slab.state.cpu().numpy().astype(np.complex128)
+        raw = self._gates.apply_d1(self.slab.state.cpu().numpy().astype(np.complex128))
         if self.defer_top:
             blk = raw.reshape(2, -1)
             a, b = blk[0].copy(), blk[1].copy()

@@ -1,0 +1,233 @@
+"""Learned proposal: a MADE network trained and sampled with this build's kernels.
+
+Architecture (BF16 weights and activations, FP32 accumulation, sigmoid
+conditionals): n binary inputs -> 512 -> 512 -> n logits, masked so that
+logit o depends only on inputs before o in qubit order (MADE, Germain et al.,
+arXiv:1502.03509).  Training is Adam on the mean binary cross-entropy of
+bitstrings drawn by the exact sampler; every GEMM, the loss, the bias
+gradients and the masked Adam update (FP32 master weights, BF16 working
+copy) run in librgs_amd.  Sampling is autoregressive (n forward passes) with
+host-drawn uniforms, and returns log q(x) for the acceptance test.
+No reference counterpart: the reference proposes uniformly
+(work/rgs_attack/rejection_sampler.py:21).
+"""
+import ctypes
+
+import numpy as np
+
+from . import _lib
+from ._lib import call, ptr
+
+_V = ctypes.c_void_p
+_I = ctypes.c_int
+_F = ctypes.c_float
+_lib.register('rgs_gemm_bf16', _I, [_I, _I, _I, _I, _I, _V, _I, _V, _I, _F, _V, _I, _V, _I, _V, _I, _V, _I, _I, _V])
+_lib.register('rgs_made_bce', _I, [_V, _V, _I, _I, _F, _V, _V, _V, _V])
+_lib.register('rgs_colsum', _I, [_V, _I, _I, _I, _V, _I, _V])
+_lib.register('rgs_colsum_bf16', _I, [_V, _I, _I, _I, _V, _V])
+_lib.register('rgs_adam', _I, [_V, _V, _V, _V, _V, _V, ctypes.c_longlong, _V, _F, _F, _F, _F, _V])
+_lib.register('rgs_sample_col', _I, [_V, _V, _I, _I, _I, _V, _V])
+_lib.register('rgs_f32_to_bf16', _I, [_V, _V, ctypes.c_longlong, _V])
+
+
+def made_masks(n, hidden, seed=0):
+    """Degrees: inputs 1..n; hidden degrees cycle over 1..n-1 (seeded shuffle).
+    Returns float masks [H,n], [H,H], [n,H]."""
+    rng = np.random.default_rng(seed)
+    deg_in = np.arange(1, n + 1)
+    d1 = rng.permutation(np.arange(hidden) % max(1, n - 1) + 1)
+    d2 = rng.permutation(np.arange(hidden) % max(1, n - 1) + 1)
+    m1 = (d1[:, None] >= deg_in[None, :]).astype(np.float32)
+    m2 = (d2[:, None] >= d1[None, :]).astype(np.float32)
+    m3 = (deg_in[:, None] > d2[None, :]).astype(np.float32)
+    return m1, m2, m3
+
+
+def bits_to_bf16(bits):
+    """uint8 0/1 -> bfloat16 bit patterns (0x0000 / 0x3f80) as int16."""
+    b = np.asarray(bits, np.uint8)
+    return (b.astype(np.uint16) * np.uint16(0x3F80)).view(np.int16)
+
+
+class MADE:
+    def __init__(self, n, hidden=512, batch=512, device=0, seed=0, lr=1e-3):
+        import torch
+        if not torch.cuda.is_available():
+            raise RuntimeError('no HIP device visible')
+        _lib.lib()
+        self.torch = torch
+        self.n, self.H, self.B = n, hidden, batch
+        self.lr = lr
+        self.dev = torch.device('cuda', device)
+        self.stream = torch.cuda.current_stream(self.dev)
+        n_, H = n, hidden
+        self.shapes = [(H, n_), (H,), (H, H), (H,), (n_, H), (n_,)]
+        sizes = [int(np.prod(s)) for s in self.shapes]
+        self.offsets = np.concatenate([[0], np.cumsum(sizes)]).astype(int)
+        P = int(self.offsets[-1])
+        self.P = P
+        rng = np.random.default_rng(seed)
+        m1, m2, m3 = made_masks(n, hidden, seed)
+        self.masks_np = (m1, m2, m3)
+        w = np.zeros(P, np.float32)
+        mask = np.ones(P, np.float32)
+        for k, (shape, m) in enumerate(zip(self.shapes[0::2], (m1, m2, m3))):
+            fan_in = shape[1]
+            lo = self.offsets[2 * k]
+            W = rng.uniform(-1, 1, size=shape).astype(np.float32) / np.sqrt(fan_in)
+            w[lo:lo + W.size] = (W * m).reshape(-1)
+            mask[lo:lo + W.size] = m.reshape(-1)
+        f32, bf = torch.float32, torch.int16
+        self.w = torch.as_tensor(w).to(self.dev)
+        self.mask = torch.as_tensor(mask).to(self.dev)
+        self.wb = torch.empty(P, dtype=bf, device=self.dev)
+        call('rgs_f32_to_bf16', ptr(self.w), ptr(self.wb), P, self.stream.cuda_stream)
+        self.g = torch.zeros(P, dtype=f32, device=self.dev)
+        self.m = torch.zeros(P, dtype=f32, device=self.dev)
+        self.v = torch.zeros(P, dtype=f32, device=self.dev)
+        self.t = torch.zeros(1, dtype=torch.int32, device=self.dev)
+        self._buffers(batch)
+
+    def _buffers(self, B):
+        torch = self.torch
+        n, H = self.n, self.H
+        bf, f32 = torch.int16, torch.float32
+        self.bufB = B
+        self.x = torch.zeros((B, n), dtype=bf, device=self.dev)
+        self.h1 = torch.empty((B, H), dtype=bf, device=self.dev)
+        self.h2 = torch.empty((B, H), dtype=bf, device=self.dev)
+        self.logits = torch.empty((B, n), dtype=f32, device=self.dev)
+        self.nll = torch.empty(B, dtype=f32, device=self.dev)
+        self.dl = torch.empty((B, n), dtype=f32, device=self.dev)
+        self.dlb = torch.empty((B, n), dtype=bf, device=self.dev)
+        self.da2 = torch.empty((B, H), dtype=f32, device=self.dev)
+        self.da2b = torch.empty((B, H), dtype=bf, device=self.dev)
+        self.da1 = torch.empty((B, H), dtype=f32, device=self.dev)
+        self.da1b = torch.empty((B, H), dtype=bf, device=self.dev)
+
+    # views into the flat vectors (element offsets)
+    def _p(self, tensor, k):
+        return ptr(tensor) + int(self.offsets[k]) * tensor.element_size()
+
+    def _gemm(self, ta, tb, M, N, K, A, lda, Bm, ldb, C=None, ldc=0, Cb=None, ldcb=0, bias=None, relu=0,
+              dmask=None, ldm=0, acc=0, alpha=1.0):
+        call('rgs_gemm_bf16', ta, tb, M, N, K, A, lda, Bm, ldb, alpha, C, ldc, Cb, ldcb, bias, relu, dmask, ldm,
+             acc, self.stream.cuda_stream)
+
+    def forward(self, B=None):
+        """Forward on self.x[:B]: h1, h2, logits."""
+        B = B or self.bufB
+        n, H = self.n, self.H
+        wb = self.wb
+        self._gemm(0, 1, B, H, n, ptr(self.x), n, self._p(wb, 0), n, Cb=ptr(self.h1), ldcb=H,
+                   bias=self._p(self.w, 1), relu=1)
+        self._gemm(0, 1, B, H, H, ptr(self.h1), H, self._p(wb, 2), H, Cb=ptr(self.h2), ldcb=H,
+                   bias=self._p(self.w, 3), relu=1)
+        self._gemm(0, 1, B, n, H, ptr(self.h2), H, self._p(wb, 4), H, C=ptr(self.logits), ldc=n,
+                   bias=self._p(self.w, 5))
+
+    def loss(self, B=None, grad=True):
+        B = B or self.bufB
+        call('rgs_made_bce', ptr(self.logits), ptr(self.x), B, self.n, 1.0 / B, ptr(self.nll),
+             ptr(self.dl) if grad else None, ptr(self.dlb) if grad else None, self.stream.cuda_stream)
+
+    def backward(self, B=None):
+        B = B or self.bufB
+        n, H = self.n, self.H
+        g, wb, s = self.g, self.wb, self.stream.cuda_stream
+        self._gemm(1, 0, n, H, B, ptr(self.dlb), n, ptr(self.h2), H, C=self._p(g, 4), ldc=H)
+        call('rgs_colsum', ptr(self.dl), B, n, n, self._p(g, 5), 0, s)
+        self._gemm(0, 0, B, H, n, ptr(self.dlb), n, self._p(wb, 4), H, C=ptr(self.da2), ldc=H, Cb=ptr(self.da2b),
+                   ldcb=H, dmask=ptr(self.h2), ldm=H)
+        call('rgs_colsum', ptr(self.da2), B, H, H, self._p(g, 3), 0, s)
+        self._gemm(1, 0, H, H, B, ptr(self.da2b), H, ptr(self.h1), H, C=self._p(g, 2), ldc=H)
+        self._gemm(0, 0, B, H, H, ptr(self.da2b), H, self._p(wb, 2), H, C=ptr(self.da1), ldc=H, Cb=ptr(self.da1b),
+                   ldcb=H, dmask=ptr(self.h1), ldm=H)
+        call('rgs_colsum', ptr(self.da1), B, H, H, self._p(g, 1), 0, s)
+        self._gemm(1, 0, H, n, B, ptr(self.da1b), H, ptr(self.x), n, C=self._p(g, 0), ldc=n)
+
+    def adam(self):
+        call('rgs_adam', ptr(self.w), ptr(self.g), ptr(self.mask), ptr(self.m), ptr(self.v), ptr(self.wb), self.P,
+             ptr(self.t), self.lr, 0.9, 0.999, 1e-8, self.stream.cuda_stream)
+
+    def train_step(self):
+        self.forward()
+        self.loss()
+        self.backward()
+        self.adam()
+
+    def load_batch(self, bits_bf16_dev):
+        """Copy a [B, n] int16 (bf16 0/1) device tensor into the input buffer."""
+        self.x.copy_(bits_bf16_dev)
+
+    def train(self, data_bits, steps, seed=0, log_every=0):
+        """data_bits: uint8 [N, n] exact samples.  Epochs are reshuffled on the
+        host and uploaded once; each step copies its slice into the static
+        input buffer (device to device)."""
+        torch = self.torch
+        N = len(data_bits)
+        B = self.bufB
+        rng = np.random.default_rng(seed)
+        per_epoch = N // B
+        epochs = -(-steps // per_epoch)
+        order = np.concatenate([rng.permutation(N)[:per_epoch * B] for _ in range(epochs)])[:steps * B]
+        dev = torch.as_tensor(bits_to_bf16(data_bits[order])).to(self.dev)
+        losses = []
+        for k in range(steps):
+            self.x.copy_(dev[k * B:(k + 1) * B])
+            self.train_step()
+            if log_every and (k + 1) % log_every == 0:
+                losses.append(float(self.nll.mean().item()))
+        return losses
+
+    def log_q(self, bits, chunk=None):
+        """log q(x) (natural log) of uint8 bitstrings [count, n]."""
+        torch = self.torch
+        bits = np.asarray(bits, np.uint8)
+        out = np.empty(len(bits))
+        B = self.bufB
+        for s0 in range(0, len(bits), B):
+            part = bits[s0:s0 + B]
+            b = len(part)
+            self.x[:b].copy_(torch.as_tensor(bits_to_bf16(part)))
+            self.forward(b)
+            self.loss(b, grad=False)
+            out[s0:s0 + b] = -self.nll[:b].double().cpu().numpy()
+        return out
+
+    def sample(self, uniforms):
+        """Autoregressive samples from q with uniforms [count, n] (float64):
+        x_o = [u_o < sigmoid(l_o)].  Returns (bits uint8 [count, n], log q)."""
+        torch = self.torch
+        u_all = np.ascontiguousarray(uniforms, np.float64)
+        count = len(u_all)
+        B = self.bufB
+        bits = np.empty((count, self.n), np.uint8)
+        logq = np.empty(count)
+        for s0 in range(0, count, B):
+            b = min(B, count - s0)
+            u = torch.as_tensor(u_all[s0:s0 + b]).to(self.dev)
+            self.x.zero_()
+            for o in range(self.n):
+                self.forward(b)
+                call('rgs_sample_col', ptr(self.logits), ptr(u), b, self.n, o, ptr(self.x), self.stream.cuda_stream)
+            self.forward(b)
+            self.loss(b, grad=False)
+            bits[s0:s0 + b] = (self.x[:b].cpu().numpy() != 0).astype(np.uint8)
+            logq[s0:s0 + b] = -self.nll[:b].double().cpu().numpy()
+        return bits, logq
+
+    def params_numpy(self):
+        """FP32 master weights as float64 arrays in (W1, b1, W2, b2, W3, b3) order."""
+        w = self.w.cpu().numpy().astype(np.float64)
+        return [w[self.offsets[k]:self.offsets[k + 1]].reshape(self.shapes[k]) for k in range(6)]
+
+    def bf16_params_numpy(self):
+        """The BF16 working weights as float64 arrays (what the GEMMs use)."""
+        wb = self.wb.cpu().numpy().view(np.uint16).astype(np.uint32) << 16
+        w = wb.view(np.float32).astype(np.float64)
+        return [w[self.offsets[k]:self.offsets[k + 1]].reshape(self.shapes[k]) for k in range(6)]
+
+    def grads_numpy(self):
+        g = self.g.cpu().numpy().astype(np.float64)
+        return [g[self.offsets[k]:self.offsets[k + 1]].reshape(self.shapes[k]) for k in range(6)]

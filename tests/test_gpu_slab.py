@@ -2,10 +2,13 @@
 
 Tolerances: amplitudes within an L2 distance of 1e-5 of the oracle's
 normalised state (the reference's --validate bar, distributed_sampler.py:186)
-and p(x) within 1e-5 relative wherever p(x) >= 1e-3 / 2^n; the FP64 mass
+and p(x) within 1e-5 relative wherever p(x) >= 0.1 / 2^n (complex64 rounding
+is absolute in the amplitude, so the relative error of p grows as
+1/sqrt(2^n p); 99.5% of Porter-Thomas draws have p >= 0.1 / 2^n); the FP64 mass
 pyramid and the draws are compared bit for bit with the oracle run on the
 device's own complex64 amplitudes, and draws from the complex128 oracle state
-must agree wherever the descent margin exceeds 1e-5.
+must agree wherever the descent margin exceeds three times the L1 distance
+between the two distributions (the bound under which no decision can flip).
 """
 import numpy as np
 import pytest
@@ -29,16 +32,18 @@ def check_state(got, ref):
     assert np.linalg.norm(got.astype(np.complex128) - ref) < 1e-5
     p = od.probabilities(got)
     pr = np.abs(ref) ** 2
-    big = pr >= 1e-3 / 2 ** n
+    big = pr >= 0.1 / 2 ** n
     rel = np.abs(p[big] - pr[big]) / pr[big]
     assert rel.max() < 1e-5, rel.max()
     return rel
 
 
 def check_pyramid_and_draws(sampler, ref, uniforms):
-    host = sampler.host()
+    # masses come from the device values; the D1 phases applied on readback
+    # leave |amplitude|^2 unchanged only up to complex128 rounding
+    raw = sampler.device_amplitudes()
     sl = sampler.slab
-    levels = od.pyramid(od.probabilities(host))
+    levels = od.pyramid(od.probabilities(raw))
     B = sl.base_bits
     dev = sl.pyramid.cpu().numpy()
     lev_dev = levels[sl.mass_bits:]
@@ -51,10 +56,19 @@ def check_pyramid_and_draws(sampler, ref, uniforms):
     exp_idx, _ = od.tree_draws(levels, uniforms)
     assert np.array_equal(idx, exp_idx)                         # bit-exact on the same amplitudes
     assert np.array_equal(p, levels[0][idx])
-    ref_levels = od.pyramid(np.abs(ref) ** 2)
-    ref_idx, margin = od.tree_draws(ref_levels, uniforms)
-    ok = margin > 1e-5
-    assert ok.mean() > 0.95
+    # The descent is an inverse-CDF lookup: index i with C(i) <= u R < C(i+1).
+    # It cannot flip while u R stays farther from both bin edges than the
+    # change of R and of the partial sums C between the two distributions.
+    pref = np.abs(ref) ** 2
+    ref_idx, _ = od.tree_draws(od.pyramid(pref), uniforms)
+    C = np.concatenate([[0.0], np.cumsum(pref)])
+    dC = np.abs(np.concatenate([[0.0], np.cumsum(levels[0] - pref)]))
+    R, dR = C[-1], abs(levels[0].sum() - C[-1])
+    t = uniforms * R
+    margin = np.minimum(t - C[ref_idx], C[ref_idx + 1] - t)
+    bound = uniforms * dR + np.maximum(dC[ref_idx], dC[ref_idx + 1]) + 1e-12 * R
+    ok = margin > bound
+    assert ok.mean() > 0.95, ok.mean()
     assert np.array_equal(idx[ok], ref_idx[ok])
 
 
@@ -122,7 +136,7 @@ def test_walsh_and_gates_on_arbitrary_input(tun):
     from rgs70_classical_challenge_amd.instances import haar_unitaries
     u = haar_unitaries(n, rng)
     sl = _load_slab(n, tun)
-    gates = sl.to_device(E._gates_array(u))
+    gates = E.Gates(sl, u, gate_limit=n - 1)
     for walsh, use_gates in ((True, False), (False, True), (True, True)):
         print(f'case walsh={walsh} gates={use_gates} tuning={tun}', flush=True)
         sl.state.copy_(torch.as_tensor(v))
@@ -130,6 +144,8 @@ def test_walsh_and_gates_on_arbitrary_input(tun):
         sl.run(d, mass=False)
         sl.sync()
         got = sl.state.cpu().numpy().astype(np.complex128)
+        if use_gates:
+            gates.apply_d1(got)
         ref = v.astype(np.complex128)
         if walsh:
             gs.walsh(ref)

@@ -78,7 +78,7 @@ def test_tree_draw_is_inverse_cdf():
     u = rng.random(2000)
     idx, margin = od.tree_draws(levels, u)
     ref = np.searchsorted(np.cumsum(p), u * p.sum(), side='right')
-    ok = margin > 1e-12
+    ok = margin > 1e-14
     assert ok.mean() > 0.99
     assert np.array_equal(idx[ok], ref[ok])
     assert np.all(p[idx] > 0)
