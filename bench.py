@@ -5,26 +5,32 @@ Workload (one "step"): a seeded random 3-regular graph on n=34 vertices with
 QR-drawn single-qubit unitaries; the state (x)_k u[k] |G> (2^34 complex64
 amplitudes, 128 GiB) is built by the slab pipeline - GF(2) phase init ->
 Walsh-Hadamard over all 34 bits -> per-qubit 2x2 gates -> FP64 masses - in
-five fused HBM passes, the FP64 mass pyramid is built, and 10,000 exact
-samples are drawn qubit by qubit on the device.  The graph, tables and the
-10,000 uniforms of every step are resident in HBM before the timed region.
+a few fused HBM passes (tile plan from the autotuner table, else the default
+plan), the FP64 mass pyramid is built, and 10,000 exact samples are drawn
+qubit by qubit on the device.  The graph, tables and the 10,000 uniforms of
+every step are resident in HBM before the timed region.
 
 Reported beside the headline value:
-  roofline      k_tile (the pass kernel): algorithmic bytes of the five passes
-                (8 B/amplitude for the write-only init pass, 16 B/amplitude
-                for read+write passes) / their HIP-event time on the launch
-                stream, against the 8 TB/s HBM peak; `traffic` from the
-                committed rocprofv3 FETCH_SIZE/WRITE_SIZE measurement when
-                present.
-  passes        per-pass GB/s; `walsh_gbps` over the three Walsh passes.
-  folded        the same state with H^n|0> folded into the init (init ->
-                gates: 3 passes), samples/s.
+  roofline      the dominant pass kernel (the k_tile specialisation with the
+                most time per step): algorithmic bytes per launch (16 B per
+                amplitude for a read+write pass, 8 B for the write-only init
+                pass) / its mean HIP-event duration on the launch stream,
+                against the 8 TB/s HBM peak; `traffic` = measured HBM bytes per
+                launch from the committed rocprofv3 FETCH_SIZE / WRITE_SIZE
+                summary, when present.
+  kernels       the same figures for every specialisation in the plan.
+  passes        per-pass ms and GB/s; `walsh_gbps` over the Walsh passes.
+  folded        the same state with H^n|0> folded into the init (no Walsh
+                passes), samples/s.
   scan          batched rank-scan p(x) of 65,536 uniform 70-bit proposals of a
                 70-vertex 3-regular instance per launch (complex64 tables, FP64
                 sums), proposals/s and the complex64-vs-complex128 error.
   cpu_baseline  the oracle (NumPy restatement of the reference's CPU
-                full-state path) on the host: n=24 state + mass pyramid +
-                10,000 draws, extrapolated to n=34 by 2^10 * 34/24.
+                full-state path) on one host core: n=26 state + mass pyramid +
+                10,000 draws, extrapolated to n=34 by 2^8 * 34/26.
+
+`--calibrate-copy` instead runs one plain device copy of 2^32 bytes (torch
+copy_), the reference kernel for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE.
 
 Prints one JSON line on stdout; progress goes to stderr.
 """
@@ -53,6 +59,12 @@ def pass_bytes(plan_text, n):
     for tok in plan_text.split():
         out.append((8 if '[init]' in tok else 16) * (1 << n))
     return out
+
+
+def kernel_of(tok):
+    """k_tile specialisation of a plan token such as 'strided[wht]:14+8/tb14/c6/rb2'."""
+    _, tb, c, rb = tok.split('/')
+    return f'k_tile<{int(tb[2:])}, {int(c[1:])}, {int(rb[2:])}>'
 
 
 def run_headline(args, torch):
@@ -110,14 +122,24 @@ def run_headline(args, torch):
         per_pass.append({'pass': tok, 'ms': round(pass_ms[p], 3), 'gbps': round(nbytes[p] / pass_ms[p] / 1e6, 1)})
     wht = [p for p, tok in enumerate(s.plan_text.split()) if '[wht]' in tok]
     wht_gbps = sum(nbytes[p] for p in wht) / sum(pass_ms[p] for p in wht) / 1e6 if wht else None
-    kern_ms = sum(pass_ms[:npass])
-    achieved = sum(nbytes) / kern_ms / 1e6
+    kernels = {}
+    for p, tok in enumerate(s.plan_text.split()):
+        k = kernels.setdefault(kernel_of(tok), {'launches': 0, 'ms': 0.0, 'bytes': 0})
+        k['launches'] += 1
+        k['ms'] += pass_ms[p]
+        k['bytes'] += nbytes[p]
+    for k in kernels.values():
+        k['avg_ms'] = round(k['ms'] / k['launches'], 3)
+        k['bytes_per_launch'] = k['bytes'] / k['launches']
+        k['gbps'] = round(k['bytes'] / k['ms'] / 1e6, 1)
+        k['ms'] = round(k['ms'], 3)
+    dominant = max(kernels, key=lambda name: kernels[name]['ms'])
     # probabilities of the drawn samples stay on the device; sanity-check a few on the host
     drawn = idx[:16].cpu().numpy()
     assert np.all((drawn >= 0) & (drawn < (1 << n)))
-    res = dict(s=s, t=t1 - t0, norm=norm, per_pass=per_pass, wht_gbps=wht_gbps, achieved=achieved,
-               kernel_avg_ms=kern_ms / npass, bytes_per_launch=sum(nbytes) / npass, pyramid_ms=pass_ms[npass],
-               draw_ms=pass_ms[npass + 1])
+    res = dict(s=s, t=t1 - t0, norm=norm, per_pass=per_pass, wht_gbps=wht_gbps, kernels=kernels,
+               dominant=dominant, passes_gbps=sum(nbytes) / sum(pass_ms[:npass]) / 1e6,
+               pyramid_ms=pass_ms[npass], draw_ms=pass_ms[npass + 1])
     return res
 
 
@@ -126,7 +148,7 @@ def run_folded(s, args, torch):
     from rgs70_classical_challenge_amd import _lib
     from rgs70_classical_challenge_amd.engine import Gates, ctypes_ref
     sl = s.slab
-    d = sl.desc(_lib.INIT_GRAPH, 2.0 ** (-s.n / 2), init_lut=s._lut, gates=Gates(sl, s.u))
+    d = sl.desc(_lib.INIT_GRAPH, 2.0 ** (-s.n / 2), init_lut=s._lut, init_rows=s._rows, gates=Gates(sl, s.u))
     passes, text = sl.describe(d)
     rng = np.random.default_rng(5)
     uni = torch.as_tensor(rng.random(args.samples)).to(sl.device)
@@ -180,11 +202,11 @@ def run_scan(torch, count=65536, seed=70):
 
 
 def cpu_baseline(seed):
-    """Oracle (NumPy) on the host: bounded sample at n=24, extrapolated to n=34."""
+    """Oracle (NumPy) on one host core: bounded sample at n=26, extrapolated to n=34."""
     from oracle import draw as od
     from oracle import graph_state as gs
     from rgs70_classical_challenge_amd.instances import instance
-    n0, samples = 24, 10000
+    n0, samples = 26, 10000
     g, u = instance(n0, seed)
     t0 = time.perf_counter()
     v = gs.full_state(g, u)
@@ -194,7 +216,16 @@ def cpu_baseline(seed):
     scale = 2.0 ** (34 - n0) * 34 / n0
     return {'value': round(samples / (dt * scale), 4), 'unit': 'samples/s', 'cores': 1, 'kind': 'port',
             'sample': f'oracle full state + pyramid + {samples} draws at n={n0} took {dt:.1f} s on 1 core; '
-                      f'extrapolated x2^10*34/24 to n=34 (state work is n*2^n)'}
+                      f'extrapolated x2^{34 - n0}*34/{n0} to n=34 (state work is n*2^n)'}
+
+
+def calibrate_copy(torch):
+    """One plain device-to-device copy of 2^32 bytes for counter calibration."""
+    a = torch.ones(1 << 30, dtype=torch.float32, device='cuda')
+    b = torch.empty_like(a)
+    b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({'calibration': 'torch copy_', 'bytes_read': 1 << 32, 'bytes_written': 1 << 32}), flush=True)
 
 
 def main():
@@ -208,6 +239,7 @@ def main():
     ap.add_argument('--seed', type=int, default=20261203)
     ap.add_argument('--no-cpu', action='store_true', help='skip the host baseline')
     ap.add_argument('--no-extras', action='store_true', help='skip folded / scan extras')
+    ap.add_argument('--calibrate-copy', action='store_true', help='run only the counter-calibration copy')
     args = ap.parse_args()
     world = int(os.environ.get('WORLD_SIZE', '1'))
     if args.gpus != 1 or world != 1:
@@ -218,6 +250,9 @@ def main():
     if not torch.cuda.is_available():
         print('bench.py needs a HIP device', file=sys.stderr)
         sys.exit(1)
+    if args.calibrate_copy:
+        calibrate_copy(torch)
+        return
     from rgs70_classical_challenge_amd import _lib
     _lib.lib()
 
@@ -241,21 +276,26 @@ def main():
                                f'(init -> Walsh -> gates -> FP64 masses), {args.samples} exact samples per step',
                    'n': args.n, 'samples_per_step': args.samples, 'pipeline': args.pipeline, 'seed': args.seed,
                    'passes': s.passes, 'plan': s.plan_text},
-        'roofline': {'bound': 'hbm', 'achieved': round(res['achieved'], 1), 'peak': HBM_PEAK_GBPS, 'unit': 'GB/s',
-                     'frac': round(res['achieved'] / HBM_PEAK_GBPS, 4), 'traffic': None,
-                     'kernel': 'k_tile (slab pass)', 'kernel_avg_ms': round(res['kernel_avg_ms'], 3),
-                     'algorithmic_bytes_per_launch': res['bytes_per_launch']},
+        'roofline': None,
+        'kernels': res['kernels'],
+        'passes_gbps': round(res['passes_gbps'], 1),
         'walsh_gbps': round(res['wht_gbps'], 1) if res['wht_gbps'] else None,
         'passes': res['per_pass'],
         'pyramid_ms': round(res['pyramid_ms'], 3),
         'draw_ms': round(res['draw_ms'], 3),
         'fp64_norm': res['norm'],
     }
-    traffic = os.path.join(ROOT, 'profiles', 'traffic_k_tile_n34.json')
+    dom = res['dominant']
+    k = res['kernels'][dom]
+    achieved = k['bytes_per_launch'] / k['avg_ms'] / 1e6
+    out['roofline'] = {'bound': 'hbm', 'achieved': round(achieved, 1), 'peak': HBM_PEAK_GBPS, 'unit': 'GB/s',
+                       'frac': round(achieved / HBM_PEAK_GBPS, 4), 'traffic': None, 'kernel': dom,
+                       'avg_ms': k['avg_ms'], 'algorithmic_bytes_per_launch': k['bytes_per_launch']}
+    traffic = os.path.join(ROOT, 'profiles', 'k_tile_traffic_n34.json')
     if os.path.exists(traffic):
         t = json.load(open(traffic))
-        if t.get('pipeline') == args.pipeline and t.get('n') == args.n:
-            out['roofline']['traffic'] = t['hbm_bytes_per_launch']
+        if t.get('plan') == s.plan_text and dom in t.get('kernels', {}):
+            out['roofline']['traffic'] = t['kernels'][dom]['hbm_bytes_per_launch']
     if not args.no_extras:
         try:
             out['folded'] = run_folded(s, args, torch)

@@ -95,10 +95,17 @@ typedef struct rgs_slab_desc {
   int init_kind;               /* RGS_INIT_*                                */
   float init_scale;            /* GRAPH / DELTA                             */
   const uint64_t *init_lut;    /* dev GRAPH: phase words; COV: map_lut      */
+  const uint64_t *init_rows;   /* HOST uint64[nbits], required for GRAPH:
+                                  row a has bit b set iff the phase form
+                                  holds x_a x_b (symmetric), bit a iff it
+                                  holds the linear term x_a (the adjacency
+                                  rows for a graph phase)                    */
   const float *init_wlut;      /* dev COV weights (complex64)               */
   int do_walsh;                /* unnormalised Walsh transform on all bits  */
   const uint64_t *phase_lut;   /* dev sign words applied after the Walsh,
                                   or NULL                                    */
+  const uint64_t *phase_rows;  /* HOST uint64[nbits] form rows of phase_lut
+                                  (as init_rows); required with phase_lut   */
   const float *dphase_lut;     /* dev complex64 [ceil(nbits/8)][256] diagonal
                                   phase applied with phase_lut, or NULL      */
   const float *rot;            /* HOST float32 [nbits][2] (cos, sin) of the
@@ -111,6 +118,9 @@ typedef struct rgs_slab_desc {
   int low_bits;                /* tile bits of the contiguous pass (6..14)  */
   int tile_bits;               /* tile bits of strided passes (6..14)       */
   int lane_bits;               /* contiguous bits per strided row (>= 1)    */
+  int round_bits;              /* strided passes stage 2^-round_bits of the
+                                  tile in LDS per round (0..4, <= lane_bits;
+                                  fewer LDS bytes, more workgroups per CU)  */
 } rgs_slab_desc;
 
 /* Factor 2x2 unitaries u (complex128 [n][2][2]) as u = D1 R D2 with

@@ -31,9 +31,11 @@ class SlabDesc(ctypes.Structure):
         ('init_kind', c_int),
         ('init_scale', c_float),
         ('init_lut', c_void_p),
+        ('init_rows', c_void_p),
         ('init_wlut', c_void_p),
         ('do_walsh', c_int),
         ('phase_lut', c_void_p),
+        ('phase_rows', c_void_p),
         ('dphase_lut', c_void_p),
         ('rot', c_void_p),
         ('gate_limit', c_int),
@@ -42,6 +44,7 @@ class SlabDesc(ctypes.Structure):
         ('low_bits', c_int),
         ('tile_bits', c_int),
         ('lane_bits', c_int),
+        ('round_bits', c_int),
     ]
 
 

@@ -21,22 +21,29 @@ TABLE = os.path.join(HERE, 'tuned.json')
 
 
 def candidates(n):
+    """Contiguous-pass bits 10..14, strided tiles of 12..14 bits with 3..7
+    lane bits, LDS rounds 0 or 2 (plans the pass planner rejects are
+    skipped when they are described)."""
+    if n <= 14:
+        return [dict(low_bits=n, tile_bits=14, lane_bits=4, round_bits=0)]
     out = []
-    for low in (12, 13, 14):
-        if low > n:
-            continue
+    lows = (12, 13, 14) if n >= 32 else (10, 11, 12, 13, 14)
+    for low in lows:
         for tile in (12, 13, 14):
-            for lane in (2, 3, 4, 5, 6):
+            for lane in ((4, 5, 6, 7) if n >= 32 else (3, 4, 5, 6, 7)):
                 if tile - lane < 4:
                     continue
-                out.append(dict(low_bits=low, tile_bits=tile, lane_bits=lane))
-    if n < 12:
-        out.append(dict(low_bits=n, tile_bits=14, lane_bits=4))
+                for rb in (0, 2):
+                    out.append(dict(low_bits=low, tile_bits=tile, lane_bits=lane, round_bits=rb))
     return out
 
 
+def device_key_of(name):
+    return name.replace(' ', '_') or 'gpu'
+
+
 def device_key(torch):
-    return torch.cuda.get_device_name(0).replace(' ', '_') or 'gpu'
+    return device_key_of(torch.cuda.get_device_name(0))
 
 
 def load_table(torch=None):
@@ -53,11 +60,18 @@ def lookup(n, torch):
     return t.get(str(n))
 
 
+def lookup_name(n, device_name):
+    """Entry for size n on the device of that name, or None."""
+    if not os.path.exists(TABLE):
+        return None
+    return json.load(open(TABLE)).get(device_key_of(device_name), {}).get(str(n))
+
+
 def tune(n, reps=3, seed=7):
     """Time every candidate for size n; returns (best tuning, report list)."""
     import torch
     from . import _lib
-    from .engine import GraphStateSampler, ctypes_ref
+    from .engine import GraphStateSampler, ctypes_ref, effective_low_bits
     from .instances import instance
     g, u = instance(n, seed)
     s = GraphStateSampler(g, u, pipeline='slab')
@@ -68,7 +82,10 @@ def tune(n, reps=3, seed=7):
     for cand in candidates(n):
         sl.tuning = dict(cand)
         try:
-            d = sl.desc(_lib.INIT_GRAPH, 2.0 ** (-n), init_lut=s._lut, walsh=True, gates=s._gates)
+            d = sl.desc(_lib.INIT_GRAPH, 2.0 ** (-n), init_lut=s._lut, init_rows=s._rows, walsh=True,
+                        gates=s._gates)
+            if sl.mass_bits > effective_low_bits(n, cand['low_bits']):
+                continue
             passes, text = sl.describe(d)
         except ValueError:
             continue

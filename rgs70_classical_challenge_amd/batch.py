@@ -55,11 +55,8 @@ class BatchSampler:
         self.graphs = {}
 
     def _tuning_for(self, n):
-        if self.tuning != 'auto':
-            return self.tuning
-        from .autotune import lookup
-        t = lookup(n, self.torch)
-        return None if t is None else {k: t[k] for k in ('low_bits', 'tile_bits', 'lane_bits')}
+        # 'auto': the autotuner's table for this device (default_tuning where it has no entry)
+        return None if self.tuning == 'auto' else self.tuning
 
     def _build(self, wave):
         torch = self.torch

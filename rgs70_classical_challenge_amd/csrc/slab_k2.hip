@@ -1,0 +1,4 @@
+// k_tile specialisations with 2 lane bits.
+#define RGS_C 2
+#define RGS_LAUNCH_NAME launch_c2
+#include "slab_tiles.inc"

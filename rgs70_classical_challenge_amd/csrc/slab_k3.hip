@@ -1,0 +1,4 @@
+// k_tile specialisations with 3 lane bits.
+#define RGS_C 3
+#define RGS_LAUNCH_NAME launch_c3
+#include "slab_tiles.inc"

@@ -1,0 +1,4 @@
+// k_tile specialisations with 4 lane bits.
+#define RGS_C 4
+#define RGS_LAUNCH_NAME launch_c4
+#include "slab_tiles.inc"

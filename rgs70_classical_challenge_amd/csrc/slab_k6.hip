@@ -1,0 +1,4 @@
+// k_tile specialisations with 6 lane bits.
+#define RGS_C 6
+#define RGS_LAUNCH_NAME launch_c6
+#include "slab_tiles.inc"

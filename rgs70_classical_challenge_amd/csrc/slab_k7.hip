@@ -1,0 +1,4 @@
+// k_tile specialisations with 7 lane bits.
+#define RGS_C 7
+#define RGS_LAUNCH_NAME launch_c7
+#include "slab_tiles.inc"

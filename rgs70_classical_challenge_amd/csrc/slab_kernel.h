@@ -1,0 +1,416 @@
+// Tile kernel of the slab pipeline, specialised at compile time on the tile
+// bits TB, the lane bits C and the LDS round bits RBW.
+//
+// Tiles.  A pass works on tiles of 2^TB amplitudes; a workgroup of 2^(TB-4)
+// threads holds one tile, 16 amplitudes per thread in registers.  Tile-local
+// bit L maps to a global bit: local bits 0..C-1 are the contiguous global bits
+// 0..C-1 (rows of 2^C amplitudes; C = 0 for the contiguous "low" pass) and
+// local bits C..TB-1 are the target bits qstart..qstart+TB-C-1.  The tile
+// index supplies the remaining global bits.
+//
+// Layouts.  In layout s the registers hold local bits s..s+3 and the thread
+// index enumerates the other local bits in increasing order, so the lane bits
+// are the lowest thread bits and a wavefront touches whole rows when it loads
+// or stores.  Butterflies act on register bits; a layout change goes through
+// LDS (one padded float2 per 16).  The layouts visited are s_0 = max(TB-4, C),
+// s_{k+1} = max(s_k - 4, C); layout k owns the local bits [s_k, s_{k-1})
+// (s_{-1} = TB), so every bit is operated on exactly once.  With RBW > 0 each
+// layout change runs in 2^RBW rounds, one per value of the top RBW lane bits
+// (the top thread bits in every layout), so LDS holds 2^(TB-RBW) amplitudes.
+//
+// One workgroup per tile.  At 14 tile bits a workgroup fills a CU's LDS and
+// its 4 waves per SIMD, so loads, butterflies and stores of a CU run one
+// after another; at 12-13 tile bits two to four workgroups share a CU and
+// overlap.  (A second register set to prefetch the next tile does not fit
+// the 128-VGPR budget of 1024-thread workgroups.)
+#pragma once
+#include "rgs_common.h"
+#include "rgs_amd.h"
+
+#include <type_traits>
+
+namespace rgs {
+
+enum : int { F_LOAD = 1, F_WHT = 2, F_PHASE = 4, F_GATES = 8, F_MASS = 16 };
+
+constexpr size_t kLdsBudget = 160 * 1024;
+
+struct PassArgs {
+  float2* state;
+  const uint64_t* init_lut;   // GRAPH: phase words [ic][256]; COV: map+phase [ic][256][2]
+  const float2* init_w;       // COV: [ic][256]
+  const uint64_t* phase_lut;  // [pc][256] sign words, or null
+  const float2* dphase;       // [dc][256] complex diagonal phase, or null
+  double* mass;
+  float2 rot[16];             // (cos, sin) of target bit i of the pass (identity beyond gate_limit)
+  // Separable sign evaluation of the init / phase_lut forms over the four
+  // register bits (global bits p_0..p_3) of the layout where each is applied:
+  // q[i] = symmetric form row of p_i, freg bit j = f(r_j) ^ f(0) for the
+  // register offset r_j of register j.
+  uint64_t iq[4];
+  uint64_t pq[4];
+  uint32_t ifreg;
+  uint32_t pfreg;
+  float init_scale;
+  int nbits;
+  int qstart;      // global bit of local bit C (0 for the low tile)
+  int flags;
+  int init_kind;
+  int init_chunks;
+  int phase_chunks;
+  int dphase_chunks;
+  int gate_limit;
+  int mass_bits;
+};
+
+__device__ __forceinline__ uint32_t phase_parity(uint64_t x, const uint64_t* lut, int chunks) {
+  uint64_t acc = 0;
+  for (int c = 0; c < chunks; ++c) acc ^= lut[(c << 8) | ((x >> (8 * c)) & 255u)];
+  return (uint32_t)__popcll(acc & (x | (1ull << 63))) & 1u;
+}
+
+// Signs (-1)^{f(x0 + r_j)} of the 16 register values of a thread as a 16-bit
+// mask.  A quadratic form over GF(2) satisfies
+//   f(x0 ^ r) = f(x0) ^ f(r) ^ f(0) ^ B(x0, r),   B(x0, r) = sum_i r_i parity(x0 & q[i]),
+// when x0 and r share no set bits, so one table evaluation per thread plus
+// four popcounts replace a table evaluation per amplitude.
+__device__ __forceinline__ uint32_t form_signs(uint64_t x0, const uint64_t* lut, int chunks, const uint64_t* q,
+                                               uint32_t freg) {
+  uint32_t e = freg ^ (phase_parity(x0, lut, chunks) ? 0xFFFFu : 0u);
+  if (__popcll(x0 & q[0]) & 1) e ^= 0xAAAAu;
+  if (__popcll(x0 & q[1]) & 1) e ^= 0xCCCCu;
+  if (__popcll(x0 & q[2]) & 1) e ^= 0xF0F0u;
+  if (__popcll(x0 & q[3]) & 1) e ^= 0xFF00u;
+  return e;
+}
+
+__device__ __forceinline__ float2 dphase_of(uint64_t x, const float2* lut, int chunks) {
+  float2 v = lut[x & 255u];
+  for (int c = 1; c < chunks; ++c) v = cmul(v, lut[(c << 8) | ((x >> (8 * c)) & 255u)]);
+  return v;
+}
+
+// Per-amplitude init of the kinds without a separable form.
+__device__ __forceinline__ float2 init_value(uint64_t x, const PassArgs& a, const uint64_t* lut, const float2* w) {
+  if (a.init_kind == RGS_INIT_COV) {
+    uint64_t b = 0, acc = 0;
+    for (int c = 0; c < a.init_chunks; ++c) {
+      uint32_t e = (c << 8) | ((x >> (8 * c)) & 255u);
+      b ^= lut[2 * e];
+      acc ^= lut[2 * e + 1];
+    }
+    float2 val = w[b & 255u];
+    for (int c = 1; c < a.init_chunks; ++c) val = cmul(val, w[(c << 8) | ((b >> (8 * c)) & 255u)]);
+    return cneg_if(val, (uint32_t)__popcll(acc & (x | (1ull << 63))) & 1u);
+  }
+  // RGS_INIT_DELTA
+  return x == 0 ? make_float2(a.init_scale, 0.f) : make_float2(0.f, 0.f);
+}
+
+// Wait for this wave's LDS traffic, then meet the other waves (global memory
+// needs no ordering across the barrier: every workgroup owns its tile).
+__device__ __forceinline__ void lds_barrier() { asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory"); }
+
+constexpr int cmax(int a, int b) { return a > b ? a : b; }
+constexpr int cmin(int a, int b) { return a < b ? a : b; }
+
+template <int TB, int C, int RBW>
+struct Geo {
+  static constexpr int R = 16;
+  static constexpr int NT = 1 << (TB - 4);
+  static constexpr int TLB = TB - 4 - RBW;  // thread bits below the round bits
+  static constexpr int S1 = cmax(TB - 4, C);
+  static constexpr int next(int s) { return cmax(s - 4, C); }
+  static constexpr int count() {
+    int k = 1;
+    for (int s = S1; s > C; s = next(s)) ++k;
+    return k;
+  }
+  static constexpr int at(int k) {
+    int s = S1;
+    for (int i = 0; i < k; ++i) s = next(s);
+    return s;
+  }
+  static constexpr int hi(int k) { return k == 0 ? TB : at(k - 1); }
+  static constexpr uint32_t slots = (1u << (TB - RBW)) + (1u << (TB - RBW - 4));
+  static_assert(RBW == 0 || TLB >= 6, "round bits must be wave-uniform thread bits");
+  static_assert(RBW <= C, "round bits are lane bits");
+  static_assert(TB - C >= 4, "a strided tile needs four target bits");
+};
+
+// Compressed local index (round bits removed) of register 0 in layout S.
+template <int S, int RBW>
+__device__ __forceinline__ uint32_t comp_base(uint32_t tl) {
+  constexpr int sc = S - RBW;
+  return (tl & ((1u << sc) - 1u)) | ((tl >> sc) << (sc + 4));
+}
+
+// Full local index of register 0 in layout S.
+template <int S, int C, int RBW>
+__device__ __forceinline__ uint32_t local_base(uint32_t tl, uint32_t tr) {
+  const uint32_t x = comp_base<S, RBW>(tl);
+  constexpr int lo = C - RBW;
+  return (x & ((1u << lo) - 1u)) | (tr << lo) | ((x >> lo) << C);
+}
+
+template <int S, int RBW>
+__device__ __forceinline__ constexpr uint32_t slot_off(int j) {
+  return ((uint32_t)j << (S - RBW)) + (((uint32_t)j << (S - RBW)) >> 4);
+}
+
+template <int TB, int C, int RBW, int SF, int ST>
+__device__ __forceinline__ void exchange(float2 (&v)[16], float2* lds, uint32_t tl, uint32_t tr) {
+  if constexpr (SF != ST) {
+    const uint32_t wc = comp_base<SF, RBW>(tl), rc = comp_base<ST, RBW>(tl);
+    const uint32_t wb = wc + (wc >> 4), rbs = rc + (rc >> 4);
+#pragma unroll
+    for (int r = 0; r < (1 << RBW); ++r) {
+      if (tr == (uint32_t)r) {
+#pragma unroll
+        for (int j = 0; j < 16; ++j) lds[wb + slot_off<SF, RBW>(j)] = v[j];
+      }
+      lds_barrier();
+      if (tr == (uint32_t)r) {
+#pragma unroll
+        for (int j = 0; j < 16; ++j) v[j] = lds[rbs + slot_off<ST, RBW>(j)];
+      }
+      lds_barrier();
+    }
+  }
+}
+
+template <int B>
+__device__ __forceinline__ void wht_reg(float2 (&v)[16]) {
+#pragma unroll
+  for (int j = 0; j < 16; ++j)
+    if (!(j & (1 << B))) {
+      const float2 x = v[j], y = v[j | (1 << B)];
+      v[j] = cadd(x, y);
+      v[j | (1 << B)] = csub(x, y);
+    }
+}
+
+// Real rotation [[c, -s], [s, c]] on register bit B.
+template <int B>
+__device__ __forceinline__ void rot_reg(float2 (&v)[16], float2 cs) {
+#pragma unroll
+  for (int j = 0; j < 16; ++j)
+    if (!(j & (1 << B))) {
+      const float2 x = v[j], y = v[j | (1 << B)];
+      v[j] = make_float2(cs.x * x.x - cs.y * y.x, cs.x * x.y - cs.y * y.y);
+      v[j | (1 << B)] = make_float2(cs.y * x.x + cs.x * y.x, cs.y * x.y + cs.x * y.y);
+    }
+}
+
+// Butterflies of layout K on the register bits it owns.
+template <typename G, int K>
+__device__ __forceinline__ void wht_layout(float2 (&v)[16]) {
+  constexpr int s = G::at(K), h = G::hi(K);
+  if constexpr (s + 0 < h) wht_reg<0>(v);
+  if constexpr (s + 1 < h) wht_reg<1>(v);
+  if constexpr (s + 2 < h) wht_reg<2>(v);
+  if constexpr (s + 3 < h) wht_reg<3>(v);
+}
+
+template <typename G, int C, int K>
+__device__ __forceinline__ void rot_layout(float2 (&v)[16], const PassArgs& a) {
+  constexpr int s = G::at(K), h = G::hi(K);
+  // target bit index of register bit B: s + B - C
+  if constexpr (s + 0 < h) if (a.qstart + s + 0 - C < a.gate_limit) rot_reg<0>(v, a.rot[s + 0 - C]);
+  if constexpr (s + 1 < h) if (a.qstart + s + 1 - C < a.gate_limit) rot_reg<1>(v, a.rot[s + 1 - C]);
+  if constexpr (s + 2 < h) if (a.qstart + s + 2 - C < a.gate_limit) rot_reg<2>(v, a.rot[s + 2 - C]);
+  if constexpr (s + 3 < h) if (a.qstart + s + 3 - C < a.gate_limit) rot_reg<3>(v, a.rot[s + 3 - C]);
+}
+
+// Walk down the layouts from K applying the Walsh butterflies (ends at layout count-1).
+template <int TB, int C, int RBW, int K>
+__device__ __forceinline__ void walk_wht(float2 (&v)[16], float2* lds, uint32_t tl, uint32_t tr) {
+  using G = Geo<TB, C, RBW>;
+  wht_layout<G, K>(v);
+  if constexpr (K + 1 < G::count()) {
+    exchange<TB, C, RBW, G::at(K), G::at(K + 1)>(v, lds, tl, tr);
+    walk_wht<TB, C, RBW, K + 1>(v, lds, tl, tr);
+  }
+}
+
+// Rotations walking down from layout K (ends at layout count-1).
+template <int TB, int C, int RBW, int K>
+__device__ __forceinline__ void walk_rot_down(float2 (&v)[16], float2* lds, uint32_t tl, uint32_t tr,
+                                              const PassArgs& a) {
+  using G = Geo<TB, C, RBW>;
+  rot_layout<G, C, K>(v, a);
+  if constexpr (K + 1 < G::count()) {
+    exchange<TB, C, RBW, G::at(K), G::at(K + 1)>(v, lds, tl, tr);
+    walk_rot_down<TB, C, RBW, K + 1>(v, lds, tl, tr, a);
+  }
+}
+
+// Rotations walking up from layout K (ends at layout 0 = S1).
+template <int TB, int C, int RBW, int K>
+__device__ __forceinline__ void walk_rot_up(float2 (&v)[16], float2* lds, uint32_t tl, uint32_t tr,
+                                            const PassArgs& a) {
+  using G = Geo<TB, C, RBW>;
+  rot_layout<G, C, K>(v, a);
+  if constexpr (K > 0) {
+    exchange<TB, C, RBW, G::at(K), G::at(K - 1)>(v, lds, tl, tr);
+    walk_rot_up<TB, C, RBW, K - 1>(v, lds, tl, tr, a);
+  }
+}
+
+template <int TB, int C, int RBW>
+__global__ void __launch_bounds__(1 << (TB - 4)) k_tile(PassArgs a) {
+  using G = Geo<TB, C, RBW>;
+  constexpr int S1 = G::S1;
+  constexpr int SL = G::at(G::count() - 1);  // == C
+  extern __shared__ __align__(16) unsigned char smem[];
+  float2* lds = reinterpret_cast<float2*>(smem);
+  uint64_t* lut_phase = reinterpret_cast<uint64_t*>(smem + (size_t)G::slots * sizeof(float2));
+  float2* lut_d = reinterpret_cast<float2*>(lut_phase + (size_t)a.phase_chunks * 256);
+  uint64_t* lut_init = reinterpret_cast<uint64_t*>(lut_d + (size_t)a.dphase_chunks * 256);
+  float2* lut_w =
+      reinterpret_cast<float2*>(lut_init + (size_t)a.init_chunks * 256 * (a.init_kind == RGS_INIT_COV ? 2 : 1));
+  const uint32_t tid = threadIdx.x;
+  const uint32_t tl = tid & ((1u << G::TLB) - 1u);
+  const uint32_t tr = RBW ? __builtin_amdgcn_readfirstlane(tid >> G::TLB) : 0u;
+
+  const bool load = a.flags & F_LOAD;
+  bool staged = false;
+  if (a.phase_chunks) {
+    for (int i = tid; i < a.phase_chunks * 256; i += G::NT) lut_phase[i] = a.phase_lut[i];
+    staged = true;
+  }
+  if (a.dphase_chunks) {
+    for (int i = tid; i < a.dphase_chunks * 256; i += G::NT) lut_d[i] = a.dphase[i];
+    staged = true;
+  }
+  if (!load && a.init_chunks) {
+    const int words = a.init_chunks * 256 * (a.init_kind == RGS_INIT_COV ? 2 : 1);
+    for (int i = tid; i < words; i += G::NT) lut_init[i] = a.init_lut[i];
+    if (a.init_kind == RGS_INIT_COV)
+      for (int i = tid; i < a.init_chunks * 256; i += G::NT) lut_w[i] = a.init_w[i];
+    staged = true;
+  }
+  if (staged) __syncthreads();
+
+  const int mid = a.qstart - C;
+  const uint32_t cmask = (1u << C) - 1u;
+  const uint64_t blk = (uint64_t)blockIdx.x + ((uint64_t)blockIdx.y << 24);
+  const uint64_t base = ((blk & ((1ull << mid) - 1ull)) << C) | ((blk >> mid) << (a.qstart + TB - C));
+  auto gidx = [&](uint32_t l) -> uint64_t {
+    return base | (uint64_t)(l & cmask) | ((uint64_t)(l >> C) << a.qstart);
+  };
+  // the load/store layout S1: thread offset and the global stride of its registers
+  const uint64_t g0 = gidx(local_base<S1, C, RBW>(tl, tr));
+  const int gsh = a.qstart + S1 - C;
+
+  float2 v[16];
+  if (a.flags & F_LOAD) {
+#pragma unroll
+    for (int j = 0; j < 16; ++j) v[j] = a.state[g0 + ((uint64_t)j << gsh)];
+  } else if (a.init_kind == RGS_INIT_GRAPH) {
+    const uint32_t e = form_signs(g0, lut_init, a.init_chunks, a.iq, a.ifreg);
+#pragma unroll
+    for (int j = 0; j < 16; ++j) v[j] = cneg_if(make_float2(a.init_scale, 0.f), (e >> j) & 1u);
+  } else {
+#pragma unroll
+    for (int j = 0; j < 16; ++j) v[j] = init_value(g0 + ((uint64_t)j << gsh), a, lut_init, lut_w);
+  }
+
+  // Diagonal phases on the registers of layout S.  Both are separable over
+  // the register bits: the sign through form_signs, the complex phase as the
+  // thread's base phase times the single-bit phases of the register bits.
+  auto apply_phase = [&](auto s_tag) {
+    constexpr int S = decltype(s_tag)::value;
+    const uint64_t gb = gidx(local_base<S, C, RBW>(tl, tr));
+    const uint32_t sh = a.qstart + S - C;
+    if (a.dphase_chunks) {
+      float2 ph[16];
+      ph[0] = dphase_of(gb, lut_d, a.dphase_chunks);
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const uint32_t p = sh + i;
+        const float2 f = lut_d[((p >> 3) << 8) | (1u << (p & 7))];
+#pragma unroll
+        for (int j = 1 << i; j < (2 << i); ++j) ph[j] = cmul(ph[j - (1 << i)], f);
+      }
+#pragma unroll
+      for (int j = 0; j < 16; ++j) v[j] = cmul(v[j], ph[j]);
+    }
+    if (a.phase_chunks) {
+      const uint32_t e = form_signs(gb, lut_phase, a.phase_chunks, a.pq, a.pfreg);
+#pragma unroll
+      for (int j = 0; j < 16; ++j) v[j] = cneg_if(v[j], (e >> j) & 1u);
+    }
+  };
+
+  // After the walks the registers are in layout S1 (top) or SL (bottom).
+  bool at_top;
+  if (a.flags & F_WHT) {
+    walk_wht<TB, C, RBW, 0>(v, lds, tl, tr);
+    if (a.flags & F_PHASE) apply_phase(std::integral_constant<int, SL>());
+    if (a.flags & F_GATES) {
+      walk_rot_up<TB, C, RBW, G::count() - 1>(v, lds, tl, tr, a);
+      at_top = true;
+    } else {
+      at_top = (SL == S1);
+    }
+  } else {
+    if (a.flags & F_PHASE) apply_phase(std::integral_constant<int, S1>());
+    if (a.flags & F_GATES) {
+      walk_rot_down<TB, C, RBW, 0>(v, lds, tl, tr, a);
+      at_top = (SL == S1);
+    } else {
+      at_top = true;
+    }
+  }
+  if constexpr (C == 0) {
+    if (a.flags & F_MASS) {
+      // registers must hold local bits 0..3 (layout SL = 0)
+      if (at_top) exchange<TB, C, RBW, S1, SL>(v, lds, tl, tr);
+      at_top = (SL == S1);
+      double p[16];
+#pragma unroll
+      for (int j = 0; j < 16; ++j) p[j] = prob64(v[j]);
+#pragma unroll
+      for (int w = 1; w < 16; w <<= 1) {
+#pragma unroll
+        for (int j = 0; j < 16; j += 2 * w) p[j] = __dadd_rn(p[j], p[j + w]);
+      }
+      double m = p[0];
+      const int sb = a.mass_bits - 4;
+      for (int b = 0; b < sb; ++b) m = __dadd_rn(m, __shfl_xor(m, 1 << b));
+      if ((tid & ((1u << sb) - 1u)) == 0) a.mass[(base >> a.mass_bits) + (tid >> sb)] = m;
+    }
+  }
+  if (!at_top) exchange<TB, C, RBW, SL, S1>(v, lds, tl, tr);
+#pragma unroll
+  for (int j = 0; j < 16; ++j) a.state[g0 + ((uint64_t)j << gsh)] = v[j];
+}
+
+// One workgroup per tile.
+template <int TB, int C, int RBW>
+hipError_t launch_tile(const PassArgs& a, size_t shmem, hipStream_t st) {
+  static bool attr_set = false;
+  if (!attr_set) {
+    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_tile<TB, C, RBW>),
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)kLdsBudget);
+    if (e != hipSuccess) return e;
+    attr_set = true;
+  }
+  const uint64_t tiles = 1ull << (a.nbits - TB);
+  dim3 grid((unsigned)(tiles < (1ull << 24) ? tiles : (1ull << 24)), (unsigned)(tiles >> 24 ? tiles >> 24 : 1));
+  hipLaunchKernelGGL((k_tile<TB, C, RBW>), grid, dim3(1 << (TB - 4)), shmem, st, a);
+  return hipGetLastError();
+}
+
+// Dispatch over the instantiated specialisations of one lane-bit count
+// (defined in slab_k<C>.hip).
+hipError_t launch_c0(int tb, int rbw, const PassArgs& a, size_t shmem, hipStream_t st);
+hipError_t launch_c2(int tb, int rbw, const PassArgs& a, size_t shmem, hipStream_t st);
+hipError_t launch_c3(int tb, int rbw, const PassArgs& a, size_t shmem, hipStream_t st);
+hipError_t launch_c4(int tb, int rbw, const PassArgs& a, size_t shmem, hipStream_t st);
+hipError_t launch_c5(int tb, int rbw, const PassArgs& a, size_t shmem, hipStream_t st);
+hipError_t launch_c6(int tb, int rbw, const PassArgs& a, size_t shmem, hipStream_t st);
+hipError_t launch_c7(int tb, int rbw, const PassArgs& a, size_t shmem, hipStream_t st);
+
+}  // namespace rgs

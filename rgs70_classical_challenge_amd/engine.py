@@ -40,8 +40,32 @@ def _torch():
 
 
 def default_tuning(nbits):
-    """Tile choices used when no tuned plan is given."""
-    return dict(low_bits=min(14, nbits), tile_bits=14, lane_bits=4)
+    """Tile choices used when neither the caller nor the autotuner table gives
+    one (the fastest measured full pipeline at n = 34: a 12-bit contiguous
+    pass, strided passes of 14-bit tiles with 64-amplitude rows and their LDS
+    exchanges in four rounds)."""
+    return dict(low_bits=min(12, nbits), tile_bits=14, lane_bits=6, round_bits=2)
+
+
+def resolve_tuning(nbits, device_name, tuning=None):
+    """Caller's tuning, else the autotuner's entry for this device and size,
+    else default_tuning; missing keys are filled from default_tuning."""
+    if tuning is None:
+        from .autotune import lookup_name
+        entry = lookup_name(nbits, device_name)
+        if entry is not None:
+            tuning = {k: entry[k] for k in ('low_bits', 'tile_bits', 'lane_bits', 'round_bits') if k in entry}
+    return dict(default_tuning(nbits), **(tuning or {}))
+
+
+def effective_low_bits(nbits, low_bits):
+    """Bits of the contiguous pass as the pass planner sets them: at most
+    nbits, and narrowed so that a strided group keeps a full register window
+    (4 bits)."""
+    low = min(low_bits, nbits)
+    if nbits > low and nbits - low < 4:
+        low = nbits - 4
+    return low
 
 
 class _Slab:
@@ -59,9 +83,9 @@ class _Slab:
         need = (8 << nbits) + (16 << max(0, nbits - mass_bits)) + (64 << 20)
         if free < need:
             raise MemoryError(f'device {device}: need {need} bytes, free {free}')
-        self.tuning = dict(default_tuning(nbits), **(tuning or {}))
+        self.tuning = resolve_tuning(nbits, torch.cuda.get_device_name(self.device), tuning)
         self.state = torch.empty(1 << nbits, dtype=torch.complex64, device=self.device)
-        self.mass_bits = max(4, min(mass_bits, 8, nbits, self.tuning['low_bits']))
+        self.mass_bits = max(4, min(mass_bits, 8, nbits, effective_low_bits(nbits, self.tuning['low_bits'])))
         self.base_bits = nbits - self.mass_bits
         self.pyramid = torch.empty((2 << self.base_bits) - 1, dtype=torch.float64, device=self.device)
         self.stream = torch.cuda.current_stream(self.device)
@@ -73,17 +97,21 @@ class _Slab:
         return t.to(self.device, non_blocking=False)
 
     def desc(self, init_kind, init_scale=0.0, init_lut=None, init_wlut=None, walsh=False, phase_lut=None,
-             gates=None, gate_limit=None, mass=True):
-        """gates: a Gates object (rotations + diagonal phase) or None."""
+             gates=None, gate_limit=None, mass=True, init_rows=None, phase_rows=None):
+        """gates: a Gates object (rotations + diagonal phase) or None;
+        init_rows / phase_rows: host form rows (form_rows) of the graph-phase
+        tables init_lut (GRAPH init) and phase_lut."""
         d = _lib.SlabDesc()
         d.state = ptr(self.state)
         d.nbits = self.nbits
         d.init_kind = init_kind
         d.init_scale = init_scale
         d.init_lut = ptr(init_lut)
+        d.init_rows = ptr(init_rows)
         d.init_wlut = ptr(init_wlut)
         d.do_walsh = int(walsh)
         d.phase_lut = ptr(phase_lut)
+        d.phase_rows = ptr(phase_rows)
         d.dphase_lut = ptr(gates.dphase) if gates is not None else None
         d.rot = ptr(gates.rot) if gates is not None else None
         d.gate_limit = self.nbits if gate_limit is None else gate_limit
@@ -92,8 +120,9 @@ class _Slab:
         d.low_bits = self.tuning['low_bits']
         d.tile_bits = self.tuning['tile_bits']
         d.lane_bits = self.tuning['lane_bits']
+        d.round_bits = self.tuning.get('round_bits', 0)
         # the descriptor holds raw pointers: keep their owners alive with it
-        d._owners = (self.state, self.pyramid, init_lut, init_wlut, phase_lut, gates)
+        d._owners = (self.state, self.pyramid, init_lut, init_wlut, phase_lut, gates, init_rows, phase_rows)
         return d
 
     def run(self, d, mass=True):
@@ -172,6 +201,13 @@ class Gates:
         return v
 
 
+def form_rows(g):
+    """Host rows of the graph phase form: bit b of row a = g[a][b] (uint64)."""
+    g = np.asarray(g, np.uint8)
+    w = (np.uint64(1) << np.arange(len(g), dtype=np.uint64))
+    return np.ascontiguousarray((g.astype(np.uint64) * w[None, :]).sum(axis=1, dtype=np.uint64))
+
+
 def graph_phase_lut(g):
     g = np.ascontiguousarray(np.asarray(g, np.uint8))
     n = len(g)
@@ -214,14 +250,18 @@ class GraphStateSampler:
         self.slab = _Slab(n, device, mass_bits, tuning)
         sl = self.slab
         self._lut = sl.to_device(graph_phase_lut(g))
+        self._rows = form_rows(g)
         self._gates = Gates(sl, u @ HADAMARD if pipeline == 'slab' else u)
         if pipeline == 'slab':
-            self._desc = sl.desc(_lib.INIT_GRAPH, 2.0 ** (-n), init_lut=self._lut, walsh=True, gates=self._gates)
+            self._desc = sl.desc(_lib.INIT_GRAPH, 2.0 ** (-n), init_lut=self._lut, init_rows=self._rows, walsh=True,
+                                 gates=self._gates)
         elif pipeline == 'literal':
             # H^n = WHT / 2^{n/2}: the delta carries the normalisation
-            self._desc = sl.desc(_lib.INIT_DELTA, 2.0 ** (-n / 2), walsh=True, phase_lut=self._lut, gates=self._gates)
+            self._desc = sl.desc(_lib.INIT_DELTA, 2.0 ** (-n / 2), walsh=True, phase_lut=self._lut,
+                                 phase_rows=self._rows, gates=self._gates)
         else:
-            self._desc = sl.desc(_lib.INIT_GRAPH, 2.0 ** (-n / 2), init_lut=self._lut, gates=self._gates)
+            self._desc = sl.desc(_lib.INIT_GRAPH, 2.0 ** (-n / 2), init_lut=self._lut, init_rows=self._rows,
+                                 gates=self._gates)
         self.passes, self.plan_text = sl.describe(self._desc)
         self._norm = None
 
@@ -320,6 +360,7 @@ class ConditionalSampler:
         call('rgs_cov_luts', ptr(g), len(g), ptr(Aa), ptr(Ba), m, ptr(map_lut), ptr(aphase))
         self._map = sl.to_device(map_lut)
         self._aphase = sl.to_device(aphase)
+        self._arows = form_rows(g[np.ix_(A, A)])
         self._Ba = Ba
         self._u = np.ascontiguousarray(u)
         self._gates = Gates(sl, u[A], m - 1 if defer_top else m)
@@ -328,7 +369,7 @@ class ConditionalSampler:
         self.deferred_global_unitary = _check_unitary(u[A[-1]]) if defer_top else None
         self.n = 1 << (m - 1)          # shard size of the reference's two-device layout
         self._desc = sl.desc(_lib.INIT_COV, 0.0, init_lut=self._map, init_wlut=self._wlut, walsh=True,
-                             phase_lut=self._aphase, gates=self._gates, gate_limit=m - 1 if defer_top else m)
+                             phase_lut=self._aphase, phase_rows=self._arows, gates=self._gates, gate_limit=m - 1 if defer_top else m)
         self.passes, self.plan_text = sl.describe(self._desc)
 
     def compute(self, outB):

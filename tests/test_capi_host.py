@@ -101,7 +101,8 @@ def test_cov_luts_reject_singular_cut():
         _lib.call('rgs_cov_luts', _lib.ptr(g), 8, _lib.ptr(A), _lib.ptr(B), 4, _lib.ptr(out), None)
 
 
-def describe(nbits, walsh, gates=True, phase=False, mass=True, init=_lib.INIT_GRAPH, low=14, tile=14, lane=4):
+def describe(nbits, walsh, gates=True, phase=False, mass=True, init=_lib.INIT_GRAPH, low=14, tile=14, lane=4,
+             rounds=0):
     d = _lib.SlabDesc()
     d.state = 1
     d.nbits = nbits
@@ -113,7 +114,7 @@ def describe(nbits, walsh, gates=True, phase=False, mass=True, init=_lib.INIT_GR
     d.gate_limit = nbits
     d.mass = 1 if mass else None
     d.mass_bits = 8
-    d.low_bits, d.tile_bits, d.lane_bits = low, tile, lane
+    d.low_bits, d.tile_bits, d.lane_bits, d.round_bits = low, tile, lane, rounds
     n = ctypes.c_int()
     buf = ctypes.create_string_buffer(4096)
     _lib.call('rgs_slab_describe', ctypes.byref(d), ctypes.byref(n), buf, 4096)
@@ -134,3 +135,12 @@ def test_pass_schedules():
     assert n == 1 + 2 + 1 + 2 + 1, text
     with pytest.raises(ValueError):
         describe(34, walsh=True, tile=15)
+    with pytest.raises(ValueError):
+        describe(34, walsh=True, lane=8)           # no kernel for 8 lane bits
+    with pytest.raises(ValueError):
+        describe(34, walsh=True, rounds=3)
+    # LDS rounds: at most the lane bits, two, and tile bits - 10
+    n, text = describe(34, walsh=True, rounds=2)
+    assert all(tok.endswith('rb2') for tok in text.split() if tok.startswith('strided')), text
+    n, text = describe(20, walsh=True, low=10, tile=12, lane=4, rounds=2)
+    assert all(tok.endswith('rb0') for tok in text.split() if tok.startswith('strided')), text  # 9-bit tiles

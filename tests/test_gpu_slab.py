@@ -68,12 +68,18 @@ def check_pyramid_and_draws(sampler, ref, uniforms):
     margin = np.minimum(t - C[ref_idx], C[ref_idx + 1] - t)
     bound = uniforms * dR + np.maximum(dC[ref_idx], dC[ref_idx + 1]) + 1e-12 * R
     ok = margin > bound
-    assert ok.mean() > 0.95, ok.mean()
+    # coverage: with 2^n bins the typical bin is 2^-n wide, so the share of
+    # draws whose margin beats the FP32 error bound falls as n grows
+    n = int(np.log2(len(ref)))
+    assert ok.mean() > (0.9 if n <= 16 else 0.4), ok.mean()
     assert np.array_equal(idx[ok], ref_idx[ok])
 
 
 TUNINGS = [None, dict(low_bits=8, tile_bits=8, lane_bits=3), dict(low_bits=10, tile_bits=9, lane_bits=2),
-           dict(low_bits=12, tile_bits=12, lane_bits=5)]
+           dict(low_bits=12, tile_bits=12, lane_bits=5),
+           # LDS rounds in the strided passes (engaged where the group is wide enough)
+           dict(low_bits=10, tile_bits=12, lane_bits=4, round_bits=2),
+           dict(low_bits=12, tile_bits=14, lane_bits=5, round_bits=2)]
 
 
 @pytest.mark.parametrize('pipeline', ['slab', 'literal', 'folded'])
