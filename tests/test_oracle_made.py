@@ -66,3 +66,24 @@ def test_store_hook_identity_and_rounding():
     assert l0 == l1 and all(np.array_equal(a, b) for a, b in zip(g0, g1))
     lq, gq = om.loss_and_grads(params, masks, x, store=om.bf16_round)
     assert lq != l0 and abs(lq - l0) < 1e-2 * abs(l0)
+
+
+def test_adam_closed_forms():
+    # first step (m = v = 0, t = 1): bias correction gives m_hat = g, v_hat = g^2,
+    # so every weight moves by lr * g / (|g| + eps), i.e. lr * sign(g) for |g| >> eps
+    rng = np.random.default_rng(4)
+    w = rng.normal(size=50)
+    g = rng.normal(size=50)
+    w1, m1, v1 = om.adam_step(w, g, 0.0, 0.0, 1, lr=1e-3)
+    assert np.allclose(w - w1, 1e-3 * g / (np.abs(g) + 1e-8), rtol=1e-12)
+    assert np.allclose(m1, 0.1 * g) and np.allclose(v1, 0.001 * g * g)
+    # a constant gradient keeps m_hat = g and v_hat = g^2 at every step
+    m = v = 0.0
+    wt = w.copy()
+    for t in range(1, 6):
+        wt_new, m, v = om.adam_step(wt, g, m, v, t, lr=1e-3)
+        assert np.allclose(wt - wt_new, 1e-3 * g / (np.abs(g) + 1e-8), rtol=1e-9)
+        wt = wt_new
+    # zero gradient from zero moments leaves the weights alone
+    w0, _, _ = om.adam_step(w, np.zeros_like(w), 0.0, 0.0, 1)
+    assert np.array_equal(w0, w)
