@@ -117,10 +117,11 @@ typedef struct rgs_slab_desc {
   int mass_bits;               /* granularity of the mass level (4..10)     */
   int low_bits;                /* tile bits of the contiguous pass (6..14)  */
   int tile_bits;               /* tile bits of strided passes (6..14)       */
-  int lane_bits;               /* contiguous bits per strided row (>= 1)    */
+  int lane_bits;               /* contiguous bits per strided row (2..7)    */
   int round_bits;              /* strided passes stage 2^-round_bits of the
-                                  tile in LDS per round (0..4, <= lane_bits;
-                                  fewer LDS bytes, more workgroups per CU)  */
+                                  tile in LDS per round (0..2, applied where
+                                  <= lane_bits and <= tile_bits - 10; fewer
+                                  LDS bytes, more workgroups per CU)        */
 } rgs_slab_desc;
 
 /* Factor 2x2 unitaries u (complex128 [n][2][2]) as u = D1 R D2 with
@@ -230,12 +231,37 @@ int rgs_scan_pack(const rgs_scan_step *steps, int n, uint64_t *blob, uint64_t *w
 int rgs_scan_batch(const uint64_t *plan_dev, int rmax, const double *u_dev, const uint64_t *bits_dev,
                    int count, int precision, double *out_val, int32_t *out_exp, void *stream);
 
+/* Byte-table form of a plan for rgs_scan_large (ranks <= 32): per step a
+ * header, beta, ceil(r_out/8) x 256 words of XORs of the lcols and the kernel
+ * combinations; blob[2 + j] is the word offset of step j.  Call with
+ * blob == NULL to get the size in *words.  Replaces byte_tables /
+ * kernel_combos (upstream_scan/scan_plan.py:133-152). */
+int rgs_scan_pack_tables(const rgs_scan_step *steps, int n, uint64_t *blob, uint64_t *words);
+
+/* p(x) of ONE proposal with the environment tables in HBM, for ranks up to
+ * 32: one launch per step over 2^r_out outputs, no host round trip between
+ * steps (the per-step power-of-two scale is a device maximum applied by the
+ * next step).  steps: the compiled plan (host); plan_dev: the packed tables
+ * (rgs_scan_pack_tables) on the device; plan_offsets: host copy of blob[2..]
+ * (the step offsets); u_host: complex128 [n][2][2]; bits_host: uint64[2];
+ * buf0 / buf1: device tables of buf_entries >= 2^rmax entries each
+ * (complex64 for precision 64, complex128 for 128); mx_dev: device uint64[n]
+ * scratch.  Outputs on the device: out_val complex128[1], out_exp int32[1]
+ * with amplitude = out_val * 2^out_exp * 2^{-n/2}.  Replaces GPUMixin._sweep /
+ * scan_step_scaled at large rank (upstream_scan/gpu_kernel.py:59-102,125-170,
+ * gpu_kernel_fp64.py:24-67,90-135), which read 4 bytes back to the host after
+ * every step. */
+int rgs_scan_large(const rgs_scan_step *steps, int n, const uint64_t *plan_dev, const uint64_t *plan_offsets,
+                   const double *u_host, const uint64_t *bits_host, int precision, void *buf0, void *buf1,
+                   uint64_t buf_entries, uint64_t *mx_dev, double *out_val, int32_t *out_exp, void *stream);
+
 /* ---- learned proposal (MADE) -------------------------------------------
  * The reference has no counterpart for any entry point below: the learned
  * proposal is new work.  All matrices are row-major; bf16 values are raw
  * uint16 bit patterns (round-to-nearest-even from FP32). */
 
-/* C[M][N] (+)= alpha * op(A) op(B) with op = transpose when ta / tb is 1;
+/* C[M][N] (+)= alpha * op(A) op(B) with op = transpose when ta / tb is 1
+ * (K may be 0: the product is empty and only the epilogue applies);
  * A is [M][K] (or [K][M]), B is [K][N] (or [N][K]), bf16, FP32 MFMA
  * accumulation.  Epilogue in order: + bias[N] (may be NULL), ReLU if relu,
  * multiply by (dmask[i][j] != 0) if dmask is non-NULL (ReLU backward), then
@@ -265,6 +291,13 @@ int rgs_adam(float *w, const float *grad, const float *mask, float *m, float *v,
 /* Ancestral sampling of column o: x[b][o] = (u[b][o] < sigmoid(logits[b][o]))
  * with u an FP64 [B][n] array of uniforms; x holds bf16 1.0 / 0.0. */
 int rgs_sample_col(const float *logits, const double *u, int B, int n, int o, uint16_t *x, void *stream);
+
+/* One step of incremental autoregressive sampling: x[b][o] = (u[b][o] <
+ * sigmoid(logit[b * ldl])) for b < B (u FP64 [B][n], x bf16 [B][n]), and
+ * logq[b] = (first ? 0 : logq[b]) + log sigmoid(+-logit) of the chosen bit
+ * (FP64). */
+int rgs_made_sample_step(const float *logit, int ldl, const double *u, int B, int n, int o, uint16_t *x,
+                         double *logq, int first, void *stream);
 
 /* bf16 copy (round to nearest even) of an FP32 array. */
 int rgs_f32_to_bf16(const float *in, uint16_t *out, long long count, void *stream);

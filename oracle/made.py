@@ -14,13 +14,13 @@ import numpy as np
 
 
 def made_masks(n, hidden, seed=0):
-    """Degrees: inputs 1..n in order, hidden units cycle over 1..n-1 (seeded
-    shuffle); a hidden unit of degree d sees inputs of degree <= d and output o
-    (degree o+1) sees hidden units of degree < o+1."""
-    rng = np.random.default_rng(seed)
+    """Degrees: inputs 1..n in order, hidden units take 1..n-1 evenly in
+    sorted order (``seed`` is kept for the signature; the assignment is
+    deterministic); a hidden unit of degree d sees inputs of degree <= d and
+    output o (degree o+1) sees hidden units of degree < o+1."""
     deg_in = np.arange(1, n + 1)
-    d1 = rng.permutation(np.arange(hidden) % max(1, n - 1) + 1)
-    d2 = rng.permutation(np.arange(hidden) % max(1, n - 1) + 1)
+    d1 = hidden_degrees(n, hidden)
+    d2 = hidden_degrees(n, hidden)
     m1 = (d1[:, None] >= deg_in[None, :]).astype(np.float64)     # [H, n]
     m2 = (d2[:, None] >= d1[None, :]).astype(np.float64)         # [H, H]
     m3 = (deg_in[:, None] > d2[None, :]).astype(np.float64)      # [n, H]
@@ -37,6 +37,11 @@ def bf16_round(a):
 
 def _exact(a):
     return a
+
+
+def hidden_degrees(n, hidden):
+    """Sorted hidden-unit degrees: unit i has degree floor(i (n-1) / hidden) + 1."""
+    return (np.arange(hidden) * max(1, n - 1)) // hidden + 1
 
 
 def forward(params, masks, x, store=_exact):

@@ -97,27 +97,34 @@ class BatchSampler:
             torch.cuda.empty_cache()
         return out
 
+    def _capture(self, items):
+        """One graph for a wave: fork onto the side streams, launch every
+        instance, join back."""
+        torch = self.torch
+        main = torch.cuda.Stream()
+        graph = torch.cuda.CUDAGraph()
+        # warm the launch path once outside capture (one-time attribute setup)
+        for k, it in enumerate(items):
+            self._launch(it, self.streams[k % len(self.streams)])
+        torch.cuda.synchronize()
+        with torch.cuda.stream(main):
+            graph.capture_begin()
+            for st in self.streams:
+                st.wait_stream(main)
+            for k, it in enumerate(items):
+                self._launch(it, self.streams[k % len(self.streams)])
+            for st in self.streams:
+                main.wait_stream(st)
+            graph.capture_end()
+        return graph
+
     def run_graph(self, replays=1, keep=False):
         """Capture each wave as one multi-stream graph and replay it."""
         torch = self.torch
         out = {}
         for wi, wave in enumerate(self.waves):
             items = self._build(wave)
-            main = torch.cuda.Stream()
-            graph = torch.cuda.CUDAGraph()
-            # warm the launch path once outside capture (one-time attribute setup)
-            for k, it in enumerate(items):
-                self._launch(it, self.streams[k % len(self.streams)])
-            torch.cuda.synchronize()
-            with torch.cuda.stream(main):
-                graph.capture_begin()
-                for st in self.streams:
-                    st.wait_stream(main)
-                for k, it in enumerate(items):
-                    self._launch(it, self.streams[k % len(self.streams)])
-                for st in self.streams:
-                    main.wait_stream(st)
-                graph.capture_end()
+            graph = self._capture(items)
             for _ in range(replays):
                 graph.replay()
             torch.cuda.synchronize()
@@ -129,3 +136,58 @@ class BatchSampler:
                 del graph, items
                 torch.cuda.empty_cache()
         return out
+
+    def benchmark(self, reps=3):
+        """Device time of the whole batch (state, masses, draws of every
+        instance): one launch sequence per instance on one stream, against
+        one graph replay per wave.  Building the instances (allocation and
+        host tables) is outside both timings."""
+        import time
+        torch = self.torch
+        eager_s = graph_s = 0.0
+        for wave in self.waves:
+            items = self._build(wave)
+            cur = torch.cuda.current_stream()
+            for it in items:
+                self._launch(it, cur)
+            torch.cuda.synchronize()
+            t = time.perf_counter()
+            for _ in range(reps):
+                for it in items:
+                    self._launch(it, cur)
+            torch.cuda.synchronize()
+            eager_s += (time.perf_counter() - t) / reps
+            graph = self._capture(items)
+            graph.replay()
+            torch.cuda.synchronize()
+            t = time.perf_counter()
+            for _ in range(reps):
+                graph.replay()
+            torch.cuda.synchronize()
+            graph_s += (time.perf_counter() - t) / reps
+            del graph, items
+            torch.cuda.empty_cache()
+        total = len(self.specs) * self.samples
+        amps = sum(1 << n for n, _ in self.specs)
+        return {'instances': len(self.specs), 'samples_per_instance': self.samples, 'waves': len(self.waves),
+                'streams': len(self.streams), 'amplitudes': amps,
+                'eager_ms': round(eager_s * 1e3, 3), 'graph_ms': round(graph_s * 1e3, 3),
+                'eager_samples_per_s': round(total / eager_s, 1), 'graph_samples_per_s': round(total / graph_s, 1),
+                'graph_speedup': round(eager_s / graph_s, 3)}
+
+
+def main(argv):
+    """python -m rgs70_classical_challenge_amd.batch [count] [nmin] [nmax]: one JSON line."""
+    import json
+    count = int(argv[0]) if argv else 256
+    nmin = int(argv[1]) if len(argv) > 1 else 16
+    nmax = int(argv[2]) if len(argv) > 2 else 30
+    bs = BatchSampler(uneven_instances(count, nmin=nmin, nmax=nmax), samples=1000)
+    res = bs.benchmark()
+    res['workload'] = f'{count} instances, n uniform in {nmin}..{nmax}, 1000 samples each'
+    print(json.dumps(res), flush=True)
+
+
+if __name__ == '__main__':
+    import sys
+    main(sys.argv[1:])

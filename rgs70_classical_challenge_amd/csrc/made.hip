@@ -187,6 +187,23 @@ __global__ void k_sample_col(const float* logits, const double* u, int B, int n,
   x[(size_t)r * n + o] = u[(size_t)r * n + o] < p ? (uint16_t)0x3f80 : (uint16_t)0;  // bf16 1.0 / 0.0
 }
 
+// One autoregressive step: x[b][o] = (u[b][o] < sigmoid(l[b])), and log q
+// accumulates log sigmoid(+-l[b]) of the chosen bit (in float64, from the
+// same float logit that decided the bit).
+__global__ void k_sample_step(const float* logit, int ldl, const double* u, int B, int n, int o, uint16_t* x,
+                              double* logq, int first) {
+  int r = blockIdx.x * blockDim.x + threadIdx.x;
+  if (r >= B) return;
+  const double l = (double)logit[(size_t)r * ldl];
+  const double p = 1.0 / (1.0 + exp(-l));
+  const bool one = u[(size_t)r * n + o] < p;
+  // log sigmoid(s) = -log1p(exp(-s)) for s >= 0, s - log1p(exp(s)) otherwise
+  const double s = one ? l : -l;
+  const double ls = s >= 0.0 ? -log1p(exp(-s)) : s - log1p(exp(s));
+  x[(size_t)r * n + o] = one ? (uint16_t)0x3f80 : (uint16_t)0;
+  logq[r] = first ? ls : logq[r] + ls;
+}
+
 __global__ void k_f32_to_bf16(const float* in, uint16_t* out, long long count) {
   long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < count) out[i] = f32_to_bf16(in[i]);
@@ -199,7 +216,8 @@ using namespace rgs;
 extern "C" int rgs_gemm_bf16(int ta, int tb, int M, int N, int K, const uint16_t* A, int lda, const uint16_t* B,
                              int ldb, float alpha, float* C, int ldc, uint16_t* Cb, int ldcb, const float* bias,
                              int relu, const uint16_t* dmask, int ldm, int accumulate, void* stream) {
-  if (!A || !B || (!C && !Cb) || M <= 0 || N <= 0 || K <= 0) return fail(kInvalid, "gemm: bad arguments");
+  // K = 0 is allowed: the product is empty and the epilogue sees zeros (bias only)
+  if (!A || !B || (!C && !Cb) || M <= 0 || N <= 0 || K < 0) return fail(kInvalid, "gemm: bad arguments");
   GemmArgs g{A, B, C, Cb, bias, dmask, M, N, K, lda, ldb, ldc, ldcb, ldm, ta, tb, relu, accumulate, alpha};
   dim3 grid((N + TN - 1) / TN, (M + TM - 1) / TM);
   hipLaunchKernelGGL(k_gemm, grid, dim3(256), 0, reinterpret_cast<hipStream_t>(stream), g);
@@ -257,5 +275,15 @@ extern "C" int rgs_f32_to_bf16(const float* in, uint16_t* out, long long count, 
   hipLaunchKernelGGL(k_f32_to_bf16, dim3((unsigned)((count + 255) / 256)), dim3(256), 0,
                      reinterpret_cast<hipStream_t>(stream), in, out, count);
   RGS_LAUNCH_CHECK("k_f32_to_bf16");
+  return kOk;
+}
+
+extern "C" int rgs_made_sample_step(const float* logit, int ldl, const double* u, int B, int n, int o, uint16_t* x,
+                                    double* logq, int first, void* stream) {
+  if (!logit || !u || !x || !logq || B <= 0 || o < 0 || o >= n || ldl <= 0)
+    return fail(kInvalid, "made_sample_step: bad arguments");
+  hipLaunchKernelGGL(k_sample_step, dim3((B + 255) / 256), dim3(256), 0, reinterpret_cast<hipStream_t>(stream),
+                     logit, ldl, u, B, n, o, x, logq, first);
+  RGS_LAUNCH_CHECK("k_sample_step");
   return kOk;
 }

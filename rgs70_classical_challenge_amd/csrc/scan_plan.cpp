@@ -306,3 +306,46 @@ extern "C" int rgs_scan_pack(const rgs_scan_step* steps, int n, uint64_t* blob, 
   *words = w;
   return kOk;
 }
+
+// Byte-table form of a plan for the HBM-table scan: per step a header
+// (v | r_in << 8 | r_out << 16 | nker << 24 | nbytes << 32), beta, nbytes x 256
+// words (word [b][k] = XOR of the lcols selected by bits of byte b = k) and
+// the 2^nker kernel combinations.
+extern "C" int rgs_scan_pack_tables(const rgs_scan_step* steps, int n, uint64_t* blob, uint64_t* words) {
+  if (!steps || n < 1 || n > 128 || !words) return fail(kInvalid, "scan_pack_tables: bad arguments");
+  uint64_t w = 2 + (uint64_t)n;
+  int rmax = 0;
+  for (int j = 0; j < n; ++j) {
+    const rgs_scan_step& st = steps[j];
+    if (st.r_out > 32 || st.r_in > 32 || st.nker > 16 || st.r_out < 0 || st.r_in < 0 || st.nker < 0)
+      return fail(kInvalid, "scan_pack_tables: rank above 32 or kernel above 16");
+    rmax = std::max(rmax, std::max(st.r_out, st.r_in));
+    const int nbytes = (st.r_out + 7) / 8;
+    const uint64_t off = w;
+    w += 2 + 256ull * nbytes + (1ull << st.nker);
+    if (blob) {
+      blob[2 + j] = off;
+      uint64_t* p = blob + off;
+      p[0] = (uint64_t)st.v | ((uint64_t)st.r_in << 8) | ((uint64_t)st.r_out << 16) | ((uint64_t)st.nker << 24) |
+             ((uint64_t)nbytes << 32);
+      p[1] = st.beta;
+      for (int b = 0; b < nbytes; ++b)
+        for (int k = 0; k < 256; ++k) {
+          uint64_t x = 0;
+          for (int i = 0; i < 8 && 8 * b + i < st.r_out; ++i)
+            if ((k >> i) & 1) x ^= st.lcols[8 * b + i];
+          p[2 + 256 * b + k] = x;
+        }
+      uint64_t* cb = p + 2 + 256ull * nbytes;
+      cb[0] = 0;
+      for (int k = 0; k < st.nker; ++k)
+        for (uint64_t t = 0; t < (1ull << k); ++t) cb[(1ull << k) + t] = cb[t] ^ st.kern[k];
+    }
+  }
+  if (blob) {
+    blob[0] = (uint64_t)n;
+    blob[1] = (uint64_t)rmax;
+  }
+  *words = w;
+  return kOk;
+}

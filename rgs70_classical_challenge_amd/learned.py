@@ -96,3 +96,22 @@ def run(n=34, seed=20261205, train_samples=200_000, steps=20_000, batch=512, hid
     else:
         out['equal_xeb_comparison'] = None
     return out
+
+
+def main(argv):
+    """python -m rgs70_classical_challenge_amd.learned [n] [steps] [out.json]"""
+    import json
+    import sys
+    n = int(argv[0]) if argv else 34
+    steps = int(argv[1]) if len(argv) > 1 else 20_000
+    res = run(n=n, steps=steps, log=lambda *a: print(*a, file=sys.stderr, flush=True))
+    line = json.dumps(res)
+    if len(argv) > 2:
+        with open(argv[2], 'w') as f:
+            f.write(line + '\n')
+    print(line, flush=True)
+
+
+if __name__ == '__main__':
+    import sys
+    main(sys.argv[1:])

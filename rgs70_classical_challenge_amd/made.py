@@ -28,15 +28,23 @@ _lib.register('rgs_colsum_bf16', _I, [_V, _I, _I, _I, _V, _V])
 _lib.register('rgs_adam', _I, [_V, _V, _V, _V, _V, _V, ctypes.c_longlong, _V, _F, _F, _F, _F, _V])
 _lib.register('rgs_sample_col', _I, [_V, _V, _I, _I, _I, _V, _V])
 _lib.register('rgs_f32_to_bf16', _I, [_V, _V, ctypes.c_longlong, _V])
+_lib.register('rgs_made_sample_step', _I, [_V, _I, _V, _I, _I, _I, _V, _V, _I, _V])
+
+
+def hidden_degrees(n, hidden):
+    """Sorted hidden-unit degrees: unit i has degree floor(i (n-1) / hidden) + 1,
+    so the units of degree <= d form a prefix (incremental sampling relies on
+    it)."""
+    return (np.arange(hidden) * max(1, n - 1)) // hidden + 1
 
 
 def made_masks(n, hidden, seed=0):
-    """Degrees: inputs 1..n; hidden degrees cycle over 1..n-1 (seeded shuffle).
+    """Degrees: inputs 1..n; hidden degrees 1..n-1, evenly and in sorted order
+    (``seed`` is kept for the signature; the assignment is deterministic).
     Returns float masks [H,n], [H,H], [n,H]."""
-    rng = np.random.default_rng(seed)
     deg_in = np.arange(1, n + 1)
-    d1 = rng.permutation(np.arange(hidden) % max(1, n - 1) + 1)
-    d2 = rng.permutation(np.arange(hidden) % max(1, n - 1) + 1)
+    d1 = hidden_degrees(n, hidden)
+    d2 = hidden_degrees(n, hidden)
     m1 = (d1[:, None] >= deg_in[None, :]).astype(np.float32)
     m2 = (d2[:, None] >= d1[None, :]).astype(np.float32)
     m3 = (deg_in[:, None] > d2[None, :]).astype(np.float32)
@@ -195,9 +203,65 @@ class MADE:
             out[s0:s0 + b] = -self.nll[:b].double().cpu().numpy()
         return out
 
-    def sample(self, uniforms):
+    def _sampling_buffers(self, B):
+        sb = getattr(self, '_sbuf', None)
+        if sb is not None and sb['B'] >= B:
+            return sb
+        torch = self.torch
+        n, H = self.n, self.H
+        self._sbuf = sb = dict(B=B, x=torch.zeros((B, n), dtype=torch.int16, device=self.dev),
+                               h1=torch.empty((B, H), dtype=torch.int16, device=self.dev),
+                               h2=torch.empty((B, H), dtype=torch.int16, device=self.dev),
+                               logit=torch.empty(B, dtype=torch.float32, device=self.dev),
+                               logq=torch.empty(B, dtype=torch.float64, device=self.dev),
+                               u=torch.empty((B, n), dtype=torch.float64, device=self.dev))
+        return sb
+
+    def sample(self, uniforms, batch=16384):
         """Autoregressive samples from q with uniforms [count, n] (float64):
-        x_o = [u_o < sigmoid(l_o)].  Returns (bits uint8 [count, n], log q)."""
+        x_o = [u_o < sigmoid(l_o)].  Returns (bits uint8 [count, n], log q).
+
+        Incremental: hidden degrees are sorted, so at step o the units of
+        degree <= o are a prefix of each hidden layer, and only the units of
+        degree exactly o are new.  Step o computes those units of both layers
+        (their inputs are already final) and logit o alone, i.e. about
+        H^2 + H n multiply-adds per proposal in total instead of n full
+        forward passes.  log q accumulates log sigmoid(+-l_o) of the drawn
+        bits from the same logits."""
+        torch = self.torch
+        n, H = self.n, self.H
+        u_all = np.ascontiguousarray(uniforms, np.float64)
+        count = len(u_all)
+        sb = self._sampling_buffers(min(batch, max(count, 1)))
+        B = sb['B']
+        k = np.searchsorted(hidden_degrees(n, H), np.arange(n), side='right')   # units of degree <= o
+        wb, w, s = self.wb, self.w, self.stream.cuda_stream
+        W1, W2, W3 = self._p(wb, 0), self._p(wb, 2), self._p(wb, 4)
+        b1, b2, b3 = self._p(w, 1), self._p(w, 3), self._p(w, 5)
+        x, h1, h2 = ptr(sb['x']), ptr(sb['h1']), ptr(sb['h2'])
+        logit, logq, u = ptr(sb['logit']), ptr(sb['logq']), ptr(sb['u'])
+        bits = np.empty((count, n), np.uint8)
+        out_logq = np.empty(count)
+        for s0 in range(0, count, B):
+            b = min(B, count - s0)
+            sb['u'][:b].copy_(torch.as_tensor(u_all[s0:s0 + b]))
+            for o in range(n):
+                k0, k1 = (int(k[o - 1]) if o else 0), int(k[o])
+                if k1 > k0:
+                    # new units of degree o: inputs 0..o-1, then the first-layer prefix
+                    self._gemm(0, 1, b, k1 - k0, o, x, n, W1 + 2 * k0 * n, n, Cb=h1 + 2 * k0, ldcb=H,
+                               bias=b1 + 4 * k0, relu=1)
+                    self._gemm(0, 1, b, k1 - k0, k1, h1, H, W2 + 2 * k0 * H, H, Cb=h2 + 2 * k0, ldcb=H,
+                               bias=b2 + 4 * k0, relu=1)
+                self._gemm(0, 1, b, 1, k1, h2, H, W3 + 2 * o * H, H, C=logit, ldc=1, bias=b3 + 4 * o)
+                call('rgs_made_sample_step', logit, 1, u, b, n, o, x, logq, int(o == 0), s)
+            bits[s0:s0 + b] = (sb['x'][:b].cpu().numpy() != 0).astype(np.uint8)
+            out_logq[s0:s0 + b] = sb['logq'][:b].cpu().numpy()
+        return bits, out_logq
+
+    def sample_full(self, uniforms):
+        """Autoregressive samples with a full forward pass per column (the
+        plain method; kept to check the incremental sampler)."""
         torch = self.torch
         u_all = np.ascontiguousarray(uniforms, np.float64)
         count = len(u_all)

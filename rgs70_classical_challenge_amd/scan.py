@@ -29,6 +29,12 @@ _lib.register('rgs_scan_order_search', ctypes.c_int,
 _lib.register('rgs_scan_pack', ctypes.c_int, [_V, ctypes.c_int, _V, ctypes.POINTER(ctypes.c_uint64)])
 _lib.register('rgs_scan_batch', ctypes.c_int,
               [_V, ctypes.c_int, _V, _V, ctypes.c_int, ctypes.c_int, _V, _V, _V])
+_lib.register('rgs_scan_pack_tables', ctypes.c_int, [_V, ctypes.c_int, _V, ctypes.POINTER(ctypes.c_uint64)])
+_lib.register('rgs_scan_large', ctypes.c_int,
+              [_V, ctypes.c_int, _V, _V, _V, _V, ctypes.c_int, _V, _V, ctypes.c_uint64, _V, _V, _V, _V])
+
+# Largest rank the one-launch batch kernels hold in LDS, per table precision.
+BATCH_RMAX = {64: 13, 128: 12}
 
 
 def _graph(g):
@@ -91,6 +97,14 @@ class ScanPlan:
         call('rgs_scan_pack', ctypes.addressof(self.steps), self.n, ptr(blob), ctypes.byref(words))
         return blob
 
+    def packed_tables(self):
+        """Byte-table plan for the HBM-table scan (ranks up to 32)."""
+        words = ctypes.c_uint64()
+        call('rgs_scan_pack_tables', ctypes.addressof(self.steps), self.n, None, ctypes.byref(words))
+        blob = np.zeros(words.value, np.uint64)
+        call('rgs_scan_pack_tables', ctypes.addressof(self.steps), self.n, ptr(blob), ctypes.byref(words))
+        return blob
+
     def table_bytes(self, precision=64):
         """Algorithmic table traffic of one proposal: every step reads its input
         table once and writes its output table once."""
@@ -102,7 +116,10 @@ class ScanPlan:
 class ScanOracle:
     """p(x) by the linear rank-width scan on one device (drop-in for ScanOracle)."""
 
-    def __init__(self, g, u, order, source=None, device=0, gpu=True, fp64=False):
+    def __init__(self, g, u, order, source=None, device=0, gpu=True, fp64=False, batch_rmax=None):
+        """batch_rmax: largest rank scored by the one-launch batch kernels
+        (default BATCH_RMAX, their LDS limit); above it proposals are scored
+        one at a time with HBM tables."""
         import torch
         if not gpu:
             raise ValueError('this scorer runs on the device only')
@@ -118,21 +135,55 @@ class ScanOracle:
         self.steps = self.plan.steps
         self.precision = 128 if fp64 else 64
         self.device = torch.device('cuda', device)
-        self._plan = torch.as_tensor(self.plan.packed().view(np.int64)).to(self.device)
+        self.batch_rmax = dict(BATCH_RMAX) if batch_rmax is None else {k: min(v, batch_rmax)
+                                                                         for k, v in BATCH_RMAX.items()}
+        self._plan = None
+        if self.rmax <= max(self.batch_rmax.values()):
+            self._plan = torch.as_tensor(self.plan.packed().view(np.int64)).to(self.device)
         self._u = torch.as_tensor(self.u.view(np.float64).reshape(-1)).to(self.device)
         self.stream = torch.cuda.current_stream(self.device)
+        self._large = {}
+
+    def _large_state(self, precision):
+        """Tables plan on the device, its step offsets on the host, two HBM
+        tables of 2^rmax entries and the per-step maxima."""
+        torch = self.torch
+        if 'plan' not in self._large:
+            blob = self.plan.packed_tables()
+            self._large['plan'] = torch.as_tensor(blob.view(np.int64)).to(self.device)
+            self._large['offsets'] = np.ascontiguousarray(blob[2:2 + self.n])
+            self._large['mx'] = torch.zeros(self.n, dtype=torch.int64, device=self.device)
+        if precision not in self._large:
+            entries = 1 << self.rmax
+            dt = torch.complex64 if precision == 64 else torch.complex128
+            self._large[precision] = (torch.empty(entries, dtype=dt, device=self.device),
+                                      torch.empty(entries, dtype=dt, device=self.device))
+        return self._large
 
     def raw(self, bits, precision=None):
-        """Enqueue a batch; returns device tensors (value complex128 pairs, exponent)."""
+        """Enqueue a batch; returns device tensors (value complex128 pairs,
+        exponent).  Ranks within the LDS limit run as one launch for the whole
+        batch (rgs_scan_batch); larger ranks run one proposal at a time with
+        HBM tables (rgs_scan_large)."""
         torch = self.torch
+        prec = precision or self.precision
         words = bits_to_words(bits)
         count = len(words)
-        dev_bits = torch.as_tensor(words.view(np.int64)).to(self.device)
         val = torch.empty((count, 2), dtype=torch.float64, device=self.device)
         ex = torch.empty(count, dtype=torch.int32, device=self.device)
-        call('rgs_scan_batch', ptr(self._plan), self.rmax, ptr(self._u), ptr(dev_bits), count,
-             precision or self.precision, ptr(val), ptr(ex), self.stream.cuda_stream)
-        return val, ex, dev_bits
+        if self.rmax <= self.batch_rmax[prec]:
+            dev_bits = torch.as_tensor(words.view(np.int64)).to(self.device)
+            call('rgs_scan_batch', ptr(self._plan), self.rmax, ptr(self._u), ptr(dev_bits), count, prec, ptr(val),
+                 ptr(ex), self.stream.cuda_stream)
+            return val, ex, dev_bits
+        st = self._large_state(prec)
+        b0, b1 = st[prec]
+        for k in range(count):
+            w = np.ascontiguousarray(words[k])
+            call('rgs_scan_large', ctypes.addressof(self.steps), self.n, ptr(st['plan']), ptr(st['offsets']),
+                 ptr(self.u), ptr(w), prec, ptr(b0), ptr(b1), b0.numel(), ptr(st['mx']),
+                 ptr(val) + 16 * k, ptr(ex) + 4 * k, self.stream.cuda_stream)
+        return val, ex, None
 
     def amplitudes(self, bits, precision=None):
         val, ex, _ = self.raw(bits, precision)

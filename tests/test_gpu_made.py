@@ -3,8 +3,9 @@
 GEMM: products of BF16 inputs are exact in FP32, so results agree with a
 float64 matmul of the same BF16 values to FP32 accumulation error (1e-5
 relative).  Gradients: the device rounds activations and backpropagated
-signals to BF16, so each gradient tensor must match the float64 oracle
-(evaluated on the device's BF16 weights) within 3e-2 relative L2.
+signals to BF16, so each gradient tensor must match, in relative L2, the
+float64 oracle that rounds at the same points within 2e-3, and the plain
+float64 oracle (evaluated on the device's BF16 weights) within 0.1.
 """
 import numpy as np
 import pytest
@@ -135,3 +136,25 @@ def test_training_learns_and_sampling_matches_q():
     big = expect > 20
     chi2 = ((counts[big] - expect[big]) ** 2 / expect[big]).sum()
     assert chi2 < big.sum() + 6 * np.sqrt(2 * big.sum())
+
+
+def test_incremental_sampler_equals_full_forward():
+    """The incremental sampler (new units of degree o only, logit o only)
+    reproduces the column-by-column full-forward sampler bit for bit, and its
+    log q equals the full forward's log q of the drawn strings."""
+    from rgs70_classical_challenge_amd.made import MADE
+    n, H = 20, 128
+    net = MADE(n, hidden=H, batch=256, seed=3)
+    rng = np.random.default_rng(11)
+    data = (rng.random((4096, n)) < np.linspace(0.2, 0.8, n)).astype(np.uint8)
+    data[:, 5] = data[:, 2] ^ data[:, 3]                       # some structure to learn
+    net.train(data, 200, seed=1)
+    u = rng.random((3000, n))
+    bits, logq = net.sample(u, batch=1024)                     # three partial batches
+    bits_full, logq_full = net.sample_full(u)
+    assert np.array_equal(bits, bits_full)
+    ref = net.log_q(bits)
+    assert np.max(np.abs(logq - ref)) < 1e-4 * n, np.max(np.abs(logq - ref))
+    assert np.max(np.abs(logq - logq_full)) < 1e-4 * n
+    # a decided bit follows its own logit: log q never exceeds 0
+    assert np.all(logq <= 1e-12)

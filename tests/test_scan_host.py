@@ -56,6 +56,40 @@ def test_pack_layout():
             assert int(blob[off + 2 + c]) == want
 
 
+def test_pack_tables_layout(golden_dir):
+    """Byte-table plan: every output index's input vector L c' rebuilt from the
+    byte tables equals the XOR of the lcols, and the combinations span the
+    kernel (RGS70 ordering, ranks up to 30)."""
+    d = np.load(f'{golden_dir}/rgs_graph.npz')
+    plan = scan.ScanPlan(d['graph'], np.load(f'{golden_dir}/scan_order_optimized.npy').tolist())
+    blob = plan.packed_tables()
+    assert blob[0] == 70 and blob[1] == 30
+    rng = np.random.default_rng(0)
+    for j, st in enumerate(plan.steps):
+        off = int(blob[2 + j])
+        hdr = int(blob[off])
+        nbytes = (st.r_out + 7) // 8
+        assert (hdr & 255, (hdr >> 8) & 255, (hdr >> 16) & 255, (hdr >> 24) & 255, hdr >> 32) == \
+            (st.v, st.r_in, st.r_out, st.nker, nbytes)
+        assert int(blob[off + 1]) == st.beta
+        for cp in rng.integers(0, 1 << st.r_out, 8).tolist() if st.r_out else [0]:
+            want = 0
+            for i in range(st.r_out):
+                if (cp >> i) & 1:
+                    want ^= st.lcols[i]
+            got = 0
+            for b in range(nbytes):
+                got ^= int(blob[off + 2 + 256 * b + ((cp >> (8 * b)) & 255)])
+            assert got == want
+        combos = [int(x) for x in blob[off + 2 + 256 * nbytes: off + 2 + 256 * nbytes + (1 << st.nker)]]
+        span = {0}
+        for k in range(st.nker):
+            span |= {x ^ st.kern[k] for x in span}
+        assert sorted(combos) == sorted(span) and len(set(combos)) == 1 << st.nker
+    words = blob.size
+    assert words == 2 + 70 + sum(2 + 256 * ((st.r_out + 7) // 8) + (1 << st.nker) for st in plan.steps)
+
+
 def test_order_search_improves_and_respects_cap():
     g, _ = instance(70, 1234)
     start = list(range(70))
