@@ -25,6 +25,10 @@ Reported beside the headline value:
   scan          batched rank-scan p(x) of 65,536 uniform 70-bit proposals of a
                 70-vertex 3-regular instance per launch (complex64 tables, FP64
                 sums), proposals/s and the complex64-vs-complex128 error.
+  production    the reference's production conditional sampler on its own
+                70-qubit instance (m=35: one 2^35 complex64 slab), replaying
+                6 recorded samples: seconds per sample, index matches and p
+                error against the recorded complex128 values.
   cpu_baseline  the oracle (NumPy restatement of the reference's CPU
                 full-state path) on one host core: n=26 state + mass pyramid +
                 10,000 draws, extrapolated to n=34 by 2^8 * 34/26.
@@ -201,6 +205,23 @@ def run_scan(torch, count=65536, seed=70):
                                     'max': float(rel.max())}}
 
 
+def run_production(torch, count=6):
+    """The reference's production workload on one device: the 70-qubit
+    instance's 2^35-amplitude conditional state per sample, replaying its
+    recorded B bits and draw uniforms (rgs70_born_records.json keeps 24 of
+    them); reference: median 2.4162 s per sample on 2 x H200."""
+    from rgs70_classical_challenge_amd.production import replay
+    golden = os.path.join(ROOT, 'tests', 'golden')
+    d = np.load(os.path.join(golden, 'rgs_graph.npz'))
+    data = json.load(open(os.path.join(golden, 'rgs70_born_records.json')))
+    summary, rows = replay(d['graph'], d['measurement_unitaries'], data['A_vertices'], data['B_vertices'],
+                           data['records'][:count], log=log)
+    summary['reference_median_seconds_per_sample'] = 2.4162
+    summary['reference_hardware'] = '2 x H200, 2^34 complex64 amplitudes per GPU'
+    summary['workload'] = f'70-qubit RGS instance, m=35 conditional state (256 GiB complex64), {count} recorded samples'
+    return summary
+
+
 def cpu_baseline(seed):
     """Oracle (NumPy) on one host core: bounded sample at n=26, extrapolated to n=34."""
     from oracle import draw as od
@@ -309,6 +330,12 @@ def main():
         except Exception as e:
             out['scan'] = {'error': repr(e)}
         log('[bench] scan done')
+        torch.cuda.empty_cache()
+        try:
+            out['production'] = run_production(torch)
+        except Exception as e:
+            out['production'] = {'error': repr(e)}
+        log('[bench] production replay done')
     if not args.no_cpu:
         out['cpu_baseline'] = cpu_baseline(args.seed)
         log('[bench] cpu baseline done')
