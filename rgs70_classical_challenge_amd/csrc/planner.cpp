@@ -6,6 +6,7 @@
 #include <stdint.h>
 
 #include <algorithm>
+#include <cmath>
 #include <complex>
 #include <cstring>
 #include <string>
@@ -203,10 +204,44 @@ extern "C" int rgs_cov_luts(const uint8_t* g, int nv, const int32_t* A, const in
 // D2 = diag(1, e^{i b}): |u00| = c, |u10| = s, a0 = arg u00, a1 = arg u10,
 // b = arg u11 - a1 (= arg(-u01) - a0 by unitarity).  Degenerate columns
 // (c = 0 or s = 0) put the free phase into D1.
+// Float32 rounding of a unit pair (xd, yd).  Rounding cos and sin
+// independently leaves x^2 + y^2 - 1 = eps with |eps| up to about 6e-8, and a
+// rotation scales every probability by 1 + eps; identical gates on every qubit
+// would compound eps.  Among the floats within one ulp of (xd, yd) this picks
+// the pair whose eps keeps the running sum `*carry` of earlier choices nearest
+// zero (ties: nearest to (xd, yd)), so the product of the norms stays within
+// one rounding of 1 at any qubit count.  carry == NULL: smallest |eps|.
+static void unit_pair(double xd, double yd, float* x, float* y, double* carry) {
+  const float x0 = (float)xd, y0 = (float)yd;
+  const double base = carry ? *carry : 0.0;
+  float bx = x0, by = y0;
+  double best = 1e300, best_eps = 0.0, best_dist = 1e300;
+  for (int i = -1; i <= 1; ++i)
+    for (int j = -1; j <= 1; ++j) {
+      float cx = x0, cy = y0;
+      if (i) cx = std::nextafter(x0, i > 0 ? 2.0f : -2.0f);
+      if (j) cy = std::nextafter(y0, j > 0 ? 2.0f : -2.0f);
+      const double eps = (double)cx * cx + (double)cy * cy - 1.0;
+      const double score = std::abs(base + eps);
+      const double dist = std::abs(cx - xd) + std::abs(cy - yd);
+      if (score < best - 1e-18 || (std::abs(score - best) <= 1e-18 && dist < best_dist)) {
+        best = score;
+        best_eps = eps;
+        best_dist = dist;
+        bx = cx;
+        by = cy;
+      }
+    }
+  *x = bx;
+  *y = by;
+  if (carry) *carry = base + best_eps;
+}
+
 extern "C" int rgs_gate_zyz(const double* u, int n, int gate_limit, float* rot, float* dphase_lut, double* d1) {
   if (!u || n < 1 || n > 63) return fail(kInvalid, "gate_zyz: need 1 <= n <= 63");
   using cd = std::complex<double>;
   std::vector<double> b(n, 0.0);
+  double carry = 0.0;  // running sum of the rotations' norm errors
   for (int q = 0; q < n; ++q) {
     const double* e = u + 8 * q;
     cd u00(e[0], e[1]), u01(e[2], e[3]), u10(e[4], e[5]), u11(e[6], e[7]);
@@ -234,10 +269,7 @@ extern "C" int rgs_gate_zyz(const double* u, int n, int gate_limit, float* rot, 
       a1 = std::arg(u10);
       bb = std::arg(u11) - a1;
     }
-    if (rot) {
-      rot[2 * q] = (float)c;
-      rot[2 * q + 1] = (float)s;
-    }
+    if (rot) unit_pair(c, s, rot + 2 * q, rot + 2 * q + 1, &carry);
     if (d1) {
       d1[4 * q + 0] = std::cos(a0);
       d1[4 * q + 1] = std::sin(a0);
@@ -255,8 +287,8 @@ extern "C" int rgs_gate_zyz(const double* u, int n, int gate_limit, float* rot, 
           int q = 8 * ch + t;
           if (q < n && ((code >> t) & 1)) ang += b[q];
         }
-        dphase_lut[2 * (ch * 256 + code)] = (float)std::cos(ang);
-        dphase_lut[2 * (ch * 256 + code) + 1] = (float)std::sin(ang);
+        unit_pair(std::cos(ang), std::sin(ang), dphase_lut + 2 * (ch * 256 + code),
+                  dphase_lut + 2 * (ch * 256 + code) + 1, nullptr);
       }
   }
   return kOk;

@@ -144,3 +144,41 @@ def test_pass_schedules():
     assert all(tok.endswith('rb2') for tok in text.split() if tok.startswith('strided')), text
     n, text = describe(20, walsh=True, low=10, tile=12, lane=4, rounds=2)
     assert all(tok.endswith('rb0') for tok in text.split() if tok.startswith('strided')), text  # 9-bit tiles
+
+
+def test_gate_factorisation_and_unit_rounding(golden_dir):
+    """u = D1 R D2 with R real: reconstructed from the library's outputs to
+    float32 accuracy.  Rounding cos and sin separately misses |(c, s)| = 1 by
+    up to 6e-8, which identical gates on every qubit would compound: the
+    library dithers the rounding so the running product of rotation norms
+    stays within one rounding of 1, and picks phase-table entries of smallest
+    norm error."""
+    d = np.load(f'{golden_dir}/rgs_graph.npz')
+    mats = np.concatenate([d['measurement_unitaries'][:4], instance(12, 5)[1]])
+    n = len(mats)
+    u = np.ascontiguousarray(mats.astype(np.complex128))
+    rot = np.zeros(2 * n, np.float32)
+    dph = np.zeros(((n + 7) // 8) * 256 * 2, np.float32)
+    d1 = np.zeros(4 * n, np.float64)
+    _lib.call('rgs_gate_zyz', _lib.ptr(u), n, n, _lib.ptr(rot), _lib.ptr(dph), _lib.ptr(d1))
+    d1 = d1.view(np.complex128).reshape(n, 2)
+    rot = rot.reshape(n, 2).astype(np.float64)
+    dph = dph.reshape(-1, 2).astype(np.float64)
+    eps = rot[:, 0] ** 2 + rot[:, 1] ** 2 - 1
+    assert np.max(np.abs(eps)) < 2.5e-7                        # within an ulp of the pair
+    assert np.max(np.abs(np.cumsum(eps))) < 1.2e-7             # no drift of the product
+    deps = dph[:, 0] ** 2 + dph[:, 1] ** 2 - 1
+    assert np.max(np.abs(deps)) < 1.2e-7 and abs(np.mean(deps)) < 1e-8
+    # 70 identical gates (the 70-qubit instance): the product of norms stays at 1
+    same = np.ascontiguousarray(np.repeat(d['measurement_unitaries'][:1], 63, axis=0).astype(np.complex128))
+    r2 = np.zeros(2 * 63, np.float32)
+    _lib.call('rgs_gate_zyz', _lib.ptr(same), 63, 63, _lib.ptr(r2), None, None)
+    r2 = r2.reshape(63, 2).astype(np.float64)
+    assert abs(np.prod(r2[:, 0] ** 2 + r2[:, 1] ** 2) - 1) < 1.2e-7
+    dphc = dph[:, 0] + 1j * dph[:, 1]
+    for q in range(n):
+        c, s = rot[q]
+        R = np.array([[c, -s], [s, c]])
+        e_b = dphc[(q // 8) * 256 + (1 << (q % 8))]            # single-qubit entry of the D2 table
+        rec = np.diag(d1[q]) @ R @ np.diag([1.0, e_b])
+        assert np.max(np.abs(rec - u[q])) < 3e-7, q

@@ -33,7 +33,10 @@ def test_rgs70_production_replay(golden_dir):
     # complex64 state vs the reference's complex128 rescoring: the reference's own
     # complex64 engine deviated by at most 3.3e-6 (born/README.md:13)
     assert summary['max_rel_err_vs_fp64'] < 2e-5
-    assert summary['max_norm_diff'] < 1e-6
+    # the conditional state is normalised exactly in arithmetic; both engines
+    # hold complex64 amplitudes (the recorded norms are 1 + 2.3e-7)
+    assert all(abs(r['norm'] - 1) < 1e-6 for r in rows), [r['norm'] for r in rows]
+    assert summary['max_norm_diff'] < 2e-6
     # the draw is an inverse CDF over 2^35 bins with FP32 amplitudes: a record may
     # land within rounding of a bin edge, so allow one index to differ
     assert summary['index_matches'] >= len(rows) - 1
