@@ -25,6 +25,8 @@ Reported beside the headline value:
   scan          batched rank-scan p(x) of 65,536 uniform 70-bit proposals of a
                 70-vertex 3-regular instance per launch (complex64 tables, FP64
                 sums), proposals/s and the complex64-vs-complex128 error.
+  scan_rgs70    the rank-30 scan of the reference's own 70-qubit instance
+                (HBM tables): seconds per amplitude, complex64 and complex128.
   production    the reference's production conditional sampler on its own
                 70-qubit instance (m=35: one 2^35 complex64 slab), replaying
                 6 recorded samples: seconds per sample, index matches and p
@@ -205,6 +207,37 @@ def run_scan(torch, count=65536, seed=70):
                                     'max': float(rel.max())}}
 
 
+def run_scan_rgs70(torch, count=8):
+    """Rank-30 scan of the reference's own 70-qubit instance with its
+    optimised order (HBM tables, one proposal at a time): seconds per
+    amplitude in complex64 and complex128 tables.  Reference (1 x H200):
+    0.0750 s (complex64, this order) and about 0.130 s (complex128)."""
+    from rgs70_classical_challenge_amd.scan import ScanOracle
+    golden = os.path.join(ROOT, 'tests', 'golden')
+    d = np.load(os.path.join(golden, 'rgs_graph.npz'))
+    order = np.load(os.path.join(golden, 'scan_order_optimized.npy')).tolist()
+    props = json.load(open(os.path.join(golden, 'rgs70_cap1_proposals.json')))['proposals'][:count]
+    bits = np.array([[int(c) for c in r['bitstring_q0_left']] for r in props], np.uint8)
+    want = np.array([r['p_fp64'] for r in props])
+    out = {'workload': f'70-qubit instance, optimised order (max rank 30), {count} proposals',
+           'reference_seconds_per_amplitude': {'complex64': 0.0750, 'complex128': 0.130}}
+    for name, fp64 in (('complex64', False), ('complex128', True)):
+        so = ScanOracle(d['graph'], d['measurement_unitaries'], order, fp64=fp64)
+        so.probabilities(bits[:1])                      # tables, buffers, first launch
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        p = so.probabilities(bits)
+        dt = (time.perf_counter() - t0) / count
+        size = 16 if fp64 else 8
+        out[name] = {'seconds_per_amplitude': round(dt, 5), 'rmax': so.rmax,
+                     'table_gbps': round(so.plan.table_bytes(128 if fp64 else 64) / dt / 1e9, 1),
+                     'max_rel_err_vs_recorded_fp64': float(np.max(np.abs(p - want) / want)),
+                     'table_bytes_per_amplitude': so.plan.table_bytes(128 if fp64 else 64), 'entry_bytes': size}
+        del so
+        torch.cuda.empty_cache()
+    return out
+
+
 def run_production(torch, count=6):
     """The reference's production workload on one device: the 70-qubit
     instance's 2^35-amplitude conditional state per sample, replaying its
@@ -330,6 +363,12 @@ def main():
         except Exception as e:
             out['scan'] = {'error': repr(e)}
         log('[bench] scan done')
+        torch.cuda.empty_cache()
+        try:
+            out['scan_rgs70'] = run_scan_rgs70(torch)
+        except Exception as e:
+            out['scan_rgs70'] = {'error': repr(e)}
+        log('[bench] rank-30 scan done')
         torch.cuda.empty_cache()
         try:
             out['production'] = run_production(torch)

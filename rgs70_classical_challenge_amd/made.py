@@ -118,9 +118,9 @@ class MADE:
         return ptr(tensor) + int(self.offsets[k]) * tensor.element_size()
 
     def _gemm(self, ta, tb, M, N, K, A, lda, Bm, ldb, C=None, ldc=0, Cb=None, ldcb=0, bias=None, relu=0,
-              dmask=None, ldm=0, acc=0, alpha=1.0):
+              dmask=None, ldm=0, acc=0, alpha=1.0, stream=None):
         call('rgs_gemm_bf16', ta, tb, M, N, K, A, lda, Bm, ldb, alpha, C, ldc, Cb, ldcb, bias, relu, dmask, ldm,
-             acc, self.stream.cuda_stream)
+             acc, self.stream.cuda_stream if stream is None else stream)
 
     def forward(self, B=None):
         """Forward on self.x[:B]: h1, h2, logits."""
@@ -217,44 +217,75 @@ class MADE:
                                u=torch.empty((B, n), dtype=torch.float64, device=self.dev))
         return sb
 
-    def sample(self, uniforms, batch=16384):
-        """Autoregressive samples from q with uniforms [count, n] (float64):
-        x_o = [u_o < sigmoid(l_o)].  Returns (bits uint8 [count, n], log q).
+    def _sample_steps(self, b, stream):
+        """Enqueue the n incremental steps for the first b rows of the
+        sampling buffers (uniforms in sb['u']) on `stream`.
 
-        Incremental: hidden degrees are sorted, so at step o the units of
-        degree <= o are a prefix of each hidden layer, and only the units of
-        degree exactly o are new.  Step o computes those units of both layers
-        (their inputs are already final) and logit o alone, i.e. about
-        H^2 + H n multiply-adds per proposal in total instead of n full
-        forward passes.  log q accumulates log sigmoid(+-l_o) of the drawn
-        bits from the same logits."""
-        torch = self.torch
+        Hidden degrees are sorted, so at step o the units of degree <= o are a
+        prefix of each hidden layer and only the units of degree exactly o are
+        new.  Step o computes those units of both layers (their inputs are
+        final) and logit o alone: about H^2 + H n multiply-adds per proposal
+        in total instead of n full forward passes.  log q accumulates
+        log sigmoid(+-l_o) of the drawn bits from the same logits."""
         n, H = self.n, self.H
-        u_all = np.ascontiguousarray(uniforms, np.float64)
-        count = len(u_all)
-        sb = self._sampling_buffers(min(batch, max(count, 1)))
-        B = sb['B']
+        sb = self._sbuf
         k = np.searchsorted(hidden_degrees(n, H), np.arange(n), side='right')   # units of degree <= o
-        wb, w, s = self.wb, self.w, self.stream.cuda_stream
+        wb, w = self.wb, self.w
         W1, W2, W3 = self._p(wb, 0), self._p(wb, 2), self._p(wb, 4)
         b1, b2, b3 = self._p(w, 1), self._p(w, 3), self._p(w, 5)
         x, h1, h2 = ptr(sb['x']), ptr(sb['h1']), ptr(sb['h2'])
         logit, logq, u = ptr(sb['logit']), ptr(sb['logq']), ptr(sb['u'])
-        bits = np.empty((count, n), np.uint8)
+        for o in range(n):
+            k0, k1 = (int(k[o - 1]) if o else 0), int(k[o])
+            if k1 > k0:
+                # new units of degree o: inputs 0..o-1, then the first-layer prefix
+                self._gemm(0, 1, b, k1 - k0, o, x, n, W1 + 2 * k0 * n, n, Cb=h1 + 2 * k0, ldcb=H,
+                           bias=b1 + 4 * k0, relu=1, stream=stream)
+                self._gemm(0, 1, b, k1 - k0, k1, h1, H, W2 + 2 * k0 * H, H, Cb=h2 + 2 * k0, ldcb=H,
+                           bias=b2 + 4 * k0, relu=1, stream=stream)
+            self._gemm(0, 1, b, 1, k1, h2, H, W3 + 2 * o * H, H, C=logit, ldc=1, bias=b3 + 4 * o, stream=stream)
+            call('rgs_made_sample_step', logit, 1, u, b, n, o, x, logq, int(o == 0), stream)
+
+    def sample_device(self, b, graph=True):
+        """Draw b proposals from the uniforms already in the sampling buffer
+        sb['u'][:b] (device float64 [b, n]); returns device views (x as bf16
+        0/1 [b, n], log q float64 [b]).  With graph=True the n steps are
+        captured once per b and replayed."""
+        torch = self.torch
+        sb = self._sbuf
+        if not graph:
+            self._sample_steps(b, self.stream.cuda_stream)
+            return sb['x'][:b], sb['logq'][:b]
+        graphs = self.__dict__.setdefault('_graphs', {})
+        if b not in graphs:
+            side = torch.cuda.Stream(self.dev)
+            side.wait_stream(torch.cuda.current_stream(self.dev))
+            with torch.cuda.stream(side):
+                self._sample_steps(b, side.cuda_stream)            # warm-up outside capture
+            torch.cuda.current_stream(self.dev).wait_stream(side)
+            torch.cuda.synchronize(self.dev)
+            g = torch.cuda.CUDAGraph()
+            with torch.cuda.graph(g):
+                self._sample_steps(b, torch.cuda.current_stream(self.dev).cuda_stream)
+            graphs[b] = g
+        graphs[b].replay()
+        return sb['x'][:b], sb['logq'][:b]
+
+    def sample(self, uniforms, batch=16384):
+        """Autoregressive samples from q with uniforms [count, n] (float64):
+        x_o = [u_o < sigmoid(l_o)].  Returns (bits uint8 [count, n], log q).
+        Incremental steps (see _sample_steps), in batches of `batch`."""
+        torch = self.torch
+        u_all = np.ascontiguousarray(uniforms, np.float64)
+        count = len(u_all)
+        sb = self._sampling_buffers(min(batch, max(count, 1)))
+        B = sb['B']
+        bits = np.empty((count, self.n), np.uint8)
         out_logq = np.empty(count)
         for s0 in range(0, count, B):
             b = min(B, count - s0)
             sb['u'][:b].copy_(torch.as_tensor(u_all[s0:s0 + b]))
-            for o in range(n):
-                k0, k1 = (int(k[o - 1]) if o else 0), int(k[o])
-                if k1 > k0:
-                    # new units of degree o: inputs 0..o-1, then the first-layer prefix
-                    self._gemm(0, 1, b, k1 - k0, o, x, n, W1 + 2 * k0 * n, n, Cb=h1 + 2 * k0, ldcb=H,
-                               bias=b1 + 4 * k0, relu=1)
-                    self._gemm(0, 1, b, k1 - k0, k1, h1, H, W2 + 2 * k0 * H, H, Cb=h2 + 2 * k0, ldcb=H,
-                               bias=b2 + 4 * k0, relu=1)
-                self._gemm(0, 1, b, 1, k1, h2, H, W3 + 2 * o * H, H, C=logit, ldc=1, bias=b3 + 4 * o)
-                call('rgs_made_sample_step', logit, 1, u, b, n, o, x, logq, int(o == 0), s)
+            self._sample_steps(b, self.stream.cuda_stream)
             bits[s0:s0 + b] = (sb['x'][:b].cpu().numpy() != 0).astype(np.uint8)
             out_logq[s0:s0 + b] = sb['logq'][:b].cpu().numpy()
         return bits, out_logq

@@ -95,3 +95,88 @@ class CappedRejection:
                     rel_err_c64_vs_c128={'median': float(np.median(np.abs(p - p64) / p64)),
                                          'max': float(np.max(np.abs(p - p64) / p64))},
                     bits=bits, p=p, p_fp64=p64)
+
+
+def pack_words_device(x, n):
+    """Device 0/1 tensor [count, n] (any integer type, nonzero = 1) -> int64
+    [count, 2] words with qubit q at bit q of word q // 64 (bits_to_words)."""
+    import torch
+    b = (x != 0).to(torch.int64)
+    words = torch.zeros((x.shape[0], 2), dtype=torch.int64, device=x.device)
+    for w in range(2):
+        cols = range(64 * w, min(n, 64 * w + 64))
+        if len(cols):
+            sh = torch.arange(len(cols), device=x.device, dtype=torch.int64)
+            words[:, w] = (b[:, cols.start:cols.stop] << sh).sum(1)
+    return words
+
+
+class DeviceCappedRejection:
+    """Capped rejection with every proposal, uniform and decision on the
+    device (seeded torch generator): uniform proposals (bits and acceptance
+    uniforms) or MADE proposals (incremental sampler, captured graph).  The
+    timed loop draws, scores (complex64-table scan) and decides; the
+    complex128 audit of every decision runs afterwards, untimed."""
+
+    def __init__(self, scorer, auditor, n, cap, proposal=None):
+        self.scorer, self.auditor, self.n, self.cap = scorer, auditor, n, cap
+        self.proposal = proposal
+
+    def run(self, seed, accept_target, batch=65536, max_batches=10000):
+        import torch
+        n, cap = self.n, self.cap
+        dev = self.scorer.device
+        gen = torch.Generator(device=dev)
+        gen.manual_seed(int(seed))
+        net = self.proposal
+        if net is not None:
+            sb = net._sampling_buffers(batch)
+        kept_words, kept_u, kept_logq, kept_keep = [], [], [], []
+        accepted = 0
+        proposals = 0
+        torch.cuda.synchronize(dev)
+        t0 = time.perf_counter()
+        for _ in range(max_batches):
+            if net is None:
+                x = torch.randint(0, 2, (batch, n), generator=gen, device=dev, dtype=torch.uint8)
+                logq = None
+            else:
+                sb['u'][:batch].uniform_(generator=gen)
+                x, logq = net.sample_device(batch)
+            u = torch.rand(batch, generator=gen, device=dev, dtype=torch.float64)
+            words = pack_words_device(x, n)
+            p = self.scorer.probabilities_device(words)
+            if logq is None:
+                acc = torch.clamp(p * (2.0 ** n / cap), max=1.0)
+            else:
+                acc = torch.clamp(p / (cap * torch.exp(logq)), max=1.0)
+            keep = u < acc
+            accepted += int(keep.sum().item())
+            proposals += batch
+            kept_words.append(words)
+            kept_u.append(u)
+            kept_logq.append(None if logq is None else logq.clone())
+            kept_keep.append(keep)
+            if accepted >= accept_target:
+                break
+        torch.cuda.synchronize(dev)
+        seconds = time.perf_counter() - t0
+        # audit: complex128 tables for every proposal, decisions recomputed in FP64
+        flips = 0
+        margins = []
+        z = []
+        for words, u, logq, keep in zip(kept_words, kept_u, kept_logq, kept_keep):
+            p64 = self.auditor.probabilities_device(words)
+            if logq is None:
+                acc64 = torch.clamp(p64 * (2.0 ** n / cap), max=1.0)
+            else:
+                acc64 = torch.clamp(p64 / (cap * torch.exp(logq)), max=1.0)
+            keep64 = u < acc64
+            flips += int((keep64 != keep).sum().item())
+            margins.append((u - acc64).abs().min().item())
+            z.append((p64[keep] * 2.0 ** n).cpu().numpy())
+        z = np.concatenate(z)
+        return dict(accepted=int(accepted), proposals=int(proposals), acceptance=accepted / proposals,
+                    seconds=seconds, accepted_per_s=accepted / seconds, proposals_per_s=proposals / seconds,
+                    xeb=float(z.mean() - 1), xeb_se=float(z.std(ddof=1) / np.sqrt(len(z))),
+                    decision_flips_fp64=flips, min_margin_fp64=float(min(margins)))

@@ -185,6 +185,21 @@ class ScanOracle:
                  ptr(val) + 16 * k, ptr(ex) + 4 * k, self.stream.cuda_stream)
         return val, ex, None
 
+    def probabilities_device(self, words_dev, precision=None):
+        """p(x) of proposals already on the device as int64 [count, 2] words
+        (bits_to_words layout); returns a device float64 tensor, nothing is
+        copied to the host (batch kernels only: rank within their LDS limit)."""
+        torch = self.torch
+        prec = precision or self.precision
+        if self.rmax > self.batch_rmax[prec]:
+            raise ValueError('device-resident scoring needs the batch kernels (rank within their limit)')
+        count = words_dev.shape[0]
+        val = torch.empty((count, 2), dtype=torch.float64, device=self.device)
+        ex = torch.empty(count, dtype=torch.int32, device=self.device)
+        call('rgs_scan_batch', ptr(self._plan), self.rmax, ptr(self._u), ptr(words_dev), count, prec, ptr(val),
+             ptr(ex), self.stream.cuda_stream)
+        return (val[:, 0] ** 2 + val[:, 1] ** 2) * torch.exp2(2.0 * ex.double() - self.n)
+
     def amplitudes(self, bits, precision=None):
         val, ex, _ = self.raw(bits, precision)
         v = val.cpu().numpy()

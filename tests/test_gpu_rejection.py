@@ -1,0 +1,59 @@
+"""Device-side capped rejection against the exact output distribution: the
+accepted strings follow min(p, cap q) (q uniform or the MADE's q), so their
+linear XEB and the acceptance rate have closed forms at small n."""
+import numpy as np
+import pytest
+
+from oracle import graph_state as gs
+
+pytestmark = pytest.mark.gpu
+
+
+def _torch():
+    import torch
+    if not torch.cuda.is_available():
+        pytest.skip('no HIP device')
+    return torch
+
+
+def _expected(p, q, cap, n):
+    w = np.minimum(p, cap * q)                       # accepted density (unnormalised)
+    acc_rate = w.sum() / cap                         # E_q[min(p / (cap q), 1)]
+    xeb = (w * p).sum() / w.sum() * 2 ** n - 1
+    return acc_rate, xeb
+
+
+def test_uniform_and_learned_rejection_match_closed_forms():
+    _torch()
+    from rgs70_classical_challenge_amd.instances import instance
+    from rgs70_classical_challenge_amd.made import MADE
+    from rgs70_classical_challenge_amd.rejection import DeviceCappedRejection
+    from rgs70_classical_challenge_amd.scan import ScanOracle, order_search
+    n = 12
+    g, u = instance(n, 4242)
+    psi = gs.full_state(g, u)
+    p = np.abs(psi) ** 2
+    order, _ = order_search(g, seconds=0.5)
+    scorer, auditor = ScanOracle(g, u, order), ScanOracle(g, u, order, fp64=True)
+    # uniform proposals
+    cap = 4.0
+    r = DeviceCappedRejection(scorer, auditor, n, cap).run(7, 40000, batch=8192)
+    acc, xeb = _expected(p, np.full(1 << n, 2.0 ** -n), cap, n)
+    print('uniform', r['acceptance'], acc, r['xeb'], xeb, r['xeb_se'], flush=True)
+    assert abs(r['acceptance'] - acc) < 5 * np.sqrt(acc * (1 - acc) / r['proposals'])
+    assert abs(r['xeb'] - xeb) < 5 * r['xeb_se']
+    assert r['decision_flips_fp64'] == 0
+    # learned proposals: a MADE trained briefly on exact samples
+    rng = np.random.default_rng(1)
+    data_idx = rng.choice(1 << n, size=20000, p=p / p.sum())
+    data = ((data_idx[:, None] >> np.arange(n)) & 1).astype(np.uint8)
+    net = MADE(n, hidden=64, batch=256, seed=4, lr=3e-3)
+    net.train(data, 400, seed=2)
+    all_bits = ((np.arange(1 << n)[:, None] >> np.arange(n)) & 1).astype(np.uint8)
+    q = np.exp(net.log_q(all_bits))
+    for cap in (1.0, 2.0):
+        r = DeviceCappedRejection(scorer, auditor, n, cap, proposal=net).run(9, 40000, batch=8192)
+        acc, xeb = _expected(p, q / q.sum(), cap, n)
+        print('learned', cap, r['acceptance'], acc, r['xeb'], xeb, r['xeb_se'], flush=True)
+        assert abs(r['acceptance'] - acc) < 5 * np.sqrt(acc * (1 - acc) / r['proposals']) + 2e-3
+        assert abs(r['xeb'] - xeb) < 5 * r['xeb_se'] + 2e-3
