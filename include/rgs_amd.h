@@ -299,6 +299,18 @@ int rgs_sample_col(const float *logits, const double *u, int B, int n, int o, ui
 int rgs_made_sample_step(const float *logit, int ldl, const double *u, int B, int n, int o, uint16_t *x,
                          double *logq, int first, void *stream);
 
+/* All n steps of incremental sampling in one launch (n <= 64, H <= 512, a
+ * multiple of 16): 64 proposals per workgroup with their activations in LDS,
+ * the same arithmetic as the sequence of rgs_gemm_bf16 / rgs_made_sample_step
+ * calls (bit-identical results).  W1 bf16 [H][n], W2 [H][H], W3 [n][H] (rows
+ * masked), biases FP32, kpref dev int32[n] = number of hidden units of degree
+ * <= o (degrees sorted), u FP64 [B][n].  Outputs (words and/or xout
+ * required): words uint64 [B][2] (qubit q = bit q % 64 of word q / 64), xout
+ * bf16 0/1 [B][n], logq FP64 [B]. */
+int rgs_made_sample(const uint16_t *W1, const float *b1, const uint16_t *W2, const float *b2, const uint16_t *W3,
+                    const float *b3, const int *kpref, const double *u, int B, int n, int H, uint64_t *words,
+                    uint16_t *xout, double *logq, void *stream);
+
 /* bf16 copy (round to nearest even) of an FP32 array. */
 int rgs_f32_to_bf16(const float *in, uint16_t *out, long long count, void *stream);
 

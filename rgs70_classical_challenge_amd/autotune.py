@@ -38,12 +38,16 @@ def candidates(n):
     return out
 
 
-def device_key_of(name):
-    return name.replace(' ', '_') or 'gpu'
+def device_key_of(props):
+    """Table key of a device: its architecture and compute-unit count (the
+    marketing name depends on the host's id tables and can change between
+    processes, e.g. under a profiler)."""
+    arch = getattr(props, 'gcnArchName', '') or 'gpu'
+    return f"{arch.split(':')[0]}_{props.multi_processor_count}cu"
 
 
-def device_key(torch):
-    return device_key_of(torch.cuda.get_device_name(0))
+def device_key(torch, device=0):
+    return device_key_of(torch.cuda.get_device_properties(device))
 
 
 def load_table(torch=None):
@@ -60,11 +64,11 @@ def lookup(n, torch):
     return t.get(str(n))
 
 
-def lookup_name(n, device_name):
-    """Entry for size n on the device of that name, or None."""
+def lookup_props(n, props):
+    """Entry for size n on a device with these properties, or None."""
     if not os.path.exists(TABLE):
         return None
-    return json.load(open(TABLE)).get(device_key_of(device_name), {}).get(str(n))
+    return json.load(open(TABLE)).get(device_key_of(props), {}).get(str(n))
 
 
 def tune(n, reps=3, seed=7):

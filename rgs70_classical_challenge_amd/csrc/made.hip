@@ -204,6 +204,144 @@ __global__ void k_sample_step(const float* logit, int ldl, const double* u, int 
   logq[r] = first ? ls : logq[r] + ls;
 }
 
+// Fused incremental sampler: one workgroup of four wavefronts draws 64
+// proposals through all n steps.  Hidden degrees are sorted, so at step o the
+// units of degree <= o are the prefix [0, kpref[o]) of each hidden layer and
+// the new units are [kpref[o-1], kpref[o]) (at most 16).  Each wavefront owns
+// 16 proposals and keeps their BF16 activations of both layers in LDS; the
+// step's slice of W2 (new rows x prefix columns) is staged in LDS once for the
+// four wavefronts.  Per step: the new first-layer units (K = o), the new
+// second-layer units (K = kpref[o]) and logit o (K = kpref[o]) as
+// v_mfma_f32_16x16x16_bf16 sweeps over K in ascending blocks of 16 with the
+// epilogue of k_gemm (alpha 1, + bias, ReLU, BF16 round-to-nearest-even), so
+// every value equals the unfused sequence of rgs_gemm_bf16 calls bit for bit;
+// then the bit x_o = [u < sigmoid(l)] and log q += log sigmoid(+-l) in FP64 as
+// k_sample_step does.  Outputs: the bits packed as uint64 words [B][2] (qubit
+// q = bit q % 64 of word q / 64), optionally as bf16 0/1 [B][n], and log q.
+constexpr int kSampRows = 64;   // proposals per workgroup (16 per wavefront)
+constexpr int kSampXS = 72;     // LDS row stride of the bits (n <= 64, padded)
+
+__global__ void __launch_bounds__(256) k_made_sample(const uint16_t* __restrict__ W1, const float* __restrict__ b1,
+                                                     const uint16_t* __restrict__ W2, const float* __restrict__ b2,
+                                                     const uint16_t* __restrict__ W3, const float* __restrict__ b3,
+                                                     const int* __restrict__ kpref, const double* __restrict__ u,
+                                                     int B, int n, int H, uint64_t* __restrict__ words,
+                                                     uint16_t* __restrict__ xout, double* __restrict__ logq) {
+  extern __shared__ __align__(16) uint16_t sm[];
+  const int HS = H + 8;
+  uint16_t* h1 = sm;                                  // [64][HS]
+  uint16_t* h2 = h1 + kSampRows * HS;                 // [64][HS]
+  uint16_t* w2s = h2 + kSampRows * HS;                // [16][HS]
+  uint16_t* xs = w2s + 16 * HS;                       // [64][kSampXS]
+  const int tid = threadIdx.x;
+  const int wave = tid >> 6, lane = tid & 63;
+  const int rw = wave * 16;                           // first LDS row of this wavefront
+  const int row0 = blockIdx.x * kSampRows;
+  const int c = lane & 15, kq = 4 * (lane >> 4);
+  for (int i = tid; i < kSampRows * kSampXS; i += 256) xs[i] = 0;
+  double lq[4] = {0.0, 0.0, 0.0, 0.0};
+  __syncthreads();
+  for (int o = 0; o < n; ++o) {
+    const int k0 = o ? kpref[o - 1] : 0, k1 = kpref[o];
+    const int dk = k1 - k0;
+    if (dk > 0) {
+      // stage the new rows of W2 (prefix columns) for the four wavefronts
+      for (int i = tid; i < 16 * k1; i += 256) {
+        const int rr = i / k1, cc = i - rr * k1;
+        w2s[rr * HS + cc] = rr < dk ? W2[(size_t)(k0 + rr) * H + cc] : (uint16_t)0;
+      }
+      // new first-layer units: [16 rows] x [K = o] x [16 units]
+      f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+      for (int kb = 0; kb < o; kb += 16) {
+        bf16x4 a, b;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          const int k = kb + kq + i;
+          a[i] = (short)(k < o ? xs[(rw + c) * kSampXS + k] : 0);
+          b[i] = (short)((c < dk && k < o) ? W1[(size_t)(k0 + c) * n + k] : 0);
+        }
+        acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc, 0, 0, 0);
+      }
+      if (c < dk) {
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+          float v = 1.0f * acc[t] + b1[k0 + c];
+          h1[(rw + kq + t) * HS + k0 + c] = f32_to_bf16(fmaxf(v, 0.f));
+        }
+      }
+      __syncthreads();
+      // new second-layer units: [16 rows] x [K = k1] x [16 units]
+      acc = (f32x4){0.f, 0.f, 0.f, 0.f};
+      for (int kb = 0; kb < k1; kb += 16) {
+        bf16x4 a, b;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+          const int k = kb + kq + i;
+          a[i] = (short)(k < k1 ? h1[(rw + c) * HS + k] : 0);
+          b[i] = (short)(k < k1 ? w2s[c * HS + k] : 0);
+        }
+        acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc, 0, 0, 0);
+      }
+      if (c < dk) {
+#pragma unroll
+        for (int t = 0; t < 4; ++t) {
+          float v = 1.0f * acc[t] + b2[k0 + c];
+          h2[(rw + kq + t) * HS + k0 + c] = f32_to_bf16(fmaxf(v, 0.f));
+        }
+      }
+      __syncthreads();
+    }
+    // logit o: [16 rows] x [K = k1] x [1 column] (other columns zero)
+    f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+    for (int kb = 0; kb < k1; kb += 16) {
+      bf16x4 a, b;
+#pragma unroll
+      for (int i = 0; i < 4; ++i) {
+        const int k = kb + kq + i;
+        a[i] = (short)(k < k1 ? h2[(rw + c) * HS + k] : 0);
+        b[i] = (short)((c == 0 && k < k1) ? W3[(size_t)o * H + k] : 0);
+      }
+      acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc, 0, 0, 0);
+    }
+    if (c == 0) {
+#pragma unroll
+      for (int t = 0; t < 4; ++t) {
+        const int r = row0 + rw + kq + t;
+        if (r < B) {
+          const float lf = 1.0f * acc[t] + b3[o];
+          const double l = (double)lf;
+          const double p = 1.0 / (1.0 + exp(-l));
+          const bool one = u[(size_t)r * n + o] < p;
+          const double sgn = one ? l : -l;
+          lq[t] += sgn >= 0.0 ? -log1p(exp(-sgn)) : sgn - log1p(exp(sgn));
+          xs[(rw + kq + t) * kSampXS + o] = one ? (uint16_t)0x3f80 : (uint16_t)0;
+        }
+      }
+    }
+    __syncthreads();
+  }
+  if (c == 0) {
+#pragma unroll
+    for (int t = 0; t < 4; ++t) {
+      const int lr = rw + kq + t, r = row0 + lr;
+      if (r >= B) continue;
+      uint64_t w0 = 0, w1 = 0;
+      for (int q = 0; q < n; ++q) {
+        const uint64_t bit = xs[lr * kSampXS + q] != 0;
+        if (q < 64) w0 |= bit << q;
+        else w1 |= bit << (q - 64);
+      }
+      if (words) {
+        words[2 * (size_t)r] = w0;
+        words[2 * (size_t)r + 1] = w1;
+      }
+      if (logq) logq[r] = lq[t];
+      if (xout)
+        for (int q = 0; q < n; ++q) xout[(size_t)r * n + q] = xs[lr * kSampXS + q];
+    }
+  }
+}
+
 __global__ void k_f32_to_bf16(const float* in, uint16_t* out, long long count) {
   long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < count) out[i] = f32_to_bf16(in[i]);
@@ -285,5 +423,27 @@ extern "C" int rgs_made_sample_step(const float* logit, int ldl, const double* u
   hipLaunchKernelGGL(k_sample_step, dim3((B + 255) / 256), dim3(256), 0, reinterpret_cast<hipStream_t>(stream),
                      logit, ldl, u, B, n, o, x, logq, first);
   RGS_LAUNCH_CHECK("k_sample_step");
+  return kOk;
+}
+
+extern "C" int rgs_made_sample(const uint16_t* W1, const float* b1, const uint16_t* W2, const float* b2,
+                               const uint16_t* W3, const float* b3, const int* kpref, const double* u, int B, int n,
+                               int H, uint64_t* words, uint16_t* xout, double* logq, void* stream) {
+  if (!W1 || !b1 || !W2 || !b2 || !W3 || !b3 || !kpref || !u || B <= 0 || n < 1 || n > 64 || H < 16 || H > 512 ||
+      H % 16 != 0 || (!words && !xout))
+    return fail(kInvalid, "made_sample: bad arguments (n <= 64, H <= 512 and a multiple of 16)");
+  const size_t HS = (size_t)H + 8;
+  const size_t lds = (2 * kSampRows * HS + 16 * HS + (size_t)kSampRows * kSampXS) * sizeof(uint16_t);
+  static bool attr = false;
+  if (!attr) {
+    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_made_sample),
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+    if (e != hipSuccess) return fail(kHip, std::string("made_sample: ") + hipGetErrorString(e));
+    attr = true;
+  }
+  hipLaunchKernelGGL(k_made_sample, dim3((B + kSampRows - 1) / kSampRows), dim3(256), lds,
+                     reinterpret_cast<hipStream_t>(stream), W1, b1, W2, b2, W3, b3, kpref, u, B, n, H, words, xout,
+                     logq);
+  RGS_LAUNCH_CHECK("k_made_sample");
   return kOk;
 }

@@ -47,12 +47,13 @@ def default_tuning(nbits):
     return dict(low_bits=min(12, nbits), tile_bits=14, lane_bits=6, round_bits=2)
 
 
-def resolve_tuning(nbits, device_name, tuning=None):
-    """Caller's tuning, else the autotuner's entry for this device and size,
-    else default_tuning; missing keys are filled from default_tuning."""
+def resolve_tuning(nbits, props, tuning=None):
+    """Caller's tuning, else the autotuner's entry for this device (by its
+    properties) and size, else default_tuning; missing keys are filled from
+    default_tuning."""
     if tuning is None:
-        from .autotune import lookup_name
-        entry = lookup_name(nbits, device_name)
+        from .autotune import lookup_props
+        entry = lookup_props(nbits, props)
         if entry is not None:
             tuning = {k: entry[k] for k in ('low_bits', 'tile_bits', 'lane_bits', 'round_bits') if k in entry}
     return dict(default_tuning(nbits), **(tuning or {}))
@@ -83,7 +84,7 @@ class _Slab:
         need = (8 << nbits) + (16 << max(0, nbits - mass_bits)) + (64 << 20)
         if free < need:
             raise MemoryError(f'device {device}: need {need} bytes, free {free}')
-        self.tuning = resolve_tuning(nbits, torch.cuda.get_device_name(self.device), tuning)
+        self.tuning = resolve_tuning(nbits, torch.cuda.get_device_properties(self.device), tuning)
         self.state = torch.empty(1 << nbits, dtype=torch.complex64, device=self.device)
         self.mass_bits = max(4, min(mass_bits, 8, nbits, effective_low_bits(nbits, self.tuning['low_bits'])))
         self.base_bits = nbits - self.mass_bits

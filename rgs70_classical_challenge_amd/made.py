@@ -29,6 +29,7 @@ _lib.register('rgs_adam', _I, [_V, _V, _V, _V, _V, _V, ctypes.c_longlong, _V, _F
 _lib.register('rgs_sample_col', _I, [_V, _V, _I, _I, _I, _V, _V])
 _lib.register('rgs_f32_to_bf16', _I, [_V, _V, ctypes.c_longlong, _V])
 _lib.register('rgs_made_sample_step', _I, [_V, _I, _V, _I, _I, _I, _V, _V, _I, _V])
+_lib.register('rgs_made_sample', _I, [_V, _V, _V, _V, _V, _V, _V, _V, _I, _I, _I, _V, _V, _V, _V])
 
 
 def hidden_degrees(n, hidden):
@@ -214,8 +215,26 @@ class MADE:
                                h2=torch.empty((B, H), dtype=torch.int16, device=self.dev),
                                logit=torch.empty(B, dtype=torch.float32, device=self.dev),
                                logq=torch.empty(B, dtype=torch.float64, device=self.dev),
-                               u=torch.empty((B, n), dtype=torch.float64, device=self.dev))
+                               u=torch.empty((B, n), dtype=torch.float64, device=self.dev),
+                               words=torch.empty((B, 2), dtype=torch.int64, device=self.dev))
         return sb
+
+    def sample_fused(self, b, bits=False):
+        """Draw b proposals from the uniforms in sb['u'][:b] with the fused
+        sampling kernel (one launch: 64 proposals per workgroup, activations
+        in LDS, the same arithmetic as the incremental GEMM sequence).
+        Returns device views (packed words int64 [b, 2] in the scan's layout,
+        log q float64 [b]) and, with bits=True, also writes the bf16 0/1 bits
+        to sb['x'][:b]."""
+        sb = self._sbuf
+        if not hasattr(self, '_kpref'):
+            k = np.searchsorted(hidden_degrees(self.n, self.H), np.arange(self.n), side='right').astype(np.int32)
+            self._kpref = self.torch.as_tensor(k).to(self.dev)
+        wb, w = self.wb, self.w
+        call('rgs_made_sample', self._p(wb, 0), self._p(w, 1), self._p(wb, 2), self._p(w, 3), self._p(wb, 4),
+             self._p(w, 5), ptr(self._kpref), ptr(sb['u']), b, self.n, self.H, ptr(sb['words']),
+             ptr(sb['x']) if bits else None, ptr(sb['logq']), self.stream.cuda_stream)
+        return sb['words'][:b], sb['logq'][:b]
 
     def _sample_steps(self, b, stream):
         """Enqueue the n incremental steps for the first b rows of the

@@ -142,9 +142,12 @@ class DeviceCappedRejection:
                 logq = None
             else:
                 sb['u'][:batch].uniform_(generator=gen)
-                x, logq = net.sample_device(batch)
+                words, logq = net.sample_fused(batch)
+                words = words.clone()
+                x = None
             u = torch.rand(batch, generator=gen, device=dev, dtype=torch.float64)
-            words = pack_words_device(x, n)
+            if x is not None:
+                words = pack_words_device(x, n)
             p = self.scorer.probabilities_device(words)
             if logq is None:
                 acc = torch.clamp(p * (2.0 ** n / cap), max=1.0)

@@ -182,3 +182,20 @@ def test_gate_factorisation_and_unit_rounding(golden_dir):
         e_b = dphc[(q // 8) * 256 + (1 << (q % 8))]            # single-qubit entry of the D2 table
         rec = np.diag(d1[q]) @ R @ np.diag([1.0, e_b])
         assert np.max(np.abs(rec - u[q])) < 3e-7, q
+
+
+def test_tuning_table_key_and_entries():
+    """The committed autotuner table is keyed by architecture and CU count and
+    holds a valid plan for every size it lists."""
+    import json
+    from types import SimpleNamespace
+    from rgs70_classical_challenge_amd import autotune
+    props = SimpleNamespace(gcnArchName='gfx950:sramecc+:xnack-', multi_processor_count=256)
+    assert autotune.device_key_of(props) == 'gfx950_256cu'
+    table = json.load(open(autotune.TABLE))
+    assert 'gfx950_256cu' in table
+    for n, e in table['gfx950_256cu'].items():
+        n = int(n)
+        _, text = describe(n, walsh=True, low=e['low_bits'], tile=e['tile_bits'], lane=e['lane_bits'],
+                           rounds=e.get('round_bits', 0))
+        assert text

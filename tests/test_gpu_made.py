@@ -158,3 +158,27 @@ def test_incremental_sampler_equals_full_forward():
     assert np.max(np.abs(logq - logq_full)) < 1e-4 * n
     # a decided bit follows its own logit: log q never exceeds 0
     assert np.all(logq <= 1e-12)
+
+
+def test_fused_sampler_equals_incremental_sequence():
+    """One-launch sampler against the launch-per-step incremental sampler:
+    identical bits and identical log q (same arithmetic, same order)."""
+    from rgs70_classical_challenge_amd.made import MADE
+    from rgs70_classical_challenge_amd.rejection import pack_words_device
+    for n, H, count in ((20, 128, 3000), (34, 512, 1000), (9, 64, 77)):
+        net = MADE(n, hidden=H, batch=256, seed=5)
+        rng = np.random.default_rng(n)
+        data = (rng.random((4096, n)) < np.linspace(0.15, 0.85, n)).astype(np.uint8)
+        net.train(data, 150, seed=3)
+        sb = net._sampling_buffers(count)
+        sb['u'][:count].copy_(torch.as_tensor(rng.random((count, n))))
+        x_inc, lq_inc = net.sample_device(count, graph=False)
+        words_inc = pack_words_device(x_inc, n).clone()
+        lq_inc = lq_inc.clone()
+        x_inc = x_inc.clone()
+        words, lq = net.sample_fused(count, bits=True)
+        torch.cuda.synchronize()
+        print(f'case n={n} H={H} count={count}', flush=True)
+        assert torch.equal(words, words_inc)
+        assert torch.equal(sb['x'][:count], x_inc)
+        assert torch.equal(lq, lq_inc)
