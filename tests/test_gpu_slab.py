@@ -104,12 +104,20 @@ def test_graph_state_pipelines(pipeline, n):
 
 def test_config1_thousand_draws_chi_square():
     """n=20 3-regular, 1000 draws qubit by qubit: grouped chi-square against
-    the complex128 distribution (64 equal-mass bins, cpu_draw_test design)."""
+    the complex128 distribution (16 equal-mass bins, cpu_draw_test design),
+    and the relative p(x) error of the complex64 state against complex128."""
     E = _engine()
     g, u = instance(20, 2026)
     ref = gs.full_state(g, u)
     s = E.GraphStateSampler(g, u)
     s.compute()
+    p_dev = od.probabilities(s.host())
+    pr_all = np.abs(ref) ** 2
+    typical = pr_all >= 2.0 ** -20                   # at least the mean probability
+    rel = np.abs(p_dev[typical] - pr_all[typical]) / pr_all[typical]
+    print(f'config 1 relative p error (p >= 2^-20): median {np.median(rel):.2e} '
+          f'p99 {np.quantile(rel, 0.99):.2e} max {rel.max():.2e}; norm {p_dev.sum():.10f}', flush=True)
+    assert np.median(rel) < 1e-6 and rel.max() < 1e-5
     rng = np.random.default_rng(7)
     idx, _ = s.sample(rng.random(1000))
     pr = np.abs(ref) ** 2
