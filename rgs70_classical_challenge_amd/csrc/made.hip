@@ -221,6 +221,9 @@ __global__ void k_sample_step(const float* logit, int ldl, const double* u, int 
 constexpr int kSampRows = 64;   // proposals per workgroup (16 per wavefront)
 constexpr int kSampXS = 72;     // LDS row stride of the bits (n <= 64, padded)
 
+// Four consecutive bf16 of an 8-byte aligned LDS or global address.
+__device__ __forceinline__ bf16x4 load4(const uint16_t* p) { return *reinterpret_cast<const bf16x4*>(p); }
+
 __global__ void __launch_bounds__(256) k_made_sample(const uint16_t* __restrict__ W1, const float* __restrict__ b1,
                                                      const uint16_t* __restrict__ W2, const float* __restrict__ b2,
                                                      const uint16_t* __restrict__ W3, const float* __restrict__ b3,
@@ -228,7 +231,7 @@ __global__ void __launch_bounds__(256) k_made_sample(const uint16_t* __restrict_
                                                      int B, int n, int H, uint64_t* __restrict__ words,
                                                      uint16_t* __restrict__ xout, double* __restrict__ logq) {
   extern __shared__ __align__(16) uint16_t sm[];
-  const int HS = H + 8;
+  const int HS = H + 8;                               // row stride: 2H + 16 bytes, a multiple of 8
   uint16_t* h1 = sm;                                  // [64][HS]
   uint16_t* h2 = h1 + kSampRows * HS;                 // [64][HS]
   uint16_t* w2s = h2 + kSampRows * HS;                // [16][HS]
@@ -238,26 +241,36 @@ __global__ void __launch_bounds__(256) k_made_sample(const uint16_t* __restrict_
   const int rw = wave * 16;                           // first LDS row of this wavefront
   const int row0 = blockIdx.x * kSampRows;
   const int c = lane & 15, kq = 4 * (lane >> 4);
-  for (int i = tid; i < kSampRows * kSampXS; i += 256) xs[i] = 0;
+  // Everything starts at zero: activations and staged weights beyond the
+  // current prefix K stay zero, and masked MADE weights are exactly zero, so
+  // the K loops below may run to the end of their 16-wide block without
+  // masking and still add exact zeros, as the zero-padded GEMM tiles do.
+  {
+    const int words16 = (2 * kSampRows * HS + 16 * HS + kSampRows * kSampXS) / 8;   // 16-byte units
+    uint4* z = reinterpret_cast<uint4*>(sm);
+    for (int i = tid; i < words16; i += 256) z[i] = make_uint4(0u, 0u, 0u, 0u);
+  }
   double lq[4] = {0.0, 0.0, 0.0, 0.0};
   __syncthreads();
   for (int o = 0; o < n; ++o) {
     const int k0 = o ? kpref[o - 1] : 0, k1 = kpref[o];
     const int dk = k1 - k0;
     if (dk > 0) {
-      // stage the new rows of W2 (prefix columns) for the four wavefronts
-      for (int i = tid; i < 16 * k1; i += 256) {
-        const int rr = i / k1, cc = i - rr * k1;
-        w2s[rr * HS + cc] = rr < dk ? W2[(size_t)(k0 + rr) * H + cc] : (uint16_t)0;
-      }
+      // stage the new rows of W2 (whole rows: columns >= k1 are masked zeros)
+      for (int rr = 0; rr < 16; ++rr)
+        for (int cc = 4 * tid; cc < H; cc += 4 * 256) {
+          bf16x4 v = {0, 0, 0, 0};
+          if (rr < dk) v = load4(W2 + (size_t)(k0 + rr) * H + cc);
+          *reinterpret_cast<bf16x4*>(w2s + rr * HS + cc) = v;
+        }
       // new first-layer units: [16 rows] x [K = o] x [16 units]
       f32x4 acc = {0.f, 0.f, 0.f, 0.f};
       for (int kb = 0; kb < o; kb += 16) {
-        bf16x4 a, b;
+        const bf16x4 a = load4(xs + (rw + c) * kSampXS + kb + kq);   // bits >= o are still zero
+        bf16x4 b;
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
           const int k = kb + kq + i;
-          a[i] = (short)(k < o ? xs[(rw + c) * kSampXS + k] : 0);
           b[i] = (short)((c < dk && k < o) ? W1[(size_t)(k0 + c) * n + k] : 0);
         }
         acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc, 0, 0, 0);
@@ -273,13 +286,8 @@ __global__ void __launch_bounds__(256) k_made_sample(const uint16_t* __restrict_
       // new second-layer units: [16 rows] x [K = k1] x [16 units]
       acc = (f32x4){0.f, 0.f, 0.f, 0.f};
       for (int kb = 0; kb < k1; kb += 16) {
-        bf16x4 a, b;
-#pragma unroll
-        for (int i = 0; i < 4; ++i) {
-          const int k = kb + kq + i;
-          a[i] = (short)(k < k1 ? h1[(rw + c) * HS + k] : 0);
-          b[i] = (short)(k < k1 ? w2s[c * HS + k] : 0);
-        }
+        const bf16x4 a = load4(h1 + (rw + c) * HS + kb + kq);
+        const bf16x4 b = load4(w2s + c * HS + kb + kq);
         acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc, 0, 0, 0);
       }
       if (c < dk) {
@@ -294,13 +302,9 @@ __global__ void __launch_bounds__(256) k_made_sample(const uint16_t* __restrict_
     // logit o: [16 rows] x [K = k1] x [1 column] (other columns zero)
     f32x4 acc = {0.f, 0.f, 0.f, 0.f};
     for (int kb = 0; kb < k1; kb += 16) {
-      bf16x4 a, b;
-#pragma unroll
-      for (int i = 0; i < 4; ++i) {
-        const int k = kb + kq + i;
-        a[i] = (short)(k < k1 ? h2[(rw + c) * HS + k] : 0);
-        b[i] = (short)((c == 0 && k < k1) ? W3[(size_t)o * H + k] : 0);
-      }
+      const bf16x4 a = load4(h2 + (rw + c) * HS + kb + kq);
+      bf16x4 b = {0, 0, 0, 0};
+      if (c == 0) b = load4(W3 + (size_t)o * H + kb + kq);       // units of degree > o are masked zeros
       acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc, 0, 0, 0);
     }
     if (c == 0) {
