@@ -95,11 +95,15 @@ typedef struct rgs_slab_desc {
   int init_kind;               /* RGS_INIT_*                                */
   float init_scale;            /* GRAPH / DELTA                             */
   const uint64_t *init_lut;    /* dev GRAPH: phase words; COV: map_lut      */
-  const uint64_t *init_rows;   /* HOST uint64[nbits], required for GRAPH:
-                                  row a has bit b set iff the phase form
+  const uint64_t *init_rows;   /* HOST uint64[nbits], required for GRAPH and
+                                  COV: row a has bit b set iff the sign form
                                   holds x_a x_b (symmetric), bit a iff it
                                   holds the linear term x_a (the adjacency
-                                  rows for a graph phase)                    */
+                                  rows for a graph phase; for COV the form
+                                  of the sign words of init_lut)             */
+  const uint64_t *init_cols;   /* HOST uint64[nbits], required for COV:
+                                  column a of the linear map of init_lut
+                                  (its map word for x = e_a)                 */
   const float *init_wlut;      /* dev COV weights (complex64)               */
   int do_walsh;                /* unnormalised Walsh transform on all bits  */
   const uint64_t *phase_lut;   /* dev sign words applied after the Walsh,
@@ -122,6 +126,9 @@ typedef struct rgs_slab_desc {
                                   tile in LDS per round (0..2, applied where
                                   <= lane_bits and <= tile_bits - 10; fewer
                                   LDS bytes, more workgroups per CU)        */
+  int phase_round_bits;        /* the same for the strided pass that applies
+                                  the phase tables (0..2), or -1 to use
+                                  round_bits                                 */
 } rgs_slab_desc;
 
 /* Factor 2x2 unitaries u (complex128 [n][2][2]) as u = D1 R D2 with

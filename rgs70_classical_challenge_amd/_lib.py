@@ -32,6 +32,7 @@ class SlabDesc(ctypes.Structure):
         ('init_scale', c_float),
         ('init_lut', c_void_p),
         ('init_rows', c_void_p),
+        ('init_cols', c_void_p),
         ('init_wlut', c_void_p),
         ('do_walsh', c_int),
         ('phase_lut', c_void_p),
@@ -45,6 +46,7 @@ class SlabDesc(ctypes.Structure):
         ('tile_bits', c_int),
         ('lane_bits', c_int),
         ('round_bits', c_int),
+        ('phase_round_bits', c_int),
     ]
 
 

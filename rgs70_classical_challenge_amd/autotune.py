@@ -1,11 +1,13 @@
 """Tile autotuner for the slab pipeline.
 
 For a state size n it times every admissible (low_bits, tile_bits,
-lane_bits) choice of the fused pass schedule on the device - each candidate
-runs the whole pipeline a few times and keeps its best time - and records the
-fastest.  Results are cached per device name in ``tuned.json`` next to this
-file; engines built with ``tuning='auto'`` read the table and fall back to the
-default shape for sizes it does not hold.
+lane_bits, round_bits) choice of the fused pass schedule on the device - each
+candidate runs the whole pipeline a few times and keeps its best time - and
+then, for the fastest, the LDS rounds of the pass that applies the phase
+tables (phase_round_bits).  Results are kept per device (architecture and CU
+count) in ``tuned.json`` next to this file; engines read the table and fall
+back to the default shape for sizes it does not hold.  Each candidate prints
+one progress line to stderr.
 
     python -m rgs70_classical_challenge_amd.autotune 16 30   # tune n = 16..30
 """
@@ -65,34 +67,37 @@ def lookup(n, torch):
 
 
 def lookup_props(n, props):
-    """Entry for size n on a device with these properties, or None."""
+    """Entry for key n (a size, or 'c<m>' for a conditional pipeline) on a
+    device with these properties, or None."""
     if not os.path.exists(TABLE):
         return None
     return json.load(open(TABLE)).get(device_key_of(props), {}).get(str(n))
 
 
-def tune(n, reps=3, seed=7):
-    """Time every candidate for size n; returns (best tuning, report list)."""
+def tune_engine(eng, label, reps=3, extra_stage=True):
+    """Time every candidate tuning of an engine's pipeline (its slab's tuning is
+    swapped in place; the engine's own descriptor is left as it was);
+    returns ((best tuning, ms), report)."""
     import torch
     from . import _lib
-    from .engine import GraphStateSampler, ctypes_ref, effective_low_bits
-    from .instances import instance
-    g, u = instance(n, seed)
-    s = GraphStateSampler(g, u, pipeline='slab')
-    sl = s.slab
+    from .engine import ctypes_ref, effective_low_bits
+    sl = eng.slab
+    n = sl.nbits
     st = sl.stream
+    saved = dict(sl.tuning)
     report = []
     best = None
-    for cand in candidates(n):
+
+    def run(cand):
+        nonlocal best
         sl.tuning = dict(cand)
         try:
-            d = sl.desc(_lib.INIT_GRAPH, 2.0 ** (-n), init_lut=s._lut, init_rows=s._rows, walsh=True,
-                        gates=s._gates)
+            d = eng._make_desc()
             if sl.mass_bits > effective_low_bits(n, cand['low_bits']):
-                continue
+                return
             passes, text = sl.describe(d)
         except ValueError:
-            continue
+            return
         times = []
         for _ in range(reps + 1):
             e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
@@ -103,30 +108,72 @@ def tune(n, reps=3, seed=7):
             times.append(e0.elapsed_time(e1))
         ms = min(times[1:])
         report.append(dict(cand, passes=passes, ms=ms))
+        print(f'[autotune] {label} {cand} {passes} passes {ms:.3f} ms', file=sys.stderr, flush=True)
         if best is None or ms < best[1]:
             best = (dict(cand), ms)
-    del s
-    torch.cuda.empty_cache()
+
+    for cand in candidates(n):
+        run(cand)
+    if extra_stage and best is not None and n > best[0]['low_bits']:
+        for pr in (0, 1, 2):
+            run(dict(best[0], phase_round_bits=pr))
+    sl.tuning = saved
     return best, report
 
 
-def main(argv):
-    """argv: lo [hi [output path]] - the table is merged into the output file
-    (default: the package's tuned.json)."""
+def tune(n, reps=3, seed=7):
+    """Tune the plain pipeline (graph-phase init, Walsh, gates, masses) of a
+    seeded n-qubit instance; returns ((best tuning, ms), report)."""
     import torch
-    lo, hi = int(argv[0]), int(argv[1]) if len(argv) > 1 else int(argv[0])
-    out = argv[2] if len(argv) > 2 else TABLE
+    from .engine import GraphStateSampler
+    from .instances import instance
+    g, u = instance(n, seed)
+    s = GraphStateSampler(g, u, pipeline='slab')
+    out = tune_engine(s, f'n={n}', reps)
+    del s
+    torch.cuda.empty_cache()
+    return out
+
+
+def tune_conditional(g, u, A, B, reps=2):
+    """Tune the conditional pipeline (covariant init, Walsh, A-side phase,
+    gates, masses) of one instance; returns ((best tuning, ms), report)."""
+    import torch
+    from .engine import ConditionalSampler
+    eng = ConditionalSampler(g, u, A, B)
+    eng.compute([0] * len(A))
+    out = tune_engine(eng, f'conditional m={len(A)}', reps)
+    del eng
+    torch.cuda.empty_cache()
+    return out
+
+
+def main(argv):
+    """argv: lo [hi [output path]] - the plain pipeline for n = lo..hi; or
+    --conditional instance.npz [output path] - the conditional pipeline of the
+    instance in the file (arrays graph, measurement_unitaries, A, B), stored
+    under the key 'c<m>'.  The table is merged into the output file (default:
+    the package's tuned.json)."""
+    import torch
+    if argv and argv[0] == '--conditional':
+        d = np.load(argv[1])
+        out = argv[2] if len(argv) > 2 else TABLE
+        jobs = [(f'c{len(d["A"])}', lambda: tune_conditional(d['graph'], d['measurement_unitaries'],
+                                                             d['A'].tolist(), d['B'].tolist()))]
+    else:
+        lo, hi = int(argv[0]), int(argv[1]) if len(argv) > 1 else int(argv[0])
+        out = argv[2] if len(argv) > 2 else TABLE
+        jobs = [(str(n), (lambda n=n: tune(n))) for n in range(lo, hi + 1)]
     table = json.load(open(out)) if os.path.exists(out) else load_table()
-    key = device_key(torch)
-    dev = table.setdefault(key, {})
-    for n in range(lo, hi + 1):
+    dev = table.setdefault(device_key(torch), {})
+    for key, job in jobs:
         t0 = time.time()
-        (best, ms), report = tune(n)
-        dev[str(n)] = dict(best, ms=round(ms, 4), candidates=len(report))
-        print(json.dumps({'n': n, 'best': best, 'ms': round(ms, 4), 'tuned_in_s': round(time.time() - t0, 2)}),
+        (best, ms), report = job()
+        dev[key] = dict(best, ms=round(ms, 4), candidates=len(report))
+        print(json.dumps({'key': key, 'best': best, 'ms': round(ms, 4), 'tuned_in_s': round(time.time() - t0, 2)}),
               flush=True)
-    with open(out, 'w') as f:
-        json.dump(table, f, indent=1, sort_keys=True)
+        with open(out, 'w') as f:
+            json.dump(table, f, indent=1, sort_keys=True)
 
 
 if __name__ == '__main__':

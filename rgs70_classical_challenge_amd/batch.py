@@ -55,8 +55,14 @@ class BatchSampler:
         self.graphs = {}
 
     def _tuning_for(self, n):
-        # 'auto': the autotuner's table for this device (default_tuning where it has no entry)
-        return None if self.tuning == 'auto' else self.tuning
+        # 'auto': the autotuner's table for this device (default_tuning where it has no entry);
+        # 'default': the fixed default_tuning for every size, the untuned baseline
+        if self.tuning == 'auto':
+            return None
+        if self.tuning == 'default':
+            from .engine import default_tuning
+            return default_tuning(n)
+        return self.tuning
 
     def _build(self, wave):
         torch = self.torch
@@ -182,9 +188,19 @@ def main(argv):
     count = int(argv[0]) if argv else 256
     nmin = int(argv[1]) if len(argv) > 1 else 16
     nmax = int(argv[2]) if len(argv) > 2 else 30
-    bs = BatchSampler(uneven_instances(count, nmin=nmin, nmax=nmax), samples=1000)
-    res = bs.benchmark()
+    specs = uneven_instances(count, nmin=nmin, nmax=nmax)
+    res = BatchSampler(specs, samples=1000).benchmark()
     res['workload'] = f'{count} instances, n uniform in {nmin}..{nmax}, 1000 samples each'
+    # the same batch with the fixed default tile choices: what the autotuner's table buys
+    base = BatchSampler(specs, samples=1000, tuning='default').benchmark(reps=1)
+    res['default_tuning'] = {k: base[k] for k in ('eager_ms', 'graph_ms')}
+    res['autotuner_speedup'] = round(base['graph_ms'] / res['graph_ms'], 3)
+    # the launch-bound part of the batch alone (its small instances), where the graph matters
+    small = [sp for sp in specs if sp[0] <= 22]
+    if small:
+        sres = BatchSampler(small, samples=1000).benchmark()
+        res['small_instances'] = {'n_max': 22, 'instances': len(small),
+                                  **{k: sres[k] for k in ('eager_ms', 'graph_ms', 'graph_speedup')}}
     print(json.dumps(res), flush=True)
 
 

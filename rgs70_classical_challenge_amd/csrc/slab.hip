@@ -40,14 +40,13 @@ static int chunks_of(int bits) { return (bits + 7) / 8; }
 static size_t lds_bytes(const PassPlan& p, int nbits, const rgs_slab_desc* d) {
   size_t slots = ((size_t)1 << (p.tb - p.rb)) + ((size_t)1 << (p.tb - p.rb - 4));
   size_t bytes = slots * 8;
+  // LDS tables: the phase pass's sign and diagonal-phase words, the covariant
+  // init's weights (k_tile reads the init map and sign words from global memory)
   if (p.flags & F_PHASE) {
     if (d->phase_lut) bytes += (size_t)chunks_of(nbits) * 256 * 8;
     if (d->dphase_lut) bytes += (size_t)chunks_of(nbits) * 256 * 8;
   }
-  if (!(p.flags & F_LOAD)) {
-    if (d->init_kind == RGS_INIT_GRAPH) bytes += (size_t)chunks_of(nbits) * 256 * 8;
-    if (d->init_kind == RGS_INIT_COV) bytes += (size_t)chunks_of(nbits) * 256 * 24;
-  }
+  if (!(p.flags & F_LOAD) && d->init_kind == RGS_INIT_COV) bytes += (size_t)chunks_of(nbits) * 256 * 8;
   return bytes;
 }
 
@@ -59,6 +58,7 @@ static int plan_passes(const rgs_slab_desc* d, std::vector<PassPlan>& out, std::
   const int TS = d->tile_bits;
   const int L = d->lane_bits;
   if (d->round_bits < 0 || d->round_bits > 2) { err = "round_bits must be in 0..2"; return kInvalid; }
+  if (d->phase_round_bits < -1 || d->phase_round_bits > 2) { err = "phase_round_bits must be in -1..2"; return kInvalid; }
   std::vector<PassPlan> groups;
   if (n > T0) {
     if (TS < 6 || TS > 14) { err = "tile_bits must be in 6..14"; return kInvalid; }
@@ -125,6 +125,11 @@ static int plan_passes(const rgs_slab_desc* d, std::vector<PassPlan>& out, std::
   for (auto& p : out)
     if ((p.flags & ~F_LOAD) || !(p.flags & F_LOAD)) kept.push_back(p);
   out.swap(kept);
+  // the strided pass that applies the phase tables holds them in LDS besides
+  // the tile: it may take its own number of exchange rounds
+  if (d->phase_round_bits >= 0)
+    for (auto& p : out)
+      if (!p.low && (p.flags & F_PHASE)) p.rb = std::max(0, std::min(std::min(d->phase_round_bits, p.c), p.tb - 10));
   for (auto& p : out) {
     if (lds_bytes(p, n, d) > kLdsBudget) { err = "tile plus tables exceed 160 KiB of LDS; lower the tile bits"; return kInvalid; }
     if ((p.flags & F_MASS) && (d->mass_bits < p.rbits || d->mass_bits > std::min(p.tb, p.rbits + 6))) {
@@ -182,6 +187,8 @@ static int run_passes(const rgs_slab_desc* d, int first, int last, void* stream)
     return fail(kInvalid, "init tables missing");
   if (d->init_kind == RGS_INIT_COV && !d->init_wlut) return fail(kInvalid, "covariant weights missing");
   if (d->init_kind == RGS_INIT_GRAPH && !d->init_rows) return fail(kInvalid, "graph init needs its form rows");
+  if (d->init_kind == RGS_INIT_COV && (!d->init_rows || !d->init_cols))
+    return fail(kInvalid, "covariant init needs its form rows and map columns");
   if (d->phase_lut && !d->phase_rows) return fail(kInvalid, "phase_lut needs its form rows");
   if ((d->phase_lut || d->dphase_lut || d->init_kind == RGS_INIT_GRAPH || d->init_kind == RGS_INIT_COV) && n > 63)
     return fail(kInvalid, "lookup tables need nbits <= 63");
@@ -220,7 +227,9 @@ static int run_passes(const rgs_slab_desc* d, int first, int last, void* stream)
         pi[i] = global_bit(p, s1 + i);
         pp[i] = global_bit(p, sp + i);
       }
-      if (a.init_chunks && d->init_kind == RGS_INIT_GRAPH) form_tables(d->init_rows, pi, a.iq, &a.ifreg);
+      if (a.init_chunks) form_tables(d->init_rows, pi, a.iq, &a.ifreg);
+      if (a.init_chunks && d->init_kind == RGS_INIT_COV)
+        for (int i = 0; i < 4; ++i) a.icol[i] = d->init_cols[pi[i]];
       if (a.phase_chunks) form_tables(d->phase_rows, pp, a.pq, &a.pfreg);
     }
     a.mass_bits = d->mass_bits;

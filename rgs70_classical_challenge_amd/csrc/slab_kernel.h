@@ -49,6 +49,7 @@ struct PassArgs {
   // register offset r_j of register j.
   uint64_t iq[4];
   uint64_t pq[4];
+  uint64_t icol[4];           // COV: map columns N e_{p_i} of the register bits
   uint32_t ifreg;
   uint32_t pfreg;
   float init_scale;
@@ -90,21 +91,37 @@ __device__ __forceinline__ float2 dphase_of(uint64_t x, const float2* lut, int c
   return v;
 }
 
-// Per-amplitude init of the kinds without a separable form.
-__device__ __forceinline__ float2 init_value(uint64_t x, const PassArgs& a, const uint64_t* lut, const float2* w) {
-  if (a.init_kind == RGS_INIT_COV) {
-    uint64_t b = 0, acc = 0;
-    for (int c = 0; c < a.init_chunks; ++c) {
-      uint32_t e = (c << 8) | ((x >> (8 * c)) & 255u);
-      b ^= lut[2 * e];
-      acc ^= lut[2 * e + 1];
-    }
+// Covariant init g(z) = (-1)^{Q(z)} prod_c w_c[(N z)_c] of a thread's 16
+// registers z_j = x0 ^ r_j.  The map and the sign form are evaluated once per
+// thread from the chunk tables; the registers follow from linearity,
+// N z_j = N x0 ^ sum_i j_i col_i, and from the form rows (form_signs).  Only
+// the weight product is looked up per amplitude (w staged in LDS).
+__device__ __forceinline__ void cov_init(float2 (&v)[16], uint64_t x0, const PassArgs& a, const uint64_t* lut,
+                                         const float2* w) {
+  uint64_t b0 = 0, acc = 0;
+  for (int c = 0; c < a.init_chunks; ++c) {
+    const uint32_t e = (c << 8) | ((x0 >> (8 * c)) & 255u);
+    b0 ^= lut[2 * e];
+    acc ^= lut[2 * e + 1];
+  }
+  uint32_t sg = a.ifreg ^ ((__popcll(acc & (x0 | (1ull << 63))) & 1) ? 0xFFFFu : 0u);
+  if (__popcll(x0 & a.iq[0]) & 1) sg ^= 0xAAAAu;
+  if (__popcll(x0 & a.iq[1]) & 1) sg ^= 0xCCCCu;
+  if (__popcll(x0 & a.iq[2]) & 1) sg ^= 0xF0F0u;
+  if (__popcll(x0 & a.iq[3]) & 1) sg ^= 0xFF00u;
+  uint64_t bj[16];
+  bj[0] = b0;
+#pragma unroll
+  for (int i = 0; i < 4; ++i)
+#pragma unroll
+    for (int j = 1 << i; j < (2 << i); ++j) bj[j] = bj[j - (1 << i)] ^ a.icol[i];
+#pragma unroll
+  for (int j = 0; j < 16; ++j) {
+    const uint64_t b = bj[j];
     float2 val = w[b & 255u];
     for (int c = 1; c < a.init_chunks; ++c) val = cmul(val, w[(c << 8) | ((b >> (8 * c)) & 255u)]);
-    return cneg_if(val, (uint32_t)__popcll(acc & (x | (1ull << 63))) & 1u);
+    v[j] = cneg_if(val, (sg >> j) & 1u);
   }
-  // RGS_INIT_DELTA
-  return x == 0 ? make_float2(a.init_scale, 0.f) : make_float2(0.f, 0.f);
 }
 
 // State traffic is streamed once per pass: non-temporal loads and stores keep
@@ -277,33 +294,26 @@ __global__ void __launch_bounds__(1 << (TB - 4)) k_tile(PassArgs a) {
   constexpr int SL = G::at(G::count() - 1);  // == C
   extern __shared__ __align__(16) unsigned char smem[];
   float2* lds = reinterpret_cast<float2*>(smem);
+  // LDS after the exchange slots: the phase pass's sign and diagonal-phase
+  // tables, and the covariant init's weights (read per amplitude).  The init
+  // map and sign tables are read once per thread, from global memory (they
+  // stay cache-resident), which keeps them out of the LDS budget.
   uint64_t* lut_phase = reinterpret_cast<uint64_t*>(smem + (size_t)G::slots * sizeof(float2));
   float2* lut_d = reinterpret_cast<float2*>(lut_phase + (size_t)a.phase_chunks * 256);
-  uint64_t* lut_init = reinterpret_cast<uint64_t*>(lut_d + (size_t)a.dphase_chunks * 256);
-  float2* lut_w =
-      reinterpret_cast<float2*>(lut_init + (size_t)a.init_chunks * 256 * (a.init_kind == RGS_INIT_COV ? 2 : 1));
+  float2* lut_w = lut_d + (size_t)a.dphase_chunks * 256;
+  const uint64_t* lut_init = a.init_lut;
   const uint32_t tid = threadIdx.x;
   const uint32_t tl = tid & ((1u << G::TLB) - 1u);
   const uint32_t tr = RBW ? __builtin_amdgcn_readfirstlane(tid >> G::TLB) : 0u;
 
-  const bool load = a.flags & F_LOAD;
-  bool staged = false;
-  if (a.phase_chunks) {
+  const bool cov = !(a.flags & F_LOAD) && a.init_kind == RGS_INIT_COV;
+  if (a.phase_chunks || a.dphase_chunks || cov) {
     for (int i = tid; i < a.phase_chunks * 256; i += G::NT) lut_phase[i] = a.phase_lut[i];
-    staged = true;
-  }
-  if (a.dphase_chunks) {
     for (int i = tid; i < a.dphase_chunks * 256; i += G::NT) lut_d[i] = a.dphase[i];
-    staged = true;
-  }
-  if (!load && a.init_chunks) {
-    const int words = a.init_chunks * 256 * (a.init_kind == RGS_INIT_COV ? 2 : 1);
-    for (int i = tid; i < words; i += G::NT) lut_init[i] = a.init_lut[i];
-    if (a.init_kind == RGS_INIT_COV)
+    if (cov)
       for (int i = tid; i < a.init_chunks * 256; i += G::NT) lut_w[i] = a.init_w[i];
-    staged = true;
+    __syncthreads();
   }
-  if (staged) __syncthreads();
 
   const int mid = a.qstart - C;
   const uint32_t cmask = (1u << C) - 1u;
@@ -324,9 +334,12 @@ __global__ void __launch_bounds__(1 << (TB - 4)) k_tile(PassArgs a) {
     const uint32_t e = form_signs(g0, lut_init, a.init_chunks, a.iq, a.ifreg);
 #pragma unroll
     for (int j = 0; j < 16; ++j) v[j] = cneg_if(make_float2(a.init_scale, 0.f), (e >> j) & 1u);
-  } else {
+  } else if (a.init_kind == RGS_INIT_COV) {
+    cov_init(v, g0, a, lut_init, lut_w);
+  } else {  // RGS_INIT_DELTA
 #pragma unroll
-    for (int j = 0; j < 16; ++j) v[j] = init_value(g0 + ((uint64_t)j << gsh), a, lut_init, lut_w);
+    for (int j = 0; j < 16; ++j)
+      v[j] = (g0 | ((uint64_t)j << gsh)) == 0 ? make_float2(a.init_scale, 0.f) : make_float2(0.f, 0.f);
   }
 
   // Diagonal phases on the registers of layout S.  Both are separable over

@@ -44,18 +44,22 @@ def default_tuning(nbits):
     one (the fastest measured full pipeline at n = 34: a 12-bit contiguous
     pass, strided passes of 14-bit tiles with 64-amplitude rows and their LDS
     exchanges in four rounds)."""
-    return dict(low_bits=min(12, nbits), tile_bits=14, lane_bits=6, round_bits=2)
+    return dict(low_bits=min(12, nbits), tile_bits=14, lane_bits=6, round_bits=2, phase_round_bits=-1)
 
 
-def resolve_tuning(nbits, props, tuning=None):
+def resolve_tuning(nbits, props, tuning=None, kind=''):
     """Caller's tuning, else the autotuner's entry for this device (by its
-    properties) and size, else default_tuning; missing keys are filled from
-    default_tuning."""
+    properties), pipeline kind ('' plain, 'c' conditional) and size - a
+    conditional engine falls back to the plain entry - else default_tuning;
+    missing keys are filled from default_tuning."""
     if tuning is None:
         from .autotune import lookup_props
-        entry = lookup_props(nbits, props)
+        entry = lookup_props(f'{kind}{nbits}', props) if kind else None
+        if entry is None:
+            entry = lookup_props(nbits, props)
         if entry is not None:
-            tuning = {k: entry[k] for k in ('low_bits', 'tile_bits', 'lane_bits', 'round_bits') if k in entry}
+            tuning = {k: entry[k] for k in ('low_bits', 'tile_bits', 'lane_bits', 'round_bits', 'phase_round_bits')
+                      if k in entry}
     return dict(default_tuning(nbits), **(tuning or {}))
 
 
@@ -72,7 +76,7 @@ def effective_low_bits(nbits, low_bits):
 class _Slab:
     """A 2^nbits complex64 state, its lookup tables and its mass pyramid on one device."""
 
-    def __init__(self, nbits, device, mass_bits=8, tuning=None):
+    def __init__(self, nbits, device, mass_bits=8, tuning=None, kind=''):
         torch = _torch()
         if not torch.cuda.is_available():
             raise RuntimeError('no HIP device visible')
@@ -84,7 +88,7 @@ class _Slab:
         need = (8 << nbits) + (16 << max(0, nbits - mass_bits)) + (64 << 20)
         if free < need:
             raise MemoryError(f'device {device}: need {need} bytes, free {free}')
-        self.tuning = resolve_tuning(nbits, torch.cuda.get_device_properties(self.device), tuning)
+        self.tuning = resolve_tuning(nbits, torch.cuda.get_device_properties(self.device), tuning, kind)
         self.state = torch.empty(1 << nbits, dtype=torch.complex64, device=self.device)
         self.mass_bits = max(4, min(mass_bits, 8, nbits, effective_low_bits(nbits, self.tuning['low_bits'])))
         self.base_bits = nbits - self.mass_bits
@@ -98,10 +102,11 @@ class _Slab:
         return t.to(self.device, non_blocking=False)
 
     def desc(self, init_kind, init_scale=0.0, init_lut=None, init_wlut=None, walsh=False, phase_lut=None,
-             gates=None, gate_limit=None, mass=True, init_rows=None, phase_rows=None):
+             gates=None, gate_limit=None, mass=True, init_rows=None, phase_rows=None, init_cols=None):
         """gates: a Gates object (rotations + diagonal phase) or None;
-        init_rows / phase_rows: host form rows (form_rows) of the graph-phase
-        tables init_lut (GRAPH init) and phase_lut."""
+        init_rows / phase_rows: host form rows (form_rows, cov_form) of the
+        sign tables init_lut (GRAPH and COV init) and phase_lut; init_cols:
+        host map columns of a COV init_lut (cov_form)."""
         d = _lib.SlabDesc()
         d.state = ptr(self.state)
         d.nbits = self.nbits
@@ -109,6 +114,7 @@ class _Slab:
         d.init_scale = init_scale
         d.init_lut = ptr(init_lut)
         d.init_rows = ptr(init_rows)
+        d.init_cols = ptr(init_cols)
         d.init_wlut = ptr(init_wlut)
         d.do_walsh = int(walsh)
         d.phase_lut = ptr(phase_lut)
@@ -122,8 +128,10 @@ class _Slab:
         d.tile_bits = self.tuning['tile_bits']
         d.lane_bits = self.tuning['lane_bits']
         d.round_bits = self.tuning.get('round_bits', 0)
+        d.phase_round_bits = self.tuning.get('phase_round_bits', -1)
         # the descriptor holds raw pointers: keep their owners alive with it
-        d._owners = (self.state, self.pyramid, init_lut, init_wlut, phase_lut, gates, init_rows, phase_rows)
+        d._owners = (self.state, self.pyramid, init_lut, init_wlut, phase_lut, gates, init_rows, phase_rows,
+                     init_cols)
         return d
 
     def run(self, d, mass=True):
@@ -209,6 +217,42 @@ def form_rows(g):
     return np.ascontiguousarray((g.astype(np.uint64) * w[None, :]).sum(axis=1, dtype=np.uint64))
 
 
+def cov_form(map_lut, m):
+    """Rows and columns of a covariant init table (rgs_cov_luts layout
+    [chunk][256][map word, sign word]), read off the table itself: column a
+    of the linear map is the map word of z = e_a; the sign form
+    Q(z) = parity((XOR of the sign words) & (z | 1 << 63)) is quadratic, so
+    row a holds bit b (b != a) iff Q(e_a ^ e_b) ^ Q(e_a) ^ Q(e_b) ^ Q(0) = 1
+    and bit a iff Q(e_a) ^ Q(0) = 1 (the form_rows convention)."""
+    lut = np.asarray(map_lut, np.uint64).reshape(-1, 256, 2)
+    chunks = lut.shape[0]
+    if m > 63 or chunks != (m + 7) // 8:
+        raise ValueError('table does not match the bit count')
+    top = 1 << 63
+
+    def entry(z, k):
+        acc = 0
+        for c in range(chunks):
+            acc ^= int(lut[c, (z >> (8 * c)) & 255, k])
+        return acc
+
+    def q(z):
+        return bin(entry(z, 1) & (z | top)).count('1') & 1
+
+    q0 = q(0)
+    qa = [q(1 << a) for a in range(m)]
+    rows = np.zeros(m, np.uint64)
+    cols = np.zeros(m, np.uint64)
+    for a in range(m):
+        cols[a] = entry(1 << a, 0)
+        r = (qa[a] ^ q0) << a
+        for b in range(m):
+            if b != a and q((1 << a) | (1 << b)) ^ qa[a] ^ qa[b] ^ q0:
+                r |= 1 << b
+        rows[a] = r
+    return rows, cols
+
+
 def graph_phase_lut(g):
     g = np.ascontiguousarray(np.asarray(g, np.uint8))
     n = len(g)
@@ -253,18 +297,22 @@ class GraphStateSampler:
         self._lut = sl.to_device(graph_phase_lut(g))
         self._rows = form_rows(g)
         self._gates = Gates(sl, u @ HADAMARD if pipeline == 'slab' else u)
-        if pipeline == 'slab':
-            self._desc = sl.desc(_lib.INIT_GRAPH, 2.0 ** (-n), init_lut=self._lut, init_rows=self._rows, walsh=True,
-                                 gates=self._gates)
-        elif pipeline == 'literal':
-            # H^n = WHT / 2^{n/2}: the delta carries the normalisation
-            self._desc = sl.desc(_lib.INIT_DELTA, 2.0 ** (-n / 2), walsh=True, phase_lut=self._lut,
-                                 phase_rows=self._rows, gates=self._gates)
-        else:
-            self._desc = sl.desc(_lib.INIT_GRAPH, 2.0 ** (-n / 2), init_lut=self._lut, init_rows=self._rows,
-                                 gates=self._gates)
+        self._desc = self._make_desc()
         self.passes, self.plan_text = sl.describe(self._desc)
         self._norm = None
+
+    def _make_desc(self):
+        """Descriptor of this pipeline with the slab's current tuning."""
+        sl, n = self.slab, self.n
+        if self.pipeline == 'slab':
+            return sl.desc(_lib.INIT_GRAPH, 2.0 ** (-n), init_lut=self._lut, init_rows=self._rows, walsh=True,
+                           gates=self._gates)
+        if self.pipeline == 'literal':
+            # H^n = WHT / 2^{n/2}: the delta carries the normalisation
+            return sl.desc(_lib.INIT_DELTA, 2.0 ** (-n / 2), walsh=True, phase_lut=self._lut,
+                           phase_rows=self._rows, gates=self._gates)
+        return sl.desc(_lib.INIT_GRAPH, 2.0 ** (-n / 2), init_lut=self._lut, init_rows=self._rows,
+                       gates=self._gates)
 
     def compute(self):
         """Build the state and its mass pyramid; returns stage seconds."""
@@ -351,7 +399,7 @@ class ConditionalSampler:
         self.g, self.u, self.A, self.B, self.m = g, u, A, B, m
         self.ng = 1
         self.defer_top = defer_top
-        self.slab = _Slab(m, list(devices)[0], mass_bits, tuning)
+        self.slab = _Slab(m, list(devices)[0], mass_bits, tuning, kind='c')
         sl = self.slab
         chunks = (m + 7) // 8
         map_lut = np.zeros(chunks * 256 * 2, np.uint64)
@@ -360,6 +408,7 @@ class ConditionalSampler:
         Ba = np.asarray(B, np.int32)
         call('rgs_cov_luts', ptr(g), len(g), ptr(Aa), ptr(Ba), m, ptr(map_lut), ptr(aphase))
         self._map = sl.to_device(map_lut)
+        self._cov_rows, self._cov_cols = cov_form(map_lut, m)
         self._aphase = sl.to_device(aphase)
         self._arows = form_rows(g[np.ix_(A, A)])
         self._Ba = Ba
@@ -369,9 +418,16 @@ class ConditionalSampler:
         self._wlut = sl.to_device(self._wlut_host)
         self.deferred_global_unitary = _check_unitary(u[A[-1]]) if defer_top else None
         self.n = 1 << (m - 1)          # shard size of the reference's two-device layout
-        self._desc = sl.desc(_lib.INIT_COV, 0.0, init_lut=self._map, init_wlut=self._wlut, walsh=True,
-                             phase_lut=self._aphase, phase_rows=self._arows, gates=self._gates, gate_limit=m - 1 if defer_top else m)
+        self._desc = self._make_desc()
         self.passes, self.plan_text = sl.describe(self._desc)
+
+    def _make_desc(self):
+        """Descriptor of the conditional pipeline with the slab's current tuning."""
+        m = self.m
+        return self.slab.desc(_lib.INIT_COV, 0.0, init_lut=self._map, init_wlut=self._wlut, walsh=True,
+                              init_rows=self._cov_rows, init_cols=self._cov_cols, phase_lut=self._aphase,
+                              phase_rows=self._arows, gates=self._gates,
+                              gate_limit=m - 1 if self.defer_top else m)
 
     def compute(self, outB):
         if len(outB) != self.m:

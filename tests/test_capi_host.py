@@ -10,6 +10,7 @@ import pytest
 
 from oracle import graph_state as gs
 from rgs70_classical_challenge_amd import _lib
+from rgs70_classical_challenge_amd.engine import cov_form
 from rgs70_classical_challenge_amd.instances import instance
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
@@ -82,6 +83,12 @@ def test_cov_luts_match_oracle(golden_dir):
         ref = gs.phase_map_tables(m, N, Q, ell).reshape(-1)
         assert np.array_equal(map_lut, ref), name
         assert np.array_equal(aph, gs.graph_phase_tables(g[np.ix_(A, A)])[:, :, 1].reshape(-1))
+        # rows of the sign form and columns of the map, read off the table by
+        # the product's cov_form, against the oracle's Q, ell and N
+        rows, cols = cov_form(map_lut, m)
+        want = [r | (int(ell[a]) << a) for a, r in enumerate(gs.row_masks(Q | Q.T))]
+        assert rows.tolist() == want, name
+        assert cols.tolist() == gs.row_masks(np.asarray(N).T), name
         ob = np.random.default_rng(1).integers(0, 2, m).astype(np.uint8)
         w = np.zeros(chunks * 256 * 2, np.float32)
         uc = np.ascontiguousarray(u)
@@ -102,7 +109,7 @@ def test_cov_luts_reject_singular_cut():
 
 
 def describe(nbits, walsh, gates=True, phase=False, mass=True, init=_lib.INIT_GRAPH, low=14, tile=14, lane=4,
-             rounds=0):
+             rounds=0, phase_rounds=-1):
     d = _lib.SlabDesc()
     d.state = 1
     d.nbits = nbits
@@ -115,6 +122,7 @@ def describe(nbits, walsh, gates=True, phase=False, mass=True, init=_lib.INIT_GR
     d.mass = 1 if mass else None
     d.mass_bits = 8
     d.low_bits, d.tile_bits, d.lane_bits, d.round_bits = low, tile, lane, rounds
+    d.phase_round_bits = phase_rounds
     n = ctypes.c_int()
     buf = ctypes.create_string_buffer(4096)
     _lib.call('rgs_slab_describe', ctypes.byref(d), ctypes.byref(n), buf, 4096)
@@ -144,6 +152,15 @@ def test_pass_schedules():
     assert all(tok.endswith('rb2') for tok in text.split() if tok.startswith('strided')), text
     n, text = describe(20, walsh=True, low=10, tile=12, lane=4, rounds=2)
     assert all(tok.endswith('rb0') for tok in text.split() if tok.startswith('strided')), text  # 9-bit tiles
+    # the phase pass may take its own rounds; -1 keeps round_bits
+    n, text = describe(34, walsh=True, phase=True, low=13, tile=13, lane=6, rounds=0, phase_rounds=2)
+    toks = text.split()
+    assert [t for t in toks if '[phase]' in t][0].endswith('rb2'), text
+    assert all(t.endswith('rb0') for t in toks if t.startswith('strided') and '[phase]' not in t), text
+    n, text = describe(34, walsh=True, phase=True, low=13, tile=13, lane=6, rounds=1, phase_rounds=-1)
+    assert all(t.endswith('rb1') for t in text.split() if t.startswith('strided')), text
+    with pytest.raises(ValueError):
+        describe(34, walsh=True, phase=True, phase_rounds=3)
 
 
 def test_gate_factorisation_and_unit_rounding(golden_dir):
@@ -194,8 +211,11 @@ def test_tuning_table_key_and_entries():
     assert autotune.device_key_of(props) == 'gfx950_256cu'
     table = json.load(open(autotune.TABLE))
     assert 'gfx950_256cu' in table
-    for n, e in table['gfx950_256cu'].items():
-        n = int(n)
-        _, text = describe(n, walsh=True, low=e['low_bits'], tile=e['tile_bits'], lane=e['lane_bits'],
-                           rounds=e.get('round_bits', 0))
+    for key, e in table['gfx950_256cu'].items():
+        cond = key.startswith('c')           # conditional pipeline entries: 'c<m>'
+        n = int(key[1:] if cond else key)
+        init = _lib.INIT_COV if cond else _lib.INIT_GRAPH
+        _, text = describe(n, walsh=True, phase=cond, init=init, low=e['low_bits'], tile=e['tile_bits'],
+                           lane=e['lane_bits'], rounds=e.get('round_bits', 0),
+                           phase_rounds=e.get('phase_round_bits', -1))
         assert text
