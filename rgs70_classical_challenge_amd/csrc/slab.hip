@@ -213,6 +213,10 @@ static int run_passes(const rgs_slab_desc* d, int first, int last, void* stream)
     a.nbits = n;
     a.qstart = p.low ? 0 : p.qstart;
     a.flags = p.flags;
+    // a contiguous pass that makes the state and only transforms it walks up
+    // from the lowest layout (slab_kernel.h, F_INIT_UP)
+    const bool init_up = p.low && p.flags == F_WHT;
+    if (init_up) a.flags |= F_INIT_UP;
     a.init_kind = d->init_kind;
     const bool gen = !(p.flags & F_LOAD);
     a.init_chunks = (gen && (d->init_kind == RGS_INIT_GRAPH || d->init_kind == RGS_INIT_COV)) ? chunks_of(n) : 0;
@@ -222,9 +226,10 @@ static int run_passes(const rgs_slab_desc* d, int first, int last, void* stream)
     {
       const int s1 = std::max(p.tb - 4, p.c);
       const int sp = (p.flags & F_WHT) ? p.c : s1;  // layout where the phases are applied
+      const int si = init_up ? p.c : s1;            // layout where the state is made
       int pi[4], pp[4];
       for (int i = 0; i < 4; ++i) {
-        pi[i] = global_bit(p, s1 + i);
+        pi[i] = global_bit(p, si + i);
         pp[i] = global_bit(p, sp + i);
       }
       if (a.init_chunks) form_tables(d->init_rows, pi, a.iq, &a.ifreg);

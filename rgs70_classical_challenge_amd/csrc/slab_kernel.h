@@ -31,7 +31,10 @@
 
 namespace rgs {
 
-enum : int { F_LOAD = 1, F_WHT = 2, F_PHASE = 4, F_GATES = 8, F_MASS = 16 };
+// F_INIT_UP: a contiguous pass that makes the state and only applies the
+// Walsh butterflies builds its values in the lowest layout and walks up to
+// the store layout (one LDS exchange fewer than walking down and back).
+enum : int { F_LOAD = 1, F_WHT = 2, F_PHASE = 4, F_GATES = 8, F_MASS = 16, F_INIT_UP = 32 };
 
 constexpr size_t kLdsBudget = 160 * 1024;
 
@@ -263,6 +266,17 @@ __device__ __forceinline__ void walk_wht(float2 (&v)[16], float2* lds, uint32_t 
   }
 }
 
+// Walk up from layout K applying the Walsh butterflies (ends at layout 0 = S1).
+template <int TB, int C, int RBW, int K>
+__device__ __forceinline__ void walk_wht_up(float2 (&v)[16], float2* lds, uint32_t tl, uint32_t tr) {
+  using G = Geo<TB, C, RBW>;
+  wht_layout<G, K>(v);
+  if constexpr (K > 0) {
+    exchange<TB, C, RBW, G::at(K), G::at(K - 1)>(v, lds, tl, tr);
+    walk_wht_up<TB, C, RBW, K - 1>(v, lds, tl, tr);
+  }
+}
+
 // Rotations walking down from layout K (ends at layout count-1).
 template <int TB, int C, int RBW, int K>
 __device__ __forceinline__ void walk_rot_down(float2 (&v)[16], float2* lds, uint32_t tl, uint32_t tr,
@@ -326,20 +340,25 @@ __global__ void __launch_bounds__(1 << (TB - 4)) k_tile(PassArgs a) {
   const uint64_t g0 = gidx(local_base<S1, C, RBW>(tl, tr));
   const int gsh = a.qstart + S1 - C;
 
+  // a state made in place starts in layout SL when it walks up, else in S1
+  const bool up = C == 0 && (a.flags & F_INIT_UP);
+  const uint64_t gi = up ? gidx(local_base<SL, C, RBW>(tl, tr)) : g0;
+  const int gish = up ? a.qstart + SL - C : gsh;
+
   float2 v[16];
   if (a.flags & F_LOAD) {
 #pragma unroll
     for (int j = 0; j < 16; ++j) v[j] = load_stream(a.state + g0 + ((uint64_t)j << gsh));
   } else if (a.init_kind == RGS_INIT_GRAPH) {
-    const uint32_t e = form_signs(g0, lut_init, a.init_chunks, a.iq, a.ifreg);
+    const uint32_t e = form_signs(gi, lut_init, a.init_chunks, a.iq, a.ifreg);
 #pragma unroll
     for (int j = 0; j < 16; ++j) v[j] = cneg_if(make_float2(a.init_scale, 0.f), (e >> j) & 1u);
   } else if (a.init_kind == RGS_INIT_COV) {
-    cov_init(v, g0, a, lut_init, lut_w);
+    cov_init(v, gi, a, lut_init, lut_w);
   } else {  // RGS_INIT_DELTA
 #pragma unroll
     for (int j = 0; j < 16; ++j)
-      v[j] = (g0 | ((uint64_t)j << gsh)) == 0 ? make_float2(a.init_scale, 0.f) : make_float2(0.f, 0.f);
+      v[j] = (gi | ((uint64_t)j << gish)) == 0 ? make_float2(a.init_scale, 0.f) : make_float2(0.f, 0.f);
   }
 
   // Diagonal phases on the registers of layout S.  Both are separable over
@@ -371,7 +390,10 @@ __global__ void __launch_bounds__(1 << (TB - 4)) k_tile(PassArgs a) {
 
   // After the walks the registers are in layout S1 (top) or SL (bottom).
   bool at_top;
-  if (a.flags & F_WHT) {
+  if (up) {
+    if constexpr (C == 0) walk_wht_up<TB, C, RBW, G::count() - 1>(v, lds, tl, tr);
+    at_top = true;
+  } else if (a.flags & F_WHT) {
     walk_wht<TB, C, RBW, 0>(v, lds, tl, tr);
     if (a.flags & F_PHASE) apply_phase(std::integral_constant<int, SL>());
     if (a.flags & F_GATES) {
@@ -408,7 +430,19 @@ __global__ void __launch_bounds__(1 << (TB - 4)) k_tile(PassArgs a) {
       if ((tid & ((1u << sb) - 1u)) == 0) a.mass[(base >> a.mass_bits) + (tid >> sb)] = m;
     }
   }
-  if (!at_top) exchange<TB, C, RBW, SL, S1>(v, lds, tl, tr);
+  if (!at_top) {
+    if constexpr (C > 0) {
+      // every strided layout keeps the row bits on the lowest lanes, so the
+      // tile is stored from layout SL as coalesced as from S1
+      const uint64_t gl = gidx(local_base<SL, C, RBW>(tl, tr));
+      const int lsh = a.qstart + SL - C;
+#pragma unroll
+      for (int j = 0; j < 16; ++j) store_stream(a.state + gl + ((uint64_t)j << lsh), v[j]);
+      return;
+    } else {
+      exchange<TB, C, RBW, SL, S1>(v, lds, tl, tr);
+    }
+  }
 #pragma unroll
   for (int j = 0; j < 16; ++j) store_stream(a.state + g0 + ((uint64_t)j << gsh), v[j]);
 }
