@@ -42,10 +42,7 @@ static size_t lds_bytes(const PassPlan& p, int nbits, const rgs_slab_desc* d) {
   size_t bytes = slots * 8;
   // LDS tables: the phase pass's sign and diagonal-phase words, the covariant
   // init's weights (k_tile reads the init map and sign words from global memory)
-  if (p.flags & F_PHASE) {
-    if (d->phase_lut) bytes += (size_t)chunks_of(nbits) * 256 * 8;
-    if (d->dphase_lut) bytes += (size_t)chunks_of(nbits) * 256 * 8;
-  }
+  if ((p.flags & F_PHASE) && d->dphase_lut) bytes += (size_t)chunks_of(nbits) * 256 * 8;
   if (!(p.flags & F_LOAD) && d->init_kind == RGS_INIT_COV) bytes += (size_t)chunks_of(nbits) * 256 * 8;
   return bytes;
 }
@@ -173,6 +170,23 @@ static void form_tables(const uint64_t* rows, const int pos[4], uint64_t q[4], u
 // Global bit of local tile bit L.
 static int global_bit(const PassPlan& p, int L) { return L < p.c ? L : (p.low ? L : p.qstart + L - p.c); }
 
+// A sign form (symmetric rows, linear terms on the diagonal) restricted to the
+// tile's local bits: up[L] = linear term at bit L and cross terms with the
+// local bits above L; grow[L] = the global row of local bit L.
+static void local_form(const PassPlan& p, const uint64_t* rows, LocalForm& F) {
+  for (int L = 0; L < 16; ++L) {
+    F.up[L] = 0;
+    F.grow[L] = 0;
+  }
+  for (int L = 0; L < p.tb; ++L) {
+    const int g = global_bit(p, L);
+    F.grow[L] = rows[g];
+    uint32_t up = (uint32_t)((rows[g] >> g) & 1u) << L;
+    for (int L2 = L + 1; L2 < p.tb; ++L2) up |= (uint32_t)((rows[g] >> global_bit(p, L2)) & 1u) << L2;
+    F.up[L] = up;
+  }
+}
+
 static int run_passes(const rgs_slab_desc* d, int first, int last, void* stream) {
   if (!d || !d->state) return fail(kInvalid, "null descriptor or state");
   std::vector<PassPlan> plan;
@@ -233,9 +247,13 @@ static int run_passes(const rgs_slab_desc* d, int first, int last, void* stream)
         pp[i] = global_bit(p, sp + i);
       }
       if (a.init_chunks) form_tables(d->init_rows, pi, a.iq, &a.ifreg);
+      if (a.init_chunks && d->init_kind == RGS_INIT_GRAPH) local_form(p, d->init_rows, a.iform);
       if (a.init_chunks && d->init_kind == RGS_INIT_COV)
         for (int i = 0; i < 4; ++i) a.icol[i] = d->init_cols[pi[i]];
-      if (a.phase_chunks) form_tables(d->phase_rows, pp, a.pq, &a.pfreg);
+      if (a.phase_chunks) {
+        form_tables(d->phase_rows, pp, a.pq, &a.pfreg);
+        local_form(p, d->phase_rows, a.pform);
+      }
     }
     a.mass_bits = d->mass_bits;
     hipError_t e = launch_pass(a, p, lds_bytes(p, n, d), st);

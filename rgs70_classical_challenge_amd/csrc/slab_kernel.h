@@ -38,6 +38,15 @@ enum : int { F_LOAD = 1, F_WHT = 2, F_PHASE = 4, F_GATES = 8, F_MASS = 16, F_INI
 
 constexpr size_t kLdsBudget = 160 * 1024;
 
+// A quadratic sign form over GF(2) restricted to the local bits of a tile:
+// up[L] holds bit L (the linear term of local bit L) and bits L' > L (the
+// cross terms), in local coordinates; grow[L] is the form's global row of the
+// global bit that local bit L maps to.
+struct LocalForm {
+  uint32_t up[16];
+  uint64_t grow[16];
+};
+
 struct PassArgs {
   float2* state;
   const uint64_t* init_lut;   // GRAPH: phase words [ic][256]; COV: map+phase [ic][256][2]
@@ -53,6 +62,10 @@ struct PassArgs {
   uint64_t iq[4];
   uint64_t pq[4];
   uint64_t icol[4];           // COV: map columns N e_{p_i} of the register bits
+  // The graph-init and phase_lut sign forms over the tile's local bits
+  // (split_form_bit).
+  LocalForm iform;
+  LocalForm pform;
   uint32_t ifreg;
   uint32_t pfreg;
   float init_scale;
@@ -73,14 +86,31 @@ __device__ __forceinline__ uint32_t phase_parity(uint64_t x, const uint64_t* lut
   return (uint32_t)__popcll(acc & (x | (1ull << 63))) & 1u;
 }
 
-// Signs (-1)^{f(x0 + r_j)} of the 16 register values of a thread as a 16-bit
-// mask.  A quadratic form over GF(2) satisfies
-//   f(x0 ^ r) = f(x0) ^ f(r) ^ f(0) ^ B(x0, r),   B(x0, r) = sum_i r_i parity(x0 & q[i]),
-// when x0 and r share no set bits, so one table evaluation per thread plus
-// four popcounts replace a table evaluation per amplitude.
-__device__ __forceinline__ uint32_t form_signs(uint64_t x0, const uint64_t* lut, int chunks, const uint64_t* q,
-                                               uint32_t freg) {
-  uint32_t e = freg ^ (phase_parity(x0, lut, chunks) ? 0xFFFFu : 0u);
+// A quadratic form over GF(2) satisfies, for x and y with no common set bits,
+//   f(x ^ y) = f(x) ^ f(y) ^ f(0) ^ B(x, y),   B(x, y) = sum_{a in y} parity(x & row_a).
+// Split a thread's index x0 = base ^ t, with base the tile's own bits (the
+// same for every thread of the workgroup) and t the global image of the
+// thread's local bits lb: f(base) is one table evaluation at a uniform
+// address (fbase), the cross term is parity(lb & rb) with
+// rb_L = parity(base & grow[L]) (uniform), and f(t) ^ f(0) =
+// sum_{L in lb} parity(up[L] & lb) - the sign table is neither gathered per
+// thread nor staged per tile.
+template <int TB>
+__device__ __forceinline__ uint32_t split_form_bit(uint32_t fbase, uint64_t base, uint32_t lb, const LocalForm& F) {
+  uint32_t rb = 0;
+#pragma unroll
+  for (int L = 0; L < TB; ++L) rb |= (uint32_t)(__popcll(base & F.grow[L]) & 1) << L;
+  uint32_t s = fbase ^ ((uint32_t)__popc(lb & rb) & 1u);
+#pragma unroll
+  for (int L = 0; L < TB; ++L) s ^= (lb >> L) & (uint32_t)__popc(lb & F.up[L]) & 1u;
+  return s;
+}
+
+// Signs (-1)^{f(x0 ^ r_j)} of the 16 register values of a thread as a 16-bit
+// mask, from the sign s0 = f(x0) of register 0: the register offsets add
+// f(r_j) ^ f(0) (freg) and B(x0, r_j) = sum_i j_i parity(x0 & q[i]).
+__device__ __forceinline__ uint32_t register_signs(uint32_t s0, uint64_t x0, const uint64_t* q, uint32_t freg) {
+  uint32_t e = freg ^ (s0 ? 0xFFFFu : 0u);
   if (__popcll(x0 & q[0]) & 1) e ^= 0xAAAAu;
   if (__popcll(x0 & q[1]) & 1) e ^= 0xCCCCu;
   if (__popcll(x0 & q[2]) & 1) e ^= 0xF0F0u;
@@ -97,7 +127,7 @@ __device__ __forceinline__ float2 dphase_of(uint64_t x, const float2* lut, int c
 // Covariant init g(z) = (-1)^{Q(z)} prod_c w_c[(N z)_c] of a thread's 16
 // registers z_j = x0 ^ r_j.  The map and the sign form are evaluated once per
 // thread from the chunk tables; the registers follow from linearity,
-// N z_j = N x0 ^ sum_i j_i col_i, and from the form rows (form_signs).  Only
+// N z_j = N x0 ^ sum_i j_i col_i, and from the form rows (register_signs).  Only
 // the weight product is looked up per amplitude (w staged in LDS).
 __device__ __forceinline__ void cov_init(float2 (&v)[16], uint64_t x0, const PassArgs& a, const uint64_t* lut,
                                          const float2* w) {
@@ -107,11 +137,7 @@ __device__ __forceinline__ void cov_init(float2 (&v)[16], uint64_t x0, const Pas
     b0 ^= lut[2 * e];
     acc ^= lut[2 * e + 1];
   }
-  uint32_t sg = a.ifreg ^ ((__popcll(acc & (x0 | (1ull << 63))) & 1) ? 0xFFFFu : 0u);
-  if (__popcll(x0 & a.iq[0]) & 1) sg ^= 0xAAAAu;
-  if (__popcll(x0 & a.iq[1]) & 1) sg ^= 0xCCCCu;
-  if (__popcll(x0 & a.iq[2]) & 1) sg ^= 0xF0F0u;
-  if (__popcll(x0 & a.iq[3]) & 1) sg ^= 0xFF00u;
+  const uint32_t sg = register_signs((uint32_t)__popcll(acc & (x0 | (1ull << 63))) & 1u, x0, a.iq, a.ifreg);
   uint64_t bj[16];
   bj[0] = b0;
 #pragma unroll
@@ -308,12 +334,11 @@ __global__ void __launch_bounds__(1 << (TB - 4)) k_tile(PassArgs a) {
   constexpr int SL = G::at(G::count() - 1);  // == C
   extern __shared__ __align__(16) unsigned char smem[];
   float2* lds = reinterpret_cast<float2*>(smem);
-  // LDS after the exchange slots: the phase pass's sign and diagonal-phase
-  // tables, and the covariant init's weights (read per amplitude).  The init
-  // map and sign tables are read once per thread, from global memory (they
-  // stay cache-resident), which keeps them out of the LDS budget.
-  uint64_t* lut_phase = reinterpret_cast<uint64_t*>(smem + (size_t)G::slots * sizeof(float2));
-  float2* lut_d = reinterpret_cast<float2*>(lut_phase + (size_t)a.phase_chunks * 256);
+  // LDS after the exchange slots: the diagonal-phase table of the phase
+  // pass and the covariant init's weights (read per amplitude).  Sign tables
+  // are only read at the tile's own (uniform) index (split_form_bit), and the
+  // covariant map once per thread, from global memory.
+  float2* lut_d = reinterpret_cast<float2*>(smem + (size_t)G::slots * sizeof(float2));
   float2* lut_w = lut_d + (size_t)a.dphase_chunks * 256;
   const uint64_t* lut_init = a.init_lut;
   const uint32_t tid = threadIdx.x;
@@ -321,8 +346,7 @@ __global__ void __launch_bounds__(1 << (TB - 4)) k_tile(PassArgs a) {
   const uint32_t tr = RBW ? __builtin_amdgcn_readfirstlane(tid >> G::TLB) : 0u;
 
   const bool cov = !(a.flags & F_LOAD) && a.init_kind == RGS_INIT_COV;
-  if (a.phase_chunks || a.dphase_chunks || cov) {
-    for (int i = tid; i < a.phase_chunks * 256; i += G::NT) lut_phase[i] = a.phase_lut[i];
+  if (a.dphase_chunks || cov) {
     for (int i = tid; i < a.dphase_chunks * 256; i += G::NT) lut_d[i] = a.dphase[i];
     if (cov)
       for (int i = tid; i < a.init_chunks * 256; i += G::NT) lut_w[i] = a.init_w[i];
@@ -350,7 +374,9 @@ __global__ void __launch_bounds__(1 << (TB - 4)) k_tile(PassArgs a) {
 #pragma unroll
     for (int j = 0; j < 16; ++j) v[j] = load_stream(a.state + g0 + ((uint64_t)j << gsh));
   } else if (a.init_kind == RGS_INIT_GRAPH) {
-    const uint32_t e = form_signs(gi, lut_init, a.init_chunks, a.iq, a.ifreg);
+    const uint32_t li = up ? local_base<SL, C, RBW>(tl, tr) : local_base<S1, C, RBW>(tl, tr);
+    const uint32_t fb = phase_parity(base, a.init_lut, a.init_chunks);
+    const uint32_t e = register_signs(split_form_bit<TB>(fb, base, li, a.iform), gi, a.iq, a.ifreg);
 #pragma unroll
     for (int j = 0; j < 16; ++j) v[j] = cneg_if(make_float2(a.init_scale, 0.f), (e >> j) & 1u);
   } else if (a.init_kind == RGS_INIT_COV) {
@@ -362,7 +388,7 @@ __global__ void __launch_bounds__(1 << (TB - 4)) k_tile(PassArgs a) {
   }
 
   // Diagonal phases on the registers of layout S.  Both are separable over
-  // the register bits: the sign through form_signs, the complex phase as the
+  // the register bits: the sign through register_signs, the complex phase as the
   // thread's base phase times the single-bit phases of the register bits.
   auto apply_phase = [&](auto s_tag) {
     constexpr int S = decltype(s_tag)::value;
@@ -382,7 +408,9 @@ __global__ void __launch_bounds__(1 << (TB - 4)) k_tile(PassArgs a) {
       for (int j = 0; j < 16; ++j) v[j] = cmul(v[j], ph[j]);
     }
     if (a.phase_chunks) {
-      const uint32_t e = form_signs(gb, lut_phase, a.phase_chunks, a.pq, a.pfreg);
+      const uint32_t fb = phase_parity(base, a.phase_lut, a.phase_chunks);
+      const uint32_t e = register_signs(split_form_bit<TB>(fb, base, local_base<S, C, RBW>(tl, tr), a.pform), gb,
+                                        a.pq, a.pfreg);
 #pragma unroll
       for (int j = 0; j < 16; ++j) v[j] = cneg_if(v[j], (e >> j) & 1u);
     }

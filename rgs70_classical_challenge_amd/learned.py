@@ -24,9 +24,10 @@ def index_to_bits(idx, n):
 
 def run(n=34, seed=20261205, train_samples=200_000, steps=20_000, batch=512, hidden=512, cap=8.0,
         caps_learned=(1.0, 1.2, 1.4, 1.5, 1.6, 1.7, 1.8, 1.9, 2.0, 2.2, 2.5, 4.0, 8.0), accept_target=100_000,
-        proposal_batch=65536, lr=1e-3, order_seconds=5.0, log=print):
+        proposal_batch=65536, lr=1e-3, order_seconds=5.0, analysis_resamples=2000, log=print):
     import torch
     from .engine import GraphStateSampler
+    from .records import capped_statistics
     from .instances import instance
     from .made import MADE
     from .rejection import DeviceCappedRejection
@@ -73,7 +74,7 @@ def run(n=34, seed=20261205, train_samples=200_000, steps=20_000, batch=512, hid
 
     def summary(r):
         return {k: (round(v, 6) if isinstance(v, float) else v) for k, v in r.items()
-                if k not in ('bits', 'p', 'p_fp64')}
+                if k not in ('bits', 'p', 'p_fp64', 'z')}
 
     # warm both paths (graph capture, kernel attributes) outside the measurement
     DeviceCappedRejection(scorer, auditor, n, cap).run(seed + 1, 1, batch=proposal_batch)
@@ -83,10 +84,12 @@ def run(n=34, seed=20261205, train_samples=200_000, steps=20_000, batch=512, hid
     log(f'[learned] uniform: {out["uniform"]["accepted_per_s"]:.1f} accepted/s, XEB {uni["xeb"]:.4f} '
         f'+- {uni["xeb_se"]:.4f}')
     out['learned'] = {}
+    zs = {}
     for c in caps_learned:
         r = DeviceCappedRejection(scorer, auditor, n, c, proposal=net).run(seed + 3, accept_target,
                                                                            batch=proposal_batch)
         out['learned'][str(c)] = summary(r)
+        zs[float(c)] = r['z']
         log(f'[learned] cap {c}: {r["accepted_per_s"]:.1f} accepted/s, XEB {r["xeb"]:.4f} +- {r["xeb_se"]:.4f}')
     # learned caps whose XEB matches the uniform run within one standard error;
     # the fastest of them is the comparison
@@ -103,6 +106,10 @@ def run(n=34, seed=20261205, train_samples=200_000, steps=20_000, batch=512, hid
                                        'uniform_acceptance': uni['acceptance'],
                                        'speedup': best[0] / uni['accepted_per_s'],
                                        'matching_caps': sorted(c for _, c in match)}
+        # capped-run analysis of both accepted sets (analyze_challenge.py:11-15 semantics,
+        # fewer bootstrap resamples for 10^5 strings)
+        out['analysis'] = {'uniform': capped_statistics(uni['z'] - 1, resamples=analysis_resamples),
+                           'learned': capped_statistics(zs[best[1]] - 1, resamples=analysis_resamples)}
     else:
         out['equal_xeb_comparison'] = None
     return out

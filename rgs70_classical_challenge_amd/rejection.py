@@ -182,4 +182,4 @@ class DeviceCappedRejection:
         return dict(accepted=int(accepted), proposals=int(proposals), acceptance=accepted / proposals,
                     seconds=seconds, accepted_per_s=accepted / seconds, proposals_per_s=proposals / seconds,
                     xeb=float(z.mean() - 1), xeb_se=float(z.std(ddof=1) / np.sqrt(len(z))),
-                    decision_flips_fp64=flips, min_margin_fp64=float(min(margins)))
+                    decision_flips_fp64=flips, min_margin_fp64=float(min(margins)), z=z)

@@ -86,6 +86,16 @@ def main():
     z = [2.0 ** 70 * r['probability'] for r in fp64]
     (OUT / 'rgs70_batch1_group0_z.json').write_text(json.dumps({'z': z}))
     shutil.copyfile(REF / 'born' / 'batch1' / 'analysis' / 'summary.json', OUT / 'rgs70_batch1_summary.json')
+    # capped run (cap 8, part 0): the FP64 scores of its accepted strings and
+    # the recorded score summary of the capped-run analysis (analyze_challenge.py)
+    import csv
+    with open(WORK / 'cap8_archive' / 'analysis0' / 'accepted_scores.csv') as f:
+        acc = [dict(r) for r in csv.DictReader(f)]
+    cap8 = json.loads((WORK / 'cap8_archive' / 'analysis0' / 'summary.json').read_text())
+    (OUT / 'rgs70_cap8_part0_scores.json').write_text(json.dumps({
+        'p_fp64': [float(r['p_fp64']) for r in acc], 'xeb_fp64': [float(r['xeb_fp64']) for r in acc],
+        'cap': cap8['cap'], 'proposals': cap8['proposals'], 'accepted': cap8['accepted'],
+        'score': cap8['score']}))
     print('wrote fixtures to', OUT, 'summary keys', sorted(summ))
 
 

@@ -42,3 +42,45 @@ def test_rgs70_production_replay(golden_dir):
     assert summary['index_matches'] >= len(rows) - 1
     gc.collect()
     torch.cuda.empty_cache()
+
+
+def test_rgs70_production_records_from_the_seed(golden_dir, tmp_path):
+    """The production driver end to end on the recorded group's own PCG64
+    stream: same B bits, uniforms and RNG states as the reference's records,
+    the same drawn strings, and a record file its verifier replays clean."""
+    import torch
+    if not torch.cuda.is_available():
+        pytest.skip('no HIP device')
+    from rgs70_classical_challenge_amd import records as R
+    from rgs70_classical_challenge_amd.engine import ConditionalSampler
+    gc.collect()
+    torch.cuda.empty_cache()
+    free, _ = torch.cuda.mem_get_info()
+    need = (8 << 35) + (4 << 30)
+    if free < need:
+        pytest.skip(f'the 2^35 complex64 state needs {need} bytes of free device memory; {free} free')
+    d = np.load(f'{golden_dir}/rgs_graph.npz')
+    data = json.load(open(f'{golden_dir}/rgs70_born_records.json'))
+    group, A, B, golden = data['group'], data['A_vertices'], data['B_vertices'], data['records'][:3]
+    eng = ConditionalSampler(d['graph'], d['measurement_unitaries'], A, B)
+    out = tmp_path / 'group0.jsonl'
+    recs = R.produce(eng, 70, group, out, count=len(golden), log=lambda s: print(s, flush=True))
+    del eng
+    lines = R.read_jsonl(out)
+    assert 'metadata' in lines[0] and 'summary' in lines[-1] and len(lines) == len(golden) + 2
+    for r, g in zip(recs, golden):
+        for key in ('B_bits', 'draw_uniforms'):
+            assert r[key] == g[key], key
+        assert R.same_state(r['rngstate_before'], g['rngstate_before'])
+        assert R.same_state(r['rngstate_after'], g['rngstate_after'])
+        assert abs(r['conditional_norm'] - 1) < 1e-6
+    matches = [r['A_index'] == g['A_index'] for r, g in zip(recs, golden)]
+    assert sum(matches) >= len(golden) - 1, matches
+    for r, g, m in zip(recs, golden, matches):
+        if m:
+            assert r['bitstring_q0_left'] == g['bitstring_q0_left']
+            assert abs(r['probability'] - g['p_fp64']) / g['p_fp64'] < 2e-5
+    failures, _, _, _ = R.verify_group(lines, group, A, B, 70)
+    assert failures == []
+    gc.collect()
+    torch.cuda.empty_cache()
