@@ -79,6 +79,14 @@ int rgs_cov_luts(const uint8_t *g, int nv, const int32_t *A, const int32_t *B, i
 int rgs_cov_weight_lut(const double *u, int nv, const int32_t *B, const uint8_t *outB, int m,
                        double scale, float *wlut);
 
+/* The same products over chunks of chunk_bits (8 or 9) B bits:
+ * wlut[c * 2^chunk_bits + k], ceil(m / chunk_bits) chunks.  Nine-bit chunks
+ * make m = 35 four chunks instead of five: one table read and one complex
+ * product fewer per amplitude in the covariant init (same reference
+ * function as above). */
+int rgs_cov_weight_lut_bits(const double *u, int nv, const int32_t *B, const uint8_t *outB, int m,
+                            double scale, int chunk_bits, float *wlut);
+
 /* ------------------------------------------------------------------ *
  * Slab pipeline: init -> [Walsh] -> [diagonal phase] -> [2x2 gates] -> masses
  * on one 2^nbits complex64 state, fused into few HBM passes.
@@ -104,7 +112,8 @@ typedef struct rgs_slab_desc {
   const uint64_t *init_cols;   /* HOST uint64[nbits], required for COV:
                                   column a of the linear map of init_lut
                                   (its map word for x = e_a)                 */
-  const float *init_wlut;      /* dev COV weights (complex64)               */
+  const float *init_wlut;      /* dev COV weights (complex64), chunks of
+                                  init_wbits bits                            */
   int do_walsh;                /* unnormalised Walsh transform on all bits  */
   const uint64_t *phase_lut;   /* dev sign words applied after the Walsh,
                                   or NULL                                    */
@@ -129,6 +138,11 @@ typedef struct rgs_slab_desc {
   int phase_round_bits;        /* the same for the strided pass that applies
                                   the phase tables (0..2), or -1 to use
                                   round_bits                                 */
+  int dphase_global;           /* 1: the phase pass reads the diagonal-phase
+                                  table from global memory; 0: it stages the
+                                  table in LDS per tile (a tuning choice)    */
+  int init_wbits;              /* B bits per COV weight chunk: 8 or 9 (0 is
+                                  read as 8)                                 */
 } rgs_slab_desc;
 
 /* Factor 2x2 unitaries u (complex128 [n][2][2]) as u = D1 R D2 with

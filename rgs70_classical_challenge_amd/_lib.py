@@ -47,6 +47,8 @@ class SlabDesc(ctypes.Structure):
         ('lane_bits', c_int),
         ('round_bits', c_int),
         ('phase_round_bits', c_int),
+        ('dphase_global', c_int),
+        ('init_wbits', c_int),
     ]
 
 
@@ -60,6 +62,7 @@ SIGNATURES = {
     'rgs_cov_luts': (c_int, [c_void_p, c_int, c_void_p, c_void_p, c_int, c_void_p, c_void_p]),
     'rgs_gate_zyz': (c_int, [c_void_p, c_int, c_int, c_void_p, c_void_p, c_void_p]),
     'rgs_cov_weight_lut': (c_int, [c_void_p, c_int, c_void_p, c_void_p, c_int, c_double, c_void_p]),
+    'rgs_cov_weight_lut_bits': (c_int, [c_void_p, c_int, c_void_p, c_void_p, c_int, c_double, c_int, c_void_p]),
     'rgs_slab_pipeline': (c_int, [P(SlabDesc), c_void_p]),
     'rgs_slab_pass': (c_int, [P(SlabDesc), c_int, c_void_p]),
     'rgs_slab_describe': (c_int, [P(SlabDesc), P(c_int), c_char_p, c_int]),

@@ -4,7 +4,8 @@ For a state size n it times every admissible (low_bits, tile_bits,
 lane_bits, round_bits) choice of the fused pass schedule on the device - each
 candidate runs the whole pipeline a few times and keeps its best time - and
 then, for the fastest, the LDS rounds of the pass that applies the phase
-tables (phase_round_bits).  Results are kept per device (architecture and CU
+tables (phase_round_bits) and where that pass reads its diagonal-phase table
+(dphase_global).  Results are kept per device (architecture and CU
 count) in ``tuned.json`` next to this file; engines read the table and fall
 back to the default shape for sizes it does not hold.  Each candidate prints
 one progress line to stderr.
@@ -115,8 +116,10 @@ def tune_engine(eng, label, reps=3, extra_stage=True):
     for cand in candidates(n):
         run(cand)
     if extra_stage and best is not None and n > best[0]['low_bits']:
+        base = dict(best[0])
         for pr in (0, 1, 2):
-            run(dict(best[0], phase_round_bits=pr))
+            for dg in (0, 1):
+                run(dict(base, phase_round_bits=pr, dphase_global=dg))
     sl.tuning = saved
     return best, report
 

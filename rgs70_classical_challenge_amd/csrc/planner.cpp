@@ -294,15 +294,17 @@ extern "C" int rgs_gate_zyz(const double* u, int n, int gate_limit, float* rot, 
   return kOk;
 }
 
-extern "C" int rgs_cov_weight_lut(const double* u, int nv, const int32_t* B, const uint8_t* outB, int m,
-                                  double scale, float* wlut) {
+extern "C" int rgs_cov_weight_lut_bits(const double* u, int nv, const int32_t* B, const uint8_t* outB, int m,
+                                       double scale, int chunk_bits, float* wlut) {
   if (!u || !B || !outB || !wlut || m < 1 || m > 63) return fail(kInvalid, "cov_weight_lut: need 1 <= m <= 63");
-  const int chunks = (m + 7) / 8;
+  if (chunk_bits < 8 || chunk_bits > 9) return fail(kInvalid, "cov_weight_lut: chunk_bits must be 8 or 9");
+  const int cb = chunk_bits, size = 1 << cb;
+  const int chunks = (m + cb - 1) / cb;
   for (int c = 0; c < chunks; ++c) {
-    for (int code = 0; code < 256; ++code) {
+    for (int code = 0; code < size; ++code) {
       std::complex<double> w = c == 0 ? std::complex<double>(scale, 0.0) : std::complex<double>(1.0, 0.0);
-      for (int t = 0; t < 8; ++t) {
-        int j = 8 * c + t;
+      for (int t = 0; t < cb; ++t) {
+        int j = cb * c + t;
         if (j >= m) break;
         int site = B[j];
         if (site < 0 || site >= nv || outB[j] > 1) return fail(kInvalid, "bad site or output bit");
@@ -310,9 +312,14 @@ extern "C" int rgs_cov_weight_lut(const double* u, int nv, const int32_t* B, con
         const double* e = u + 2 * (site * 4 + outB[j] * 2 + col);
         w *= std::complex<double>(e[0], e[1]);
       }
-      wlut[2 * (c * 256 + code)] = (float)w.real();
-      wlut[2 * (c * 256 + code) + 1] = (float)w.imag();
+      wlut[2 * (c * size + code)] = (float)w.real();
+      wlut[2 * (c * size + code) + 1] = (float)w.imag();
     }
   }
   return kOk;
+}
+
+extern "C" int rgs_cov_weight_lut(const double* u, int nv, const int32_t* B, const uint8_t* outB, int m,
+                                  double scale, float* wlut) {
+  return rgs_cov_weight_lut_bits(u, nv, B, outB, m, scale, 8, wlut);
 }

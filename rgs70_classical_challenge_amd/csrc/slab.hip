@@ -40,10 +40,14 @@ static int chunks_of(int bits) { return (bits + 7) / 8; }
 static size_t lds_bytes(const PassPlan& p, int nbits, const rgs_slab_desc* d) {
   size_t slots = ((size_t)1 << (p.tb - p.rb)) + ((size_t)1 << (p.tb - p.rb - 4));
   size_t bytes = slots * 8;
-  // LDS tables: the phase pass's sign and diagonal-phase words, the covariant
-  // init's weights (k_tile reads the init map and sign words from global memory)
-  if ((p.flags & F_PHASE) && d->dphase_lut) bytes += (size_t)chunks_of(nbits) * 256 * 8;
-  if (!(p.flags & F_LOAD) && d->init_kind == RGS_INIT_COV) bytes += (size_t)chunks_of(nbits) * 256 * 8;
+  // LDS tables: the phase pass's diagonal-phase table (unless it is read from
+  // global memory) and the covariant init's weights; sign and map tables are
+  // read from global memory
+  if ((p.flags & F_PHASE) && d->dphase_lut && !d->dphase_global) bytes += (size_t)chunks_of(nbits) * 256 * 8;
+  if (!(p.flags & F_LOAD) && d->init_kind == RGS_INIT_COV) {
+    const int wb = d->init_wbits ? d->init_wbits : 8;
+    bytes += (size_t)((nbits + wb - 1) / wb) * ((size_t)1 << wb) * 8;
+  }
   return bytes;
 }
 
@@ -203,6 +207,8 @@ static int run_passes(const rgs_slab_desc* d, int first, int last, void* stream)
   if (d->init_kind == RGS_INIT_GRAPH && !d->init_rows) return fail(kInvalid, "graph init needs its form rows");
   if (d->init_kind == RGS_INIT_COV && (!d->init_rows || !d->init_cols))
     return fail(kInvalid, "covariant init needs its form rows and map columns");
+  if (d->init_wbits != 0 && d->init_wbits != 8 && d->init_wbits != 9)
+    return fail(kInvalid, "init_wbits must be 8 or 9");
   if (d->phase_lut && !d->phase_rows) return fail(kInvalid, "phase_lut needs its form rows");
   if ((d->phase_lut || d->dphase_lut || d->init_kind == RGS_INIT_GRAPH || d->init_kind == RGS_INIT_COV) && n > 63)
     return fail(kInvalid, "lookup tables need nbits <= 63");
@@ -256,6 +262,9 @@ static int run_passes(const rgs_slab_desc* d, int first, int last, void* stream)
       }
     }
     a.mass_bits = d->mass_bits;
+    a.dphase_global = d->dphase_global ? 1 : 0;
+    a.wbits = d->init_wbits ? d->init_wbits : 8;
+    a.wchunks = (n + a.wbits - 1) / a.wbits;
     hipError_t e = launch_pass(a, p, lds_bytes(p, n, d), st);
     if (e != hipSuccess) return fail(kHip, std::string("k_pass launch: ") + hipGetErrorString(e));
   }

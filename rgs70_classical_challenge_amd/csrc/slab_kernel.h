@@ -78,6 +78,9 @@ struct PassArgs {
   int dphase_chunks;
   int gate_limit;
   int mass_bits;
+  int dphase_global;  // read dphase from global memory instead of staging it
+  int wbits;          // COV weight chunk bits (8 or 9)
+  int wchunks;        // COV weight chunks
 };
 
 __device__ __forceinline__ uint32_t phase_parity(uint64_t x, const uint64_t* lut, int chunks) {
@@ -118,9 +121,24 @@ __device__ __forceinline__ uint32_t register_signs(uint32_t s0, uint64_t x0, con
   return e;
 }
 
-__device__ __forceinline__ float2 dphase_of(uint64_t x, const float2* lut, int chunks) {
-  float2 v = lut[x & 255u];
-  for (int c = 1; c < chunks; ++c) v = cmul(v, lut[(c << 8) | ((x >> (8 * c)) & 255u)]);
+// Address-space-qualified views of a table, so that a table read from LDS
+// and the same table read from global memory compile to LDS and global
+// instructions on their own code paths.
+typedef float f32x2 __attribute__((ext_vector_type(2)));
+typedef const f32x2 __attribute__((address_space(3)))* lds_f2;
+typedef const f32x2 __attribute__((address_space(1)))* glb_f2;
+
+__device__ __forceinline__ float2 tab_at(const float2* p, uint32_t i) { return p[i]; }
+template <typename P>
+__device__ __forceinline__ float2 tab_at(P p, uint32_t i) {
+  const f32x2 t = p[i];
+  return make_float2(t.x, t.y);
+}
+
+template <typename P>
+__device__ __forceinline__ float2 dphase_of(uint64_t x, P lut, int chunks) {
+  float2 v = tab_at(lut, (uint32_t)(x & 255u));
+  for (int c = 1; c < chunks; ++c) v = cmul(v, tab_at(lut, (c << 8) | ((x >> (8 * c)) & 255u)));
   return v;
 }
 
@@ -144,11 +162,13 @@ __device__ __forceinline__ void cov_init(float2 (&v)[16], uint64_t x0, const Pas
   for (int i = 0; i < 4; ++i)
 #pragma unroll
     for (int j = 1 << i; j < (2 << i); ++j) bj[j] = bj[j - (1 << i)] ^ a.icol[i];
+  const int wb = a.wbits;
+  const uint32_t wm = (1u << wb) - 1u;
 #pragma unroll
   for (int j = 0; j < 16; ++j) {
     const uint64_t b = bj[j];
-    float2 val = w[b & 255u];
-    for (int c = 1; c < a.init_chunks; ++c) val = cmul(val, w[(c << 8) | ((b >> (8 * c)) & 255u)]);
+    float2 val = w[b & wm];
+    for (int c = 1; c < a.wchunks; ++c) val = cmul(val, w[(c << wb) | ((b >> (wb * c)) & wm)]);
     v[j] = cneg_if(val, (sg >> j) & 1u);
   }
 }
@@ -156,7 +176,6 @@ __device__ __forceinline__ void cov_init(float2 (&v)[16], uint64_t x0, const Pas
 // State traffic is streamed once per pass: non-temporal loads and stores keep
 // it from displacing the lookup tables and the neighbouring tiles' rows in
 // the caches (measured: 4-7% faster strided passes at n = 34).
-typedef float f32x2 __attribute__((ext_vector_type(2)));
 __device__ __forceinline__ float2 load_stream(const float2* p) {
   const f32x2 t = __builtin_nontemporal_load(reinterpret_cast<const f32x2*>(p));
   return make_float2(t.x, t.y);
@@ -339,17 +358,21 @@ __global__ void __launch_bounds__(1 << (TB - 4)) k_tile(PassArgs a) {
   // are only read at the tile's own (uniform) index (split_form_bit), and the
   // covariant map once per thread, from global memory.
   float2* lut_d = reinterpret_cast<float2*>(smem + (size_t)G::slots * sizeof(float2));
-  float2* lut_w = lut_d + (size_t)a.dphase_chunks * 256;
+  float2* lut_w = lut_d + (a.dphase_global ? 0 : (size_t)a.dphase_chunks * 256);
   const uint64_t* lut_init = a.init_lut;
   const uint32_t tid = threadIdx.x;
   const uint32_t tl = tid & ((1u << G::TLB) - 1u);
   const uint32_t tr = RBW ? __builtin_amdgcn_readfirstlane(tid >> G::TLB) : 0u;
 
   const bool cov = !(a.flags & F_LOAD) && a.init_kind == RGS_INIT_COV;
-  if (a.dphase_chunks || cov) {
-    for (int i = tid; i < a.dphase_chunks * 256; i += G::NT) lut_d[i] = a.dphase[i];
+  // the diagonal-phase table is read through LDS or straight from global
+  // memory (a per-plan choice: which is faster depends on the pass's other
+  // work); each has its own code path, so LDS reads stay LDS instructions
+  if ((a.dphase_chunks && !a.dphase_global) || cov) {
+    if (!a.dphase_global)
+      for (int i = tid; i < a.dphase_chunks * 256; i += G::NT) lut_d[i] = a.dphase[i];
     if (cov)
-      for (int i = tid; i < a.init_chunks * 256; i += G::NT) lut_w[i] = a.init_w[i];
+      for (int i = tid; i < (a.wchunks << a.wbits); i += G::NT) lut_w[i] = a.init_w[i];
     __syncthreads();
   }
 
@@ -394,18 +417,24 @@ __global__ void __launch_bounds__(1 << (TB - 4)) k_tile(PassArgs a) {
     constexpr int S = decltype(s_tag)::value;
     const uint64_t gb = gidx(local_base<S, C, RBW>(tl, tr));
     const uint32_t sh = a.qstart + S - C;
-    if (a.dphase_chunks) {
+    auto apply_d = [&](auto tab) {
       float2 ph[16];
-      ph[0] = dphase_of(gb, lut_d, a.dphase_chunks);
+      ph[0] = dphase_of(gb, tab, a.dphase_chunks);
 #pragma unroll
       for (int i = 0; i < 4; ++i) {
         const uint32_t p = sh + i;
-        const float2 f = lut_d[((p >> 3) << 8) | (1u << (p & 7))];
+        const float2 f = tab_at(tab, ((p >> 3) << 8) | (1u << (p & 7)));
 #pragma unroll
         for (int j = 1 << i; j < (2 << i); ++j) ph[j] = cmul(ph[j - (1 << i)], f);
       }
 #pragma unroll
       for (int j = 0; j < 16; ++j) v[j] = cmul(v[j], ph[j]);
+    };
+    if (a.dphase_chunks) {
+      if (a.dphase_global)
+        apply_d((glb_f2)a.dphase);
+      else
+        apply_d((lds_f2)lut_d);
     }
     if (a.phase_chunks) {
       const uint32_t fb = phase_parity(base, a.phase_lut, a.phase_chunks);

@@ -44,7 +44,8 @@ def default_tuning(nbits):
     one (the fastest measured full pipeline at n = 34: a 12-bit contiguous
     pass, strided passes of 14-bit tiles with 64-amplitude rows and their LDS
     exchanges in four rounds)."""
-    return dict(low_bits=min(12, nbits), tile_bits=14, lane_bits=6, round_bits=2, phase_round_bits=-1)
+    return dict(low_bits=min(12, nbits), tile_bits=14, lane_bits=6, round_bits=2, phase_round_bits=-1,
+                dphase_global=0)
 
 
 def resolve_tuning(nbits, props, tuning=None, kind=''):
@@ -58,8 +59,8 @@ def resolve_tuning(nbits, props, tuning=None, kind=''):
         if entry is None:
             entry = lookup_props(nbits, props)
         if entry is not None:
-            tuning = {k: entry[k] for k in ('low_bits', 'tile_bits', 'lane_bits', 'round_bits', 'phase_round_bits')
-                      if k in entry}
+            tuning = {k: entry[k] for k in ('low_bits', 'tile_bits', 'lane_bits', 'round_bits', 'phase_round_bits',
+                                            'dphase_global') if k in entry}
     return dict(default_tuning(nbits), **(tuning or {}))
 
 
@@ -102,7 +103,8 @@ class _Slab:
         return t.to(self.device, non_blocking=False)
 
     def desc(self, init_kind, init_scale=0.0, init_lut=None, init_wlut=None, walsh=False, phase_lut=None,
-             gates=None, gate_limit=None, mass=True, init_rows=None, phase_rows=None, init_cols=None):
+             gates=None, gate_limit=None, mass=True, init_rows=None, phase_rows=None, init_cols=None,
+             init_wbits=8):
         """gates: a Gates object (rotations + diagonal phase) or None;
         init_rows / phase_rows: host form rows (form_rows, cov_form) of the
         sign tables init_lut (GRAPH and COV init) and phase_lut; init_cols:
@@ -115,6 +117,7 @@ class _Slab:
         d.init_lut = ptr(init_lut)
         d.init_rows = ptr(init_rows)
         d.init_cols = ptr(init_cols)
+        d.init_wbits = init_wbits
         d.init_wlut = ptr(init_wlut)
         d.do_walsh = int(walsh)
         d.phase_lut = ptr(phase_lut)
@@ -129,6 +132,7 @@ class _Slab:
         d.lane_bits = self.tuning['lane_bits']
         d.round_bits = self.tuning.get('round_bits', 0)
         d.phase_round_bits = self.tuning.get('phase_round_bits', -1)
+        d.dphase_global = int(self.tuning.get('dphase_global', 0))
         # the descriptor holds raw pointers: keep their owners alive with it
         d._owners = (self.state, self.pyramid, init_lut, init_wlut, phase_lut, gates, init_rows, phase_rows,
                      init_cols)
@@ -385,7 +389,7 @@ class ConditionalSampler:
     """
 
     def __init__(self, g, u, A, B, devices=(0,), chunk_mib=1, fusion_bits=5, lane_bits=5, lookup_bits=8,
-                 defer_top=True, mass_bits=8, tuning=None):
+                 defer_top=True, mass_bits=8, tuning=None, weight_bits=None):
         g = np.ascontiguousarray(np.asarray(g, np.uint8))
         u = np.ascontiguousarray(np.asarray(u, np.complex128))
         A, B = [int(x) for x in A], [int(x) for x in B]
@@ -414,7 +418,11 @@ class ConditionalSampler:
         self._Ba = Ba
         self._u = np.ascontiguousarray(u)
         self._gates = Gates(sl, u[A], m - 1 if defer_top else m)
-        self._wlut_host = np.zeros(chunks * 256 * 2, np.float32)
+        if weight_bits not in (None, 8, 9):
+            raise ValueError('weight_bits must be 8 or 9')
+        self._weight_bits = weight_bits
+        # room for either chunking (the descriptor names the one in use)
+        self._wlut_host = np.zeros(max(((m + 7) // 8) << 8, ((m + 8) // 9) << 9) * 2, np.float32)
         self._wlut = sl.to_device(self._wlut_host)
         self.deferred_global_unitary = _check_unitary(u[A[-1]]) if defer_top else None
         self.n = 1 << (m - 1)          # shard size of the reference's two-device layout
@@ -422,12 +430,18 @@ class ConditionalSampler:
         self.passes, self.plan_text = sl.describe(self._desc)
 
     def _make_desc(self):
-        """Descriptor of the conditional pipeline with the slab's current tuning."""
+        """Descriptor of the conditional pipeline with the slab's current tuning.
+        Weight chunks are 9 bits wide where that saves a chunk and the
+        contiguous tile leaves LDS room for the larger table, else 8."""
         m = self.m
+        wbits = self._weight_bits
+        if wbits is None:
+            low = effective_low_bits(m, self.slab.tuning['low_bits'])
+            wbits = 9 if (m + 8) // 9 < (m + 7) // 8 and low <= 12 else 8
         return self.slab.desc(_lib.INIT_COV, 0.0, init_lut=self._map, init_wlut=self._wlut, walsh=True,
                               init_rows=self._cov_rows, init_cols=self._cov_cols, phase_lut=self._aphase,
                               phase_rows=self._arows, gates=self._gates,
-                              gate_limit=m - 1 if self.defer_top else m)
+                              gate_limit=m - 1 if self.defer_top else m, init_wbits=wbits)
 
     def compute(self, outB):
         if len(outB) != self.m:
@@ -437,8 +451,8 @@ class ConditionalSampler:
             raise ValueError('outB must contain bits')
         stage = {}
         t = time.perf_counter()
-        call('rgs_cov_weight_lut', ptr(self._u), len(self.g), ptr(self._Ba), ptr(ob), self.m,
-             2.0 ** (-self.m / 2), ptr(self._wlut_host))
+        call('rgs_cov_weight_lut_bits', ptr(self._u), len(self.g), ptr(self._Ba), ptr(ob), self.m,
+             2.0 ** (-self.m / 2), self._desc.init_wbits, ptr(self._wlut_host))
         self._wlut.copy_(self.slab.torch.as_tensor(self._wlut_host))
         stage['weights'] = time.perf_counter() - t
         t = time.perf_counter()

@@ -99,6 +99,21 @@ def test_cov_luts_match_oracle(golden_dir):
         for c in range(chunks):
             valid = 1 << min(8, m - 8 * c)
             assert np.allclose(wc[c, :valid], wref[c, :valid], rtol=2e-7, atol=0)
+        # nine-bit chunks: the same weight for every B string as the pinned 8-bit tables
+        c9 = (m + 8) // 9
+        w9 = np.zeros(c9 * 512 * 2, np.float32)
+        _lib.call('rgs_cov_weight_lut_bits', _lib.ptr(uc), len(g), _lib.ptr(Ba), _lib.ptr(ob), m, 2.0 ** (-m / 2),
+                  9, _lib.ptr(w9))
+        w9 = w9.view(np.complex64).reshape(c9, 512)
+        w8 = np.zeros(chunks * 256 * 2, np.float32)
+        _lib.call('rgs_cov_weight_lut_bits', _lib.ptr(uc), len(g), _lib.ptr(Ba), _lib.ptr(ob), m, 2.0 ** (-m / 2),
+                  8, _lib.ptr(w8))
+        assert np.array_equal(w8, w)
+        for b in np.random.default_rng(3).integers(0, 1 << m, 300, dtype=np.int64):
+            b = int(b)
+            p8 = np.prod([complex(wc[c, (b >> (8 * c)) & 255]) for c in range(chunks)])
+            p9 = np.prod([complex(w9[c, (b >> (9 * c)) & 511]) for c in range(c9)])
+            assert abs(p9 - p8) <= 1e-6 * abs(p8), (name, b)
 
 
 def test_cov_luts_reject_singular_cut():

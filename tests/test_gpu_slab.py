@@ -79,7 +79,9 @@ TUNINGS = [None, dict(low_bits=8, tile_bits=8, lane_bits=3), dict(low_bits=10, t
            dict(low_bits=12, tile_bits=12, lane_bits=5),
            # LDS rounds in the strided passes (engaged where the group is wide enough)
            dict(low_bits=10, tile_bits=12, lane_bits=4, round_bits=2),
-           dict(low_bits=12, tile_bits=14, lane_bits=5, round_bits=2)]
+           dict(low_bits=12, tile_bits=14, lane_bits=5, round_bits=2),
+           # the phase pass with its own rounds, reading its diagonal phases from global memory
+           dict(low_bits=10, tile_bits=12, lane_bits=4, round_bits=2, phase_round_bits=0, dphase_global=1)]
 
 
 @pytest.mark.parametrize('pipeline', ['slab', 'literal', 'folded'])
@@ -170,16 +172,18 @@ def test_walsh_and_gates_on_arbitrary_input(tun):
         assert err < 2e-6, err
 
 
+@pytest.mark.parametrize('tuning,weight_bits', [(None, None), (dict(low_bits=8, tile_bits=11, lane_bits=3,
+                                                                    phase_round_bits=1, dphase_global=1), 9)])
 @pytest.mark.parametrize('name', ['induced_24q.npz', 'induced_40q.npz'])
-def test_conditional_sampler_matches_oracle(golden_dir, name):
+def test_conditional_sampler_matches_oracle(golden_dir, name, tuning, weight_bits):
     E = _engine()
     d = np.load(f'{golden_dir}/{name}')
     g, u, A, B = d['graph'], d['measurement_unitaries'], d['A'].tolist(), d['B'].tolist()
     rng = np.random.default_rng(2026103103)
-    eng = E.ConditionalSampler(g, u, A, B, [0])
+    eng = E.ConditionalSampler(g, u, A, B, [0], tuning=tuning, weight_bits=weight_bits)
     for trial in range(2):
         ob = rng.integers(0, 2, len(B)).tolist()
-        print(f'case {name} trial={trial}', flush=True)
+        print(f'case {name} trial={trial} tuning={tuning} plan={eng.plan_text}', flush=True)
         eng.compute(ob)
         raw, final = gs.deferred(g, u, A, B, ob)
         got = eng.host()
