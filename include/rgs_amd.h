@@ -143,6 +143,15 @@ typedef struct rgs_slab_desc {
                                   table in LDS per tile (a tuning choice)    */
   int init_wbits;              /* B bits per COV weight chunk: 8 or 9 (0 is
                                   read as 8)                                 */
+  int group_order;             /* with the Walsh transform: 0 walks the
+                                  strided groups from the low bits up (the
+                                  phase in the highest group), 1 from the
+                                  high bits down (the phase in the lowest)   */
+  int top_bits;                /* 0: strided groups as even as the tile
+                                  allows; else the group that applies the
+                                  phase tables has exactly this many bits
+                                  (4..tile_bits-lane_bits; with four it needs
+                                  no LDS exchange) and the others are even   */
 } rgs_slab_desc;
 
 /* Factor 2x2 unitaries u (complex128 [n][2][2]) as u = D1 R D2 with

@@ -49,6 +49,8 @@ class SlabDesc(ctypes.Structure):
         ('phase_round_bits', c_int),
         ('dphase_global', c_int),
         ('init_wbits', c_int),
+        ('group_order', c_int),
+        ('top_bits', c_int),
     ]
 
 

@@ -3,9 +3,11 @@
 For a state size n it times every admissible (low_bits, tile_bits,
 lane_bits, round_bits) choice of the fused pass schedule on the device - each
 candidate runs the whole pipeline a few times and keeps its best time - and
-then, for the fastest, the LDS rounds of the pass that applies the phase
-tables (phase_round_bits) and where that pass reads its diagonal-phase table
-(dphase_global).  Results are kept per device (architecture and CU
+then, for the fastest, a narrow phase group (top_bits, with the contiguous
+pass one bit wider or narrower), the LDS rounds of the pass that applies the
+phase tables (phase_round_bits), where that pass reads its diagonal-phase
+table (dphase_global), and the order in which the strided groups are walked
+(group_order; the reversed order wins ties within 0.5 %).  Results are kept per device (architecture and CU
 count) in ``tuned.json`` next to this file; engines read the table and fall
 back to the default shape for sizes it does not hold.  Each candidate prints
 one progress line to stderr.
@@ -116,10 +118,25 @@ def tune_engine(eng, label, reps=3, extra_stage=True):
     for cand in candidates(n):
         run(cand)
     if extra_stage and best is not None and n > best[0]['low_bits']:
+        # a narrow phase group (four bits need no LDS exchange), the rest even
+        base = dict(best[0])
+        for top in (4, 5, 6):
+            for low in sorted({base['low_bits'] - 1, base['low_bits'], base['low_bits'] + 1}):
+                if 6 <= low <= 14:
+                    run(dict(base, low_bits=low, top_bits=top))
         base = dict(best[0])
         for pr in (0, 1, 2):
             for dg in (0, 1):
                 run(dict(base, phase_round_bits=pr, dphase_global=dg))
+        # the groups walked from the high bits down put the VALU-heavy phase
+        # pass on the lowest strided bits, whose memory pattern is the fastest;
+        # within timing noise (0.5 %) of the best it is preferred
+        base, base_ms = dict(best[0]), best[1]
+        run(dict(base, group_order=1 - base.get('group_order', 0)))
+        last = report[-1] if report else None
+        if base.get('group_order', 0) == 0 and best[0] == base and last is not None and \
+                last.get('group_order') == 1 and last['ms'] <= 1.005 * base_ms:
+            best = (dict(base, group_order=1), last['ms'])
     sl.tuning = saved
     return best, report
 

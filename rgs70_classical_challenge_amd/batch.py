@@ -1,13 +1,15 @@
-"""Many uneven graph-state instances in one captured launch per memory wave.
+"""Many uneven graph-state instances in one captured HIP graph.
 
 Each instance (n qubits, its graph and unitaries) gets a state slab, mass
-pyramid, uniforms and outputs; instances are packed into waves whose total
-device footprint stays under a budget, and each wave is captured once as a
-HIP graph that forks its instances over several streams (default priority)
-and joins them - so the whole wave is one ``graphLaunch`` from the host.
+pyramid, uniforms and outputs.  Instances are packed into memory waves whose
+total device footprint stays under a budget.  ``run_single`` places every
+wave in one shared arena (wave k+1 reuses wave k's memory) and captures the
+whole batch as ONE graph: each wave forks its instances over several streams
+(default priority) and joins them before the next wave starts, so the batch
+is a single ``graphLaunch`` from the host.  ``run_graph`` captures one graph
+per wave; ``run_eager`` is the one-launch-sequence-per-instance baseline.
 Tile shapes per n come from the autotuner table (``autotune.py``) when
-present.  ``run_eager`` is the one-launch-sequence-per-instance baseline.
-The reference runs one state per process invocation
+present.  The reference runs one state per process invocation
 (work/rgs_attack/born_production.py:44-58); no batched counterpart exists.
 """
 import numpy as np
@@ -29,6 +31,32 @@ def uneven_instances(count=256, seed=20261204, nmin=16, nmax=30):
 
 def footprint(n, samples, mass_bits=8):
     return (8 << n) + (16 << max(0, n - mass_bits)) + samples * 24 + (1 << 16)
+
+
+class Arena:
+    """One device allocation handed out as aligned views (take), rewound
+    between memory waves (reset)."""
+
+    ALIGN = 256
+
+    def __init__(self, nbytes, device=0):
+        import torch
+        self.buf = torch.empty(nbytes, dtype=torch.uint8, device=torch.device('cuda', device))
+        self.offset = 0
+        self.high = 0
+
+    def take(self, count, dtype):
+        import torch
+        size = count * torch.empty((), dtype=dtype).element_size()
+        start = -(-self.offset // self.ALIGN) * self.ALIGN
+        if start + size > self.buf.numel():
+            raise MemoryError('arena exhausted')
+        self.offset = start + size
+        self.high = max(self.high, self.offset)
+        return self.buf[start:start + size].view(dtype)
+
+    def reset(self):
+        self.offset = 0
 
 
 class BatchSampler:
@@ -64,13 +92,13 @@ class BatchSampler:
             return default_tuning(n)
         return self.tuning
 
-    def _build(self, wave):
+    def _build(self, wave, arena=None):
         torch = self.torch
         items = []
         for i in wave:
             n, seed = self.specs[i]
             g, u = instance(n, seed)
-            s = GraphStateSampler(g, u, pipeline='slab', tuning=self._tuning_for(n))
+            s = GraphStateSampler(g, u, pipeline='slab', tuning=self._tuning_for(n), arena=arena)
             uni = torch.as_tensor(self.rng.random(self.samples)).to(s.slab.device)
             idx = torch.empty(self.samples, dtype=torch.int64, device=s.slab.device)
             pr = torch.empty(self.samples, dtype=torch.float64, device=s.slab.device)
@@ -143,6 +171,49 @@ class BatchSampler:
                 torch.cuda.empty_cache()
         return out
 
+    def _capture_all(self, waves):
+        """One graph for the whole batch: each wave forks onto the side
+        streams and joins back before the next wave (which reuses its memory)
+        is launched."""
+        torch = self.torch
+        main = torch.cuda.Stream()
+        graph = torch.cuda.CUDAGraph()
+        for items in waves:                  # warm the launch path outside capture
+            for k, it in enumerate(items):
+                self._launch(it, self.streams[k % len(self.streams)])
+            torch.cuda.synchronize()
+        with torch.cuda.stream(main):
+            graph.capture_begin()
+            for items in waves:
+                for st in self.streams:
+                    st.wait_stream(main)
+                for k, it in enumerate(items):
+                    self._launch(it, self.streams[k % len(self.streams)])
+                for st in self.streams:
+                    main.wait_stream(st)
+            graph.capture_end()
+        return graph
+
+    def build_single(self):
+        """Every wave's instances with their states in one shared arena (sized
+        for the largest wave), and the batch captured as one graph."""
+        sizes = [sum(footprint(self.specs[i][0], self.samples) for i in wave) for wave in self.waves]
+        self.arena = Arena(max(sizes) + (64 << 20))
+        waves = []
+        for wave in self.waves:
+            self.arena.reset()
+            waves.append(self._build(wave, arena=self.arena))
+        self.single = (self._capture_all(waves), waves)
+        return self.single
+
+    def run_single(self, replays=1):
+        """Replay the one-graph batch; returns {instance: drawn indices}."""
+        graph, waves = self.single if getattr(self, 'single', None) else self.build_single()
+        for _ in range(replays):
+            graph.replay()
+        self.torch.cuda.synchronize()
+        return {it['i']: it['idx'].cpu().numpy() for items in waves for it in items}
+
     def benchmark(self, reps=3):
         """Device time of the whole batch (state, masses, draws of every
         instance): one launch sequence per instance on one stream, against
@@ -173,13 +244,28 @@ class BatchSampler:
             graph_s += (time.perf_counter() - t) / reps
             del graph, items
             torch.cuda.empty_cache()
+        # the whole batch as one graph launch (waves share one arena)
+        graph, _ = self.build_single()
+        graph.replay()
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        for _ in range(reps):
+            graph.replay()
+        torch.cuda.synchronize()
+        single_s = (time.perf_counter() - t) / reps
+        self.single = None
+        del graph
+        self.arena = None
+        torch.cuda.empty_cache()
         total = len(self.specs) * self.samples
         amps = sum(1 << n for n, _ in self.specs)
         return {'instances': len(self.specs), 'samples_per_instance': self.samples, 'waves': len(self.waves),
                 'streams': len(self.streams), 'amplitudes': amps,
                 'eager_ms': round(eager_s * 1e3, 3), 'graph_ms': round(graph_s * 1e3, 3),
+                'single_graph_ms': round(single_s * 1e3, 3),
                 'eager_samples_per_s': round(total / eager_s, 1), 'graph_samples_per_s': round(total / graph_s, 1),
-                'graph_speedup': round(eager_s / graph_s, 3)}
+                'single_graph_samples_per_s': round(total / single_s, 1),
+                'graph_speedup': round(eager_s / graph_s, 3), 'single_graph_speedup': round(eager_s / single_s, 3)}
 
 
 def main(argv):
@@ -193,14 +279,15 @@ def main(argv):
     res['workload'] = f'{count} instances, n uniform in {nmin}..{nmax}, 1000 samples each'
     # the same batch with the fixed default tile choices: what the autotuner's table buys
     base = BatchSampler(specs, samples=1000, tuning='default').benchmark(reps=1)
-    res['default_tuning'] = {k: base[k] for k in ('eager_ms', 'graph_ms')}
-    res['autotuner_speedup'] = round(base['graph_ms'] / res['graph_ms'], 3)
+    res['default_tuning'] = {k: base[k] for k in ('eager_ms', 'graph_ms', 'single_graph_ms')}
+    res['autotuner_speedup'] = round(base['single_graph_ms'] / res['single_graph_ms'], 3)
     # the launch-bound part of the batch alone (its small instances), where the graph matters
     small = [sp for sp in specs if sp[0] <= 22]
     if small:
         sres = BatchSampler(small, samples=1000).benchmark()
         res['small_instances'] = {'n_max': 22, 'instances': len(small),
-                                  **{k: sres[k] for k in ('eager_ms', 'graph_ms', 'graph_speedup')}}
+                                  **{k: sres[k] for k in ('eager_ms', 'graph_ms', 'single_graph_ms', 'graph_speedup',
+                                                          'single_graph_speedup')}}
     print(json.dumps(res), flush=True)
 
 

@@ -70,10 +70,27 @@ static int plan_passes(const rgs_slab_desc* d, std::vector<PassPlan>& out, std::
     if (T0 < 6) { err = "state too small for a strided pass; raise low_bits"; return kInvalid; }
     const int F = TS - L;
     const int rem = n - T0;
-    const int ng = (rem + F - 1) / F;
+    // group widths: as even as the tile allows, or a phase group of exactly
+    // top_bits bits and the rest even
+    const int top = d->top_bits;
+    if (top < 0 || (top > 0 && (top < RBs || top > F))) {
+      err = "top_bits must be 0 or in 4..tile_bits-lane_bits";
+      return kInvalid;
+    }
+    std::vector<int> widths;
+    const int even = top > 0 && rem > top ? rem - top : rem;
+    const int ng = (even + F - 1) / F;
+    for (int i = 0; i < ng; ++i) widths.push_back(even / ng + (i < even % ng ? 1 : 0));
+    if (even != rem) {
+      // the phase group: the last walked with the Walsh transform (highest
+      // bits, or lowest with group_order 1), the first without
+      if (d->do_walsh && !d->group_order)
+        widths.push_back(top);
+      else
+        widths.insert(widths.begin(), top);
+    }
     int q = T0;
-    for (int i = 0; i < ng; ++i) {
-      int cnt = rem / ng + (i < rem % ng ? 1 : 0);
+    for (int cnt : widths) {
       PassPlan g{};
       g.low = 0;
       g.qstart = q;
@@ -110,11 +127,16 @@ static int plan_passes(const rgs_slab_desc* d, std::vector<PassPlan>& out, std::
   if (k == 0) {
     out.push_back(with(low, src | (walsh ? F_WHT : 0) | (phase ? F_PHASE : 0) | (gates ? F_GATES : 0) | (mass ? F_MASS : 0)));
   } else if (walsh) {
+    // the Walsh groups may run in either order (the transform factors over
+    // bits); the last one also applies the phase and its gates, and the
+    // others are walked back for their gates
+    std::vector<int> ord(k);
+    for (int i = 0; i < k; ++i) ord[i] = d->group_order ? k - 1 - i : i;
     out.push_back(with(low, src | F_WHT));
-    for (int i = 0; i + 1 < k; ++i) out.push_back(with(groups[i], F_LOAD | F_WHT));
-    out.push_back(with(groups[k - 1], F_LOAD | F_WHT | (phase ? F_PHASE : 0) | (gates ? F_GATES : 0)));
+    for (int i = 0; i + 1 < k; ++i) out.push_back(with(groups[ord[i]], F_LOAD | F_WHT));
+    out.push_back(with(groups[ord[k - 1]], F_LOAD | F_WHT | (phase ? F_PHASE : 0) | (gates ? F_GATES : 0)));
     if (gates)
-      for (int i = k - 2; i >= 0; --i) out.push_back(with(groups[i], F_LOAD | F_GATES));
+      for (int i = k - 2; i >= 0; --i) out.push_back(with(groups[ord[i]], F_LOAD | F_GATES));
     if (gates || mass) out.push_back(with(low, F_LOAD | (gates ? F_GATES : 0) | (mass ? F_MASS : 0)));
   } else {
     out.push_back(with(groups[0], src | (phase ? F_PHASE : 0) | (gates ? F_GATES : 0)));

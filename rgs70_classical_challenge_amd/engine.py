@@ -45,7 +45,7 @@ def default_tuning(nbits):
     pass, strided passes of 14-bit tiles with 64-amplitude rows and their LDS
     exchanges in four rounds)."""
     return dict(low_bits=min(12, nbits), tile_bits=14, lane_bits=6, round_bits=2, phase_round_bits=-1,
-                dphase_global=0)
+                dphase_global=0, top_bits=0, group_order=0)
 
 
 def resolve_tuning(nbits, props, tuning=None, kind=''):
@@ -60,7 +60,7 @@ def resolve_tuning(nbits, props, tuning=None, kind=''):
             entry = lookup_props(nbits, props)
         if entry is not None:
             tuning = {k: entry[k] for k in ('low_bits', 'tile_bits', 'lane_bits', 'round_bits', 'phase_round_bits',
-                                            'dphase_global') if k in entry}
+                                            'dphase_global', 'top_bits', 'group_order') if k in entry}
     return dict(default_tuning(nbits), **(tuning or {}))
 
 
@@ -77,7 +77,7 @@ def effective_low_bits(nbits, low_bits):
 class _Slab:
     """A 2^nbits complex64 state, its lookup tables and its mass pyramid on one device."""
 
-    def __init__(self, nbits, device, mass_bits=8, tuning=None, kind=''):
+    def __init__(self, nbits, device, mass_bits=8, tuning=None, kind='', arena=None):
         torch = _torch()
         if not torch.cuda.is_available():
             raise RuntimeError('no HIP device visible')
@@ -85,15 +85,19 @@ class _Slab:
         self.torch = torch
         self.nbits = nbits
         self.device = torch.device('cuda', device)
-        free, _total = torch.cuda.mem_get_info(self.device)
-        need = (8 << nbits) + (16 << max(0, nbits - mass_bits)) + (64 << 20)
-        if free < need:
-            raise MemoryError(f'device {device}: need {need} bytes, free {free}')
+        if arena is None:
+            free, _total = torch.cuda.mem_get_info(self.device)
+            need = (8 << nbits) + (16 << max(0, nbits - mass_bits)) + (64 << 20)
+            if free < need:
+                raise MemoryError(f'device {device}: need {need} bytes, free {free}')
         self.tuning = resolve_tuning(nbits, torch.cuda.get_device_properties(self.device), tuning, kind)
-        self.state = torch.empty(1 << nbits, dtype=torch.complex64, device=self.device)
         self.mass_bits = max(4, min(mass_bits, 8, nbits, effective_low_bits(nbits, self.tuning['low_bits'])))
         self.base_bits = nbits - self.mass_bits
-        self.pyramid = torch.empty((2 << self.base_bits) - 1, dtype=torch.float64, device=self.device)
+        # state and pyramid: own tensors, or views into a caller's arena (take(count, dtype))
+        take = arena.take if arena is not None else (
+            lambda count, dtype: torch.empty(count, dtype=dtype, device=self.device))
+        self.state = take(1 << nbits, torch.complex64)
+        self.pyramid = take((2 << self.base_bits) - 1, torch.float64)
         self.stream = torch.cuda.current_stream(self.device)
 
     def to_device(self, array, dtype=None):
@@ -133,6 +137,8 @@ class _Slab:
         d.round_bits = self.tuning.get('round_bits', 0)
         d.phase_round_bits = self.tuning.get('phase_round_bits', -1)
         d.dphase_global = int(self.tuning.get('dphase_global', 0))
+        d.top_bits = int(self.tuning.get('top_bits', 0))
+        d.group_order = int(self.tuning.get('group_order', 0))
         # the descriptor holds raw pointers: keep their owners alive with it
         d._owners = (self.state, self.pyramid, init_lut, init_wlut, phase_lut, gates, init_rows, phase_rows,
                      init_cols)
@@ -285,7 +291,7 @@ class GraphStateSampler:
 
     PIPELINES = ('slab', 'literal', 'folded')
 
-    def __init__(self, g, u, device=0, pipeline='slab', mass_bits=8, tuning=None):
+    def __init__(self, g, u, device=0, pipeline='slab', mass_bits=8, tuning=None, arena=None):
         g = np.asarray(g, np.uint8)
         u = np.asarray(u, np.complex128)
         n = len(g)
@@ -296,7 +302,7 @@ class GraphStateSampler:
         if pipeline not in self.PIPELINES:
             raise ValueError(f'pipeline must be one of {self.PIPELINES}')
         self.g, self.u, self.n, self.pipeline = g, u, n, pipeline
-        self.slab = _Slab(n, device, mass_bits, tuning)
+        self.slab = _Slab(n, device, mass_bits, tuning, arena=arena)
         sl = self.slab
         self._lut = sl.to_device(graph_phase_lut(g))
         self._rows = form_rows(g)

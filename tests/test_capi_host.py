@@ -124,7 +124,7 @@ def test_cov_luts_reject_singular_cut():
 
 
 def describe(nbits, walsh, gates=True, phase=False, mass=True, init=_lib.INIT_GRAPH, low=14, tile=14, lane=4,
-             rounds=0, phase_rounds=-1):
+             rounds=0, phase_rounds=-1, top=0, order=0):
     d = _lib.SlabDesc()
     d.state = 1
     d.nbits = nbits
@@ -138,6 +138,8 @@ def describe(nbits, walsh, gates=True, phase=False, mass=True, init=_lib.INIT_GR
     d.mass_bits = 8
     d.low_bits, d.tile_bits, d.lane_bits, d.round_bits = low, tile, lane, rounds
     d.phase_round_bits = phase_rounds
+    d.top_bits = top
+    d.group_order = order
     n = ctypes.c_int()
     buf = ctypes.create_string_buffer(4096)
     _lib.call('rgs_slab_describe', ctypes.byref(d), ctypes.byref(n), buf, 4096)
@@ -176,6 +178,22 @@ def test_pass_schedules():
     assert all(t.endswith('rb1') for t in text.split() if t.startswith('strided')), text
     with pytest.raises(ValueError):
         describe(34, walsh=True, phase=True, phase_rounds=3)
+    # a narrow phase group: last with the Walsh transform, first without
+    n, text = describe(34, walsh=True, phase=True, low=14, tile=14, lane=6, top=4)
+    assert text.split()[1:4] == ['strided[wht]:14+8/tb14/c6/rb0', 'strided[wht]:22+8/tb14/c6/rb0',
+                                 'strided[wht][phase][gates]:30+4/tb10/c6/rb0'], text
+    n, text = describe(34, walsh=False, phase=True, low=14, tile=14, lane=6, top=4)
+    assert text.split()[0].startswith('strided[init][phase][gates]:14+4/tb10'), text
+    for bad in (3, 9, -1):
+        with pytest.raises(ValueError):
+            describe(34, walsh=True, phase=True, low=14, tile=14, lane=6, top=bad)
+    # groups walked from the high bits down: the phase lands in the lowest group
+    n, text = describe(34, walsh=True, phase=True, low=13, tile=13, lane=6, order=1)
+    assert [t.split('/')[0] for t in text.split()] == [
+        'low[init][wht]:0+13', 'strided[wht]:27+7', 'strided[wht]:20+7', 'strided[wht][phase][gates]:13+7',
+        'strided[gates]:20+7', 'strided[gates]:27+7', 'low[gates][mass]:0+13'], text
+    n, text = describe(34, walsh=True, phase=True, low=14, tile=14, lane=6, top=4, order=1)
+    assert '[phase]' in text.split()[3] and text.split()[3].startswith('strided[wht][phase][gates]:14+4/tb10'), text
 
 
 def test_gate_factorisation_and_unit_rounding(golden_dir):
@@ -232,5 +250,6 @@ def test_tuning_table_key_and_entries():
         init = _lib.INIT_COV if cond else _lib.INIT_GRAPH
         _, text = describe(n, walsh=True, phase=cond, init=init, low=e['low_bits'], tile=e['tile_bits'],
                            lane=e['lane_bits'], rounds=e.get('round_bits', 0),
-                           phase_rounds=e.get('phase_round_bits', -1))
+                           phase_rounds=e.get('phase_round_bits', -1), top=e.get('top_bits', 0),
+                           order=e.get('group_order', 0))
         assert text

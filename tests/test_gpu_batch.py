@@ -1,5 +1,6 @@
-"""Uneven batched instances (one captured multi-stream graph per memory wave)
-against one launch sequence per instance and against the oracle."""
+"""Uneven batched instances (one captured multi-stream graph per memory wave,
+or one graph for the whole batch over a shared arena) against one launch
+sequence per instance and against the oracle."""
 import numpy as np
 import pytest
 
@@ -52,6 +53,31 @@ def test_graph_replays_match_eager_and_oracle():
             levels = od.pyramid(od.probabilities(raw))
             exp_idx, _ = od.tree_draws(levels, it['uni'].cpu().numpy())
             assert np.array_equal(eager[it['i']], exp_idx), it['i']
+
+
+def test_single_graph_over_shared_arena_matches_eager():
+    """The whole batch as one graph: waves share one arena (later waves reuse
+    the memory of earlier ones) and every instance draws what its eager
+    launch sequence draws, on every replay."""
+    torch = _torch()
+    from rgs70_classical_challenge_amd.batch import BatchSampler, footprint, uneven_instances
+    specs = uneven_instances(count=12, seed=78, nmin=10, nmax=16)
+    budget = max(footprint(n, 300) for n, _ in specs) * 3      # several waves
+    eager = BatchSampler(specs, samples=300, streams=3, seed=6, budget_bytes=budget).run_eager()
+    bs = BatchSampler(specs, samples=300, streams=3, seed=6, budget_bytes=budget)
+    assert len(bs.waves) >= 2
+    print('waves', [len(w) for w in bs.waves], flush=True)
+    graph, waves = bs.build_single()
+    assert bs.arena.high <= max(sum(footprint(specs[i][0], 300) for i in w) for w in bs.waves) + (64 << 20)
+    for rep in range(3):
+        for items in waves:
+            for it in items:
+                it['idx'].fill_(-1)                             # eager work between replays
+        torch.cuda.synchronize()
+        got = bs.run_single()
+        assert set(got) == set(eager)
+        for i in eager:
+            assert np.array_equal(got[i], eager[i]), (rep, i)
 
 
 def test_waves_respect_the_memory_budget():

@@ -81,7 +81,11 @@ TUNINGS = [None, dict(low_bits=8, tile_bits=8, lane_bits=3), dict(low_bits=10, t
            dict(low_bits=10, tile_bits=12, lane_bits=4, round_bits=2),
            dict(low_bits=12, tile_bits=14, lane_bits=5, round_bits=2),
            # the phase pass with its own rounds, reading its diagonal phases from global memory
-           dict(low_bits=10, tile_bits=12, lane_bits=4, round_bits=2, phase_round_bits=0, dphase_global=1)]
+           dict(low_bits=10, tile_bits=12, lane_bits=4, round_bits=2, phase_round_bits=0, dphase_global=1),
+           # a four-bit phase group (no LDS exchange in that pass)
+           dict(low_bits=8, tile_bits=12, lane_bits=4, top_bits=4),
+           # groups walked from the high bits down (the phase in the lowest group)
+           dict(low_bits=8, tile_bits=10, lane_bits=3, group_order=1)]
 
 
 @pytest.mark.parametrize('pipeline', ['slab', 'literal', 'folded'])
