@@ -1,4 +1,4 @@
-"""Capped rejection sampling with exact p(x) as the judge.
+"""Capped rejection sampling, every acceptance decided by the exact p(x).
 
 Uniform proposals follow the reference's rule and RNG schedule
 (work/rgs_attack/rejection_sampler.py:19-24): per proposal
