@@ -1,11 +1,11 @@
 // Batched linear rank-width scan: p(x) of many proposals in one persistent
-// launch.  One wavefront owns one proposal at a time and sweeps all n steps
-// with the environment table resident in LDS (2^rmax entries); outputs of a
-// step are gathered into registers (K per lane), the wave computes their
-// maximum modulus, and they are written back scaled by an exact power of two,
-// so there is no host round trip between steps.  Sums and weights are FP64;
-// the table is stored as complex64 (S = float2) or complex128 (S = double2,
-// the audit precision).
+// launch.  One workgroup of W wavefronts owns one proposal at a time and
+// sweeps all n steps with the environment table resident in LDS (2^rmax
+// entries); outputs of a step are gathered into registers (K per lane), the
+// waves combine their maximum modulus through LDS, and the outputs are written
+// back in place scaled by an exact power of two, so there is no host round
+// trip between steps.  Sums and weights are FP64; the table is stored as
+// complex64 (S = float2) or complex128 (S = double2, the audit precision).
 #include "rgs_common.h"
 #include "rgs_amd.h"
 
@@ -29,17 +29,25 @@ __device__ __forceinline__ double2 make_s<double2>(double re, double im) {
   return make_double2(re, im);
 }
 
-template <typename S, int K>
-__global__ void __launch_bounds__(64) k_scan_batch(const uint64_t* __restrict__ plan, const double2* __restrict__ u,
-                                                   const uint64_t* __restrict__ bits, int count,
-                                                   double2* __restrict__ out_val, int32_t* __restrict__ out_exp) {
+// Wave w of the workgroup computes the output blocks k = w, w + W, ... of a
+// step (at most K of them per lane).  Every read of the step's input table
+// ends at the barrier that also publishes the waves' maxima, so the outputs
+// overwrite the table in place.  W waves per proposal keep the LDS per
+// proposal while giving each CU W times the wavefronts to hide LDS and FP64
+// latency, and need W times fewer output registers per lane.
+template <typename S, int K, int W>
+__global__ void __launch_bounds__(64 * W) k_scan_batch(const uint64_t* __restrict__ plan,
+                                                      const double2* __restrict__ u,
+                                                      const uint64_t* __restrict__ bits, int count,
+                                                      double2* __restrict__ out_val, int32_t* __restrict__ out_exp) {
   extern __shared__ __align__(16) unsigned char smem[];
   S* E = reinterpret_cast<S*>(smem);
-  const int lane = threadIdx.x;
+  __shared__ double wmx[W];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int n = (int)plan[0];
   for (int pidx = blockIdx.x; pidx < count; pidx += gridDim.x) {
     const uint64_t x0 = bits[2 * pidx], x1 = bits[2 * pidx + 1];
-    if (lane == 0) E[0] = make_s<S>(1.0, 0.0);
+    if (threadIdx.x == 0) E[0] = make_s<S>(1.0, 0.0);
     __syncthreads();
     int esum = 0;
     for (int j = 0; j < n; ++j) {
@@ -60,7 +68,8 @@ __global__ void __launch_bounds__(64) k_scan_batch(const uint64_t* __restrict__ 
       S outv[K];
       double mx = 0.0;
 #pragma unroll
-      for (int k = 0; k < K; ++k) {
+      for (int kk = 0; kk < K; ++kk) {
+        const int k = wave + W * kk;
         if (k < nk && lane_ok) {
           const uint64_t p0 = lo ^ hi[k];
           double ar = 0.0, ai = 0.0;
@@ -82,18 +91,22 @@ __global__ void __launch_bounds__(64) k_scan_batch(const uint64_t* __restrict__ 
             }
           }
           mx = fmax(mx, ar * ar + ai * ai);
-          outv[k] = make_s<S>(ar, ai);
+          outv[kk] = make_s<S>(ar, ai);
         }
       }
 #pragma unroll
       for (int off = 32; off > 0; off >>= 1) mx = fmax(mx, __shfl_xor(mx, off));
+      if (lane == 0) wmx[wave] = mx;
+      __syncthreads();  // every read of E done; the waves' maxima visible
+#pragma unroll
+      for (int w2 = 0; w2 < W; ++w2) mx = fmax(mx, wmx[w2]);
       int e = 0;
       if (mx > 0.0) frexp(sqrt(mx), &e);
-      __syncthreads();
 #pragma unroll
-      for (int k = 0; k < K; ++k) {
+      for (int kk = 0; kk < K; ++kk) {
+        const int k = wave + W * kk;
         if (k < nk && lane_ok) {
-          S val = outv[k];
+          S val = outv[kk];
           val.x = ldexp(val.x, -e);
           val.y = ldexp(val.y, -e);
           E[lane + 64 * k] = val;
@@ -102,7 +115,7 @@ __global__ void __launch_bounds__(64) k_scan_batch(const uint64_t* __restrict__ 
       esum += e;
       __syncthreads();
     }
-    if (lane == 0) {
+    if (threadIdx.x == 0) {
       out_val[pidx] = load_c(E[0]);
       out_exp[pidx] = esum;
     }
@@ -110,118 +123,20 @@ __global__ void __launch_bounds__(64) k_scan_batch(const uint64_t* __restrict__ 
   }
 }
 
-// Ping-pong variant: input and output tables are separate LDS buffers, so no
-// outputs are held in registers; the power-of-two scale of step j is applied
-// while step j+1 reads its input.  Used where K outputs per lane would not fit
-// in registers (complex128 tables at rmax 12).
-template <typename S>
-__global__ void __launch_bounds__(64) k_scan_batch_pp(const uint64_t* __restrict__ plan, const double2* __restrict__ u,
-                                                      const uint64_t* __restrict__ bits, int count,
-                                                      double2* __restrict__ out_val, int32_t* __restrict__ out_exp,
-                                                      int rmax) {
-  extern __shared__ __align__(16) unsigned char smem[];
-  S* buf0 = reinterpret_cast<S*>(smem);
-  S* buf1 = buf0 + ((size_t)1 << rmax);
-  const int lane = threadIdx.x;
-  const int n = (int)plan[0];
-  for (int pidx = blockIdx.x; pidx < count; pidx += gridDim.x) {
-    const uint64_t x0 = bits[2 * pidx], x1 = bits[2 * pidx + 1];
-    S* Ein = buf0;
-    S* Eout = buf1;
-    if (lane == 0) Ein[0] = make_s<S>(1.0, 0.0);
-    __syncthreads();
-    int esum = 0;
-    double inscale = 1.0;  // 2^-e of the previous step, applied on read
-    for (int j = 0; j < n; ++j) {
-      const uint64_t* st = plan + plan[2 + j];
-      const uint64_t hdr = st[0];
-      const int v = (int)(hdr & 255u), r_in = (int)((hdr >> 8) & 255u), r_out = (int)((hdr >> 16) & 255u);
-      const int nker = (int)((hdr >> 24) & 255u);
-      const uint64_t beta = st[1];
-      const int xv = (int)((v < 64 ? (x0 >> v) : (x1 >> (v - 64))) & 1u);
-      const double2 f0 = u[(v * 2 + xv) * 2 + 0], f1 = u[(v * 2 + xv) * 2 + 1];
-      const uint64_t lo = st[2 + lane];
-      const uint64_t* hi = st + 66;
-      const uint64_t* combos = st + 66 + (1ull << (r_out > 6 ? r_out - 6 : 0));
-      const int ncomb = 1 << nker;
-      const uint64_t mask = (1ull << r_in) - 1ull;
-      const int nk = r_out > 6 ? 1 << (r_out - 6) : 1;
-      const bool lane_ok = r_out >= 6 || lane < (1 << r_out);
-      double mx = 0.0;
-      for (int k = 0; k < nk; ++k) {
-        if (!lane_ok) break;
-        const uint64_t p0 = lo ^ hi[k];
-        double ar = 0.0, ai = 0.0;
-        for (int t = 0; t < ncomb; ++t) {
-          const uint64_t p = p0 ^ combos[t];
-          const uint64_t c = p & mask;
-          const uint32_t y = (uint32_t)(p >> r_in) & 1u;
-          const uint32_t neg = y & ((uint32_t)__popcll(c & beta) & 1u);
-          const double2 w = y ? f1 : f0;
-          const double2 e = load_c(Ein[c]);
-          const double pr = e.x * w.x - e.y * w.y;
-          const double pi = e.x * w.y + e.y * w.x;
-          if (neg) {
-            ar -= pr;
-            ai -= pi;
-          } else {
-            ar += pr;
-            ai += pi;
-          }
-        }
-        ar *= inscale;
-        ai *= inscale;
-        mx = fmax(mx, ar * ar + ai * ai);
-        Eout[lane + 64 * k] = make_s<S>(ar, ai);
-      }
-#pragma unroll
-      for (int off = 32; off > 0; off >>= 1) mx = fmax(mx, __shfl_xor(mx, off));
-      int e = 0;
-      if (mx > 0.0) frexp(sqrt(mx), &e);
-      inscale = ldexp(1.0, -e);
-      esum += e;
-      S* tmp = Ein;
-      Ein = Eout;
-      Eout = tmp;
-      __syncthreads();
-    }
-    // Ein holds the last table unscaled by its own exponent, which esum counts.
-    if (lane == 0) {
-      double2 r = load_c(Ein[0]);
-      out_val[pidx] = make_double2(r.x * inscale, r.y * inscale);
-      out_exp[pidx] = esum;
-    }
-    __syncthreads();
-  }
-}
-
-template <typename S>
-static hipError_t launch_scan_pp(const uint64_t* plan, const double2* u, const uint64_t* bits, int count,
-                                 double2* val, int32_t* ex, int rmax, hipStream_t st) {
-  size_t shmem = ((size_t)2 << rmax) * sizeof(S);
-  int grid = std::min(count, 256 * 8);
-  static bool attr = false;
-  if (!attr) {
-    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_scan_batch_pp<S>),
-                              hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
-    attr = true;
-  }
-  hipLaunchKernelGGL((k_scan_batch_pp<S>), dim3(grid), dim3(64), shmem, st, plan, u, bits, count, val, ex, rmax);
-  return hipGetLastError();
-}
-
-template <typename S, int K>
+template <typename S, int K, int W>
 static hipError_t launch_scan(const uint64_t* plan, const double2* u, const uint64_t* bits, int count,
-                              double2* val, int32_t* ex, int rmax, hipStream_t st) {
+                                double2* val, int32_t* ex, int rmax, hipStream_t st) {
   size_t shmem = ((size_t)1 << rmax) * sizeof(S);
   int grid = std::min(count, 256 * 16);
   static bool attr = false;
   if (!attr) {
-    (void)hipFuncSetAttribute(reinterpret_cast<const void*>(&k_scan_batch<S, K>),
-                              hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
+    // dynamic LDS up to the CU's 160 KiB less the kernel's own static array
+    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_scan_batch<S, K, W>),
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024 - 256);
+    if (e != hipSuccess) return e;
     attr = true;
   }
-  hipLaunchKernelGGL((k_scan_batch<S, K>), dim3(grid), dim3(64), shmem, st, plan, u, bits, count, val, ex);
+  hipLaunchKernelGGL((k_scan_batch<S, K, W>), dim3(grid), dim3(64 * W), shmem, st, plan, u, bits, count, val, ex);
   return hipGetLastError();
 }
 
@@ -354,19 +269,22 @@ extern "C" int rgs_scan_batch(const uint64_t* plan_dev, int rmax, const double* 
   const double2* u = reinterpret_cast<const double2*>(u_dev);
   double2* val = reinterpret_cast<double2*>(out_val);
   hipError_t e;
+  // <K, W>: W waves per proposal, K = 2^(rmax-6) / W outputs per lane.  Four
+  // waves for complex64 tables and eight for complex128 were the fastest
+  // measured at rank 12 (5.5 and 3.9 M proposals/s).
   if (precision == 64) {
-    if (rmax <= 6) e = launch_scan<float2, 1>(plan_dev, u, bits_dev, count, val, out_exp, 6, st);
-    else if (rmax <= 8) e = launch_scan<float2, 4>(plan_dev, u, bits_dev, count, val, out_exp, 8, st);
-    else if (rmax <= 10) e = launch_scan<float2, 16>(plan_dev, u, bits_dev, count, val, out_exp, 10, st);
-    else if (rmax <= 12) e = launch_scan<float2, 64>(plan_dev, u, bits_dev, count, val, out_exp, 12, st);
-    else if (rmax <= 13) e = launch_scan_pp<float2>(plan_dev, u, bits_dev, count, val, out_exp, 13, st);
+    if (rmax <= 6) e = launch_scan<float2, 1, 1>(plan_dev, u, bits_dev, count, val, out_exp, 6, st);
+    else if (rmax <= 8) e = launch_scan<float2, 1, 4>(plan_dev, u, bits_dev, count, val, out_exp, 8, st);
+    else if (rmax <= 10) e = launch_scan<float2, 4, 4>(plan_dev, u, bits_dev, count, val, out_exp, 10, st);
+    else if (rmax <= 12) e = launch_scan<float2, 16, 4>(plan_dev, u, bits_dev, count, val, out_exp, 12, st);
+    else if (rmax <= 13) e = launch_scan<float2, 32, 4>(plan_dev, u, bits_dev, count, val, out_exp, 13, st);
     else return fail(kInvalid, "scan_batch: complex64 batch limited to rmax <= 13");
   } else if (precision == 128) {
-    if (rmax <= 6) e = launch_scan<double2, 1>(plan_dev, u, bits_dev, count, val, out_exp, 6, st);
-    else if (rmax <= 8) e = launch_scan<double2, 4>(plan_dev, u, bits_dev, count, val, out_exp, 8, st);
-    else if (rmax <= 10) e = launch_scan<double2, 16>(plan_dev, u, bits_dev, count, val, out_exp, 10, st);
-    else if (rmax <= 11) e = launch_scan<double2, 32>(plan_dev, u, bits_dev, count, val, out_exp, 11, st);
-    else if (rmax <= 12) e = launch_scan_pp<double2>(plan_dev, u, bits_dev, count, val, out_exp, 12, st);
+    if (rmax <= 6) e = launch_scan<double2, 1, 1>(plan_dev, u, bits_dev, count, val, out_exp, 6, st);
+    else if (rmax <= 8) e = launch_scan<double2, 1, 4>(plan_dev, u, bits_dev, count, val, out_exp, 8, st);
+    else if (rmax <= 10) e = launch_scan<double2, 2, 8>(plan_dev, u, bits_dev, count, val, out_exp, 10, st);
+    else if (rmax <= 11) e = launch_scan<double2, 4, 8>(plan_dev, u, bits_dev, count, val, out_exp, 11, st);
+    else if (rmax <= 12) e = launch_scan<double2, 8, 8>(plan_dev, u, bits_dev, count, val, out_exp, 12, st);
     else return fail(kInvalid, "scan_batch: complex128 batch limited to rmax <= 12");
   } else {
     return fail(kInvalid, "precision must be 64 (complex64 tables) or 128 (complex128)");
