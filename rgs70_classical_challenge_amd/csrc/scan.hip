@@ -18,6 +18,17 @@ __device__ __forceinline__ double2 load_c(const S& s) {
   return make_double2((double)s.x, (double)s.y);
 }
 
+// An entry widened to FP64 with its sign flipped when sgn = 0x80000000.
+__device__ __forceinline__ double2 load_signed(const float2& s, uint32_t sgn) {
+  return make_double2((double)__uint_as_float(__float_as_uint(s.x) ^ sgn),
+                      (double)__uint_as_float(__float_as_uint(s.y) ^ sgn));
+}
+__device__ __forceinline__ double2 load_signed(const double2& s, uint32_t sgn) {
+  const uint64_t m = (uint64_t)sgn << 32;
+  return make_double2(__longlong_as_double(__double_as_longlong(s.x) ^ m),
+                      __longlong_as_double(__double_as_longlong(s.y) ^ m));
+}
+
 template <typename S>
 __device__ __forceinline__ S make_s(double re, double im);
 template <>
@@ -29,21 +40,165 @@ __device__ __forceinline__ double2 make_s<double2>(double re, double im) {
   return make_double2(re, im);
 }
 
-// Wave w of the workgroup computes the output blocks k = w, w + W, ... of a
-// step (at most K of them per lane).  Every read of the step's input table
-// ends at the barrier that also publishes the waves' maxima, so the outputs
-// overwrite the table in place.  W waves per proposal keep the LDS per
-// proposal while giving each CU W times the wavefronts to hide LDS and FP64
-// latency, and need W times fewer output registers per lane.
+// Maximum of a non-negative 32-bit value over the wavefront: DPP within each
+// row of 16 lanes (quad swaps, then rotations by 4 and 8), then the four row
+// results read into scalars.  All 64 lanes must be active.
+__device__ __forceinline__ uint32_t wave_max_u32(uint32_t v) {
+  v = max(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0xB1, 0xF, 0xF, false));    // quad_perm 1,0,3,2
+  v = max(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x4E, 0xF, 0xF, false));    // quad_perm 2,3,0,1
+  v = max(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x124, 0xF, 0xF, false));   // row_ror 4
+  v = max(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x128, 0xF, 0xF, false));   // row_ror 8
+  const uint32_t r0 = __builtin_amdgcn_readlane(v, 0), r1 = __builtin_amdgcn_readlane(v, 16);
+  const uint32_t r2 = __builtin_amdgcn_readlane(v, 32), r3 = __builtin_amdgcn_readlane(v, 48);
+  return max(max(r0, r1), max(r2, r3));
+}
+
+// The high word of a non-negative double orders like the double itself, so
+// the maximum of |out|^2 is tracked as the maximum high word; only its
+// exponent E is used.  The renormalisation exponent is e = floor(E / 2) + 1
+// (the frexp exponent of sqrt(max |out|^2)), so max |out| 2^-e lies in
+// [1/2, 1); a zero or subnormal maximum leaves the table unscaled.
+__device__ __forceinline__ uint32_t sq_hi(double re, double im) {
+  return (uint32_t)(__double_as_longlong(re * re + im * im) >> 32);
+}
+__device__ __forceinline__ int scale_exponent(uint32_t mxh) {
+  const int biased = (int)(mxh >> 20);
+  return biased ? ((biased - 1023) >> 1) + 1 : 0;
+}
+
+// One step's uniform constants.
+struct ScanStepArgs {
+  const uint64_t* combos;  // 2^nker kernel combinations, y = 1 in the upper half when split
+  uint32_t beta, mask;
+  int r_in, nker;
+  double2 f0, f1;
+};
+
+// Outputs of one step for this lane.  The wave's blocks are visited in Gray
+// order g(kk) = kk ^ (kk >> 1), block k = wave + W g(kk), so that each block's
+// p0 = lo ^ hi[k] differs from the previous one by a single uniform word
+// hstep[ctz(kk)] = hi[W 2^ctz(kk)]; lo_w = lo ^ hi[wave] is per lane.  Blocks
+// kk < nkw are live (nkw uniform).  Term counts 2^NKER are compile-time
+// constants, so the table reads of all outputs are straight-line loads.
+// Split steps: every p0 has y = 0; the first half of the combinations keeps
+// y = 0 (weight f0, no sign), the second has y = 1 (weight f1, sign
+// (-1)^(c . beta)).  Plain steps: every term of an output has y = y(p0),
+// weight f_y and sign (-1)^(y (c . beta)); the lane's part of y is folded
+// into the per-lane weights fa / fb and sign masks ba / bb once per step.
+// Returns the lane's maximum high word of |out|^2.
+template <typename S, int K, int NKER, bool SPLIT>
+__device__ __forceinline__ uint32_t scan_outputs(const S* E, const ScanStepArgs& a, uint32_t lo_w,
+                                                 const uint32_t* hstep, int nkw, S (&outv)[K]) {
+  constexpr int NC = 1 << NKER, HALF = SPLIT ? NC / 2 : NC;
+  uint32_t cmb[NC];
+#pragma unroll
+  for (int t = 0; t < NC; ++t) cmb[t] = (uint32_t)a.combos[t];
+  const uint32_t ylo = SPLIT ? 0u : (lo_w >> a.r_in) & 1u;
+  const double2 fa = ylo ? a.f1 : a.f0, fb = ylo ? a.f0 : a.f1;   // weights for y(hi part) = 0 / 1
+  const uint32_t ba = ylo ? a.beta : 0u, bb = a.beta ^ ba;
+  uint32_t mxh = 0, hs = 0;
+#pragma unroll
+  for (int kk = 0; kk < K; ++kk) {
+    if (kk) hs ^= hstep[__builtin_ctz(kk)];
+    if (kk < nkw) {
+      const uint32_t p0 = lo_w ^ hs;
+      const bool yh = !SPLIT && ((hs >> a.r_in) & 1u);
+      const uint32_t bm = yh ? bb : ba;
+      double s0r = 0.0, s0i = 0.0, s1r = 0.0, s1i = 0.0;
+#pragma unroll
+      for (int t = 0; t < NC; ++t) {
+        const uint32_t c = (p0 ^ cmb[t]) & a.mask;
+        const uint32_t sg = t < HALF ? (SPLIT ? 0u : (uint32_t)__popc(c & bm) << 31)
+                                     : (uint32_t)__popc(c & a.beta) << 31;
+        const double2 e = load_signed(E[c], sg);
+        if (t == 0) {
+          s0r = e.x;
+          s0i = e.y;
+        } else if (t < HALF) {
+          s0r += e.x;
+          s0i += e.y;
+        } else if (t == HALF) {
+          s1r = e.x;
+          s1i = e.y;
+        } else {
+          s1r += e.x;
+          s1i += e.y;
+        }
+      }
+      double orr, oi;
+      if (SPLIT) {
+        orr = (s0r * a.f0.x - s0i * a.f0.y) + (s1r * a.f1.x - s1i * a.f1.y);
+        oi = (s0r * a.f0.y + s0i * a.f0.x) + (s1r * a.f1.y + s1i * a.f1.x);
+      } else {
+        const double2 f = yh ? fb : fa;
+        orr = s0r * f.x - s0i * f.y;
+        oi = s0r * f.y + s0i * f.x;
+      }
+      mxh = max(mxh, sq_hi(orr, oi));
+      outv[kk] = make_s<S>(orr, oi);
+    }
+  }
+  return mxh;
+}
+
+// Any kernel size, with run-time loops over the terms (steps with nker > 2).
+template <typename S, int K>
+__device__ __forceinline__ uint32_t scan_outputs_any(const S* E, const ScanStepArgs& a, bool split, uint32_t lo_w,
+                                                     const uint32_t* hstep, int nkw, S (&outv)[K]) {
+  const int ncomb = 1 << a.nker, half = split ? ncomb >> 1 : ncomb;
+  uint32_t mxh = 0, hs = 0;
+#pragma unroll
+  for (int kk = 0; kk < K; ++kk) {
+    if (kk) hs ^= hstep[__builtin_ctz(kk)];
+    if (kk < nkw) {
+      const uint32_t p0 = lo_w ^ hs;
+      const uint32_t y = (p0 >> a.r_in) & 1u;   // 0 on split steps
+      const uint32_t bm = y ? a.beta : 0u;
+      double s0r = 0.0, s0i = 0.0, s1r = 0.0, s1i = 0.0;
+      for (int t = 0; t < half; ++t) {
+        const uint32_t c = (p0 ^ (uint32_t)a.combos[t]) & a.mask;
+        const double2 e = load_signed(E[c], (uint32_t)__popc(c & bm) << 31);
+        s0r += e.x;
+        s0i += e.y;
+      }
+      for (int t = half; t < ncomb; ++t) {
+        const uint32_t c = (p0 ^ (uint32_t)a.combos[t]) & a.mask;
+        const double2 e = load_signed(E[c], (uint32_t)__popc(c & a.beta) << 31);
+        s1r += e.x;
+        s1i += e.y;
+      }
+      const double2 f = y ? a.f1 : a.f0;
+      const double orr = (s0r * f.x - s0i * f.y) + (s1r * a.f1.x - s1i * a.f1.y);
+      const double oi = (s0r * f.y + s0i * f.x) + (s1r * a.f1.y + s1i * a.f1.x);
+      mxh = max(mxh, sq_hi(orr, oi));
+      outv[kk] = make_s<S>(orr, oi);
+    }
+  }
+  return mxh;
+}
+
+// Wave w of the workgroup computes the output blocks k = w + W g of a step
+// (at most K of them per lane).  Output c' sums E[c] over the kernel
+// combinations t of p = p0(c') ^ combos[t] = (c, y), each term weighted by
+// f_y and negated when y = 1 and c . beta is odd; the packed plan orders the
+// combinations so that each output takes its weights once (split and plain
+// steps above).  Indices fit 32 bits (rank <= 13).  Every read of the step's
+// input table ends at the barrier that also publishes the waves' maxima, so
+// the outputs overwrite the table in place.  W waves per proposal keep the
+// LDS per proposal while giving each CU W times the wavefronts to hide LDS
+// and FP64 latency, and need W times fewer output registers per lane.
 template <typename S, int K, int W>
 __global__ void __launch_bounds__(64 * W) k_scan_batch(const uint64_t* __restrict__ plan,
                                                       const double2* __restrict__ u,
                                                       const uint64_t* __restrict__ bits, int count,
                                                       double2* __restrict__ out_val, int32_t* __restrict__ out_exp) {
+  static_assert((K & (K - 1)) == 0 && (W & (W - 1)) == 0, "K and W are powers of two");
+  constexpr int KB = K > 1 ? __builtin_ctz(K) : 0;   // Gray steps within the wave's blocks
   extern __shared__ __align__(16) unsigned char smem[];
   S* E = reinterpret_cast<S*>(smem);
-  __shared__ double wmx[W];
-  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  __shared__ uint32_t wmx[W];
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);   // uniform: plan reads stay scalar
   const int n = (int)plan[0];
   for (int pidx = blockIdx.x; pidx < count; pidx += gridDim.x) {
     const uint64_t x0 = bits[2 * pidx], x1 = bits[2 * pidx + 1];
@@ -53,63 +208,57 @@ __global__ void __launch_bounds__(64 * W) k_scan_batch(const uint64_t* __restric
     for (int j = 0; j < n; ++j) {
       const uint64_t* st = plan + plan[2 + j];
       const uint64_t hdr = st[0];
-      const int v = (int)(hdr & 255u), r_in = (int)((hdr >> 8) & 255u), r_out = (int)((hdr >> 16) & 255u);
-      const int nker = (int)((hdr >> 24) & 255u);
-      const uint64_t beta = st[1];
+      const int v = (int)(hdr & 255u), r_out = (int)((hdr >> 16) & 255u);
+      const bool split = (hdr >> 32) & 1u;
+      ScanStepArgs a;
+      a.r_in = (int)((hdr >> 8) & 255u);
+      a.nker = (int)((hdr >> 24) & 255u);
+      a.beta = (uint32_t)st[1];
       const int xv = (int)((v < 64 ? (x0 >> v) : (x1 >> (v - 64))) & 1u);
-      const double2 f0 = u[(v * 2 + xv) * 2 + 0], f1 = u[(v * 2 + xv) * 2 + 1];
-      const uint64_t lo = st[2 + lane];
-      const uint64_t* hi = st + 66;
-      const uint64_t* combos = st + 66 + (1ull << (r_out > 6 ? r_out - 6 : 0));
-      const int ncomb = 1 << nker;
-      const uint64_t mask = (1ull << r_in) - 1ull;
+      a.f0 = u[(v * 2 + xv) * 2 + 0];
+      a.f1 = u[(v * 2 + xv) * 2 + 1];
+      a.combos = st + 66 + (1ull << (r_out > 6 ? r_out - 6 : 0));
+      a.mask = (1u << a.r_in) - 1u;
       const int nk = r_out > 6 ? 1 << (r_out - 6) : 1;
+      const int nkw = nk >= W ? nk / W : (wave < nk ? 1 : 0);
       const bool lane_ok = r_out >= 6 || lane < (1 << r_out);
-      S outv[K];
-      double mx = 0.0;
+      const uint32_t lo_w = (uint32_t)st[2 + lane] ^ (wave < nk ? (uint32_t)st[66 + wave] : 0u);
+      uint32_t hstep[KB > 0 ? KB : 1];
 #pragma unroll
-      for (int kk = 0; kk < K; ++kk) {
-        const int k = wave + W * kk;
-        if (k < nk && lane_ok) {
-          const uint64_t p0 = lo ^ hi[k];
-          double ar = 0.0, ai = 0.0;
-          for (int t = 0; t < ncomb; ++t) {
-            const uint64_t p = p0 ^ combos[t];
-            const uint64_t c = p & mask;
-            const uint32_t y = (uint32_t)(p >> r_in) & 1u;
-            const uint32_t neg = y & ((uint32_t)__popcll(c & beta) & 1u);
-            const double2 w = y ? f1 : f0;
-            const double2 e = load_c(E[c]);
-            const double pr = e.x * w.x - e.y * w.y;
-            const double pi = e.x * w.y + e.y * w.x;
-            if (neg) {
-              ar -= pr;
-              ai -= pi;
-            } else {
-              ar += pr;
-              ai += pi;
-            }
-          }
-          mx = fmax(mx, ar * ar + ai * ai);
-          outv[kk] = make_s<S>(ar, ai);
+      for (int b = 0; b < KB; ++b) hstep[b] = (W << b) < nk ? (uint32_t)st[66 + (W << b)] : 0u;
+      S outv[K];
+      uint32_t mxh;
+      if (split) {
+        switch (a.nker) {
+          case 1: mxh = scan_outputs<S, K, 1, true>(E, a, lo_w, hstep, nkw, outv); break;
+          case 2: mxh = scan_outputs<S, K, 2, true>(E, a, lo_w, hstep, nkw, outv); break;
+          default: mxh = scan_outputs_any<S, K>(E, a, true, lo_w, hstep, nkw, outv); break;
+        }
+      } else {
+        switch (a.nker) {
+          case 0: mxh = scan_outputs<S, K, 0, false>(E, a, lo_w, hstep, nkw, outv); break;
+          case 1: mxh = scan_outputs<S, K, 1, false>(E, a, lo_w, hstep, nkw, outv); break;
+          case 2: mxh = scan_outputs<S, K, 2, false>(E, a, lo_w, hstep, nkw, outv); break;
+          default: mxh = scan_outputs_any<S, K>(E, a, false, lo_w, hstep, nkw, outv); break;
         }
       }
-#pragma unroll
-      for (int off = 32; off > 0; off >>= 1) mx = fmax(mx, __shfl_xor(mx, off));
-      if (lane == 0) wmx[wave] = mx;
+      // lanes past 2^r_out (steps of rank below six) computed in-range values
+      // that are neither counted nor stored
+      mxh = wave_max_u32(lane_ok ? mxh : 0u);
+      if (lane == 0) wmx[wave] = mxh;
       __syncthreads();  // every read of E done; the waves' maxima visible
 #pragma unroll
-      for (int w2 = 0; w2 < W; ++w2) mx = fmax(mx, wmx[w2]);
-      int e = 0;
-      if (mx > 0.0) frexp(sqrt(mx), &e);
+      for (int w2 = 0; w2 < W; ++w2) mxh = max(mxh, wmx[w2]);
+      const int e = scale_exponent(mxh);
+      if (lane_ok) {
 #pragma unroll
-      for (int kk = 0; kk < K; ++kk) {
-        const int k = wave + W * kk;
-        if (k < nk && lane_ok) {
-          S val = outv[kk];
-          val.x = ldexp(val.x, -e);
-          val.y = ldexp(val.y, -e);
-          E[lane + 64 * k] = val;
+        for (int kk = 0; kk < K; ++kk) {
+          if (kk < nkw) {
+            S val = outv[kk];
+            val.x = ldexp(val.x, -e);
+            val.y = ldexp(val.y, -e);
+            E[lane + 64 * (wave + W * (kk ^ (kk >> 1)))] = val;
+          }
         }
       }
       esum += e;
@@ -138,6 +287,27 @@ static hipError_t launch_scan(const uint64_t* plan, const double2* u, const uint
   }
   hipLaunchKernelGGL((k_scan_batch<S, K, W>), dim3(grid), dim3(64 * W), shmem, st, plan, u, bits, count, val, ex);
   return hipGetLastError();
+}
+
+// Wavefronts per proposal for each table precision (the fastest measured at
+// rank 12); ranks with fewer 64-entry output blocks than that use one wave
+// per block.
+#ifndef RGS_SCAN_WAVES_C64
+#define RGS_SCAN_WAVES_C64 4
+#endif
+#ifndef RGS_SCAN_WAVES_C128
+#define RGS_SCAN_WAVES_C128 8
+#endif
+constexpr int kWavesC64 = RGS_SCAN_WAVES_C64, kWavesC128 = RGS_SCAN_WAVES_C128;
+
+// Tables of rank up to R: 2^(R-6) output blocks of 64 entries, shared by
+// min(W, blocks) waves with K blocks each.
+template <typename S, int R, int W>
+static hipError_t launch_rank(const uint64_t* plan, const double2* u, const uint64_t* bits, int count, double2* val,
+                              int32_t* ex, hipStream_t st) {
+  constexpr int NB = R > 6 ? 1 << (R - 6) : 1;
+  constexpr int WE = W < NB ? W : NB;
+  return launch_scan<S, NB / WE, WE>(plan, u, bits, count, val, ex, R, st);
 }
 
 // ---------------------------------------------------------------- large rank
@@ -269,22 +439,19 @@ extern "C" int rgs_scan_batch(const uint64_t* plan_dev, int rmax, const double* 
   const double2* u = reinterpret_cast<const double2*>(u_dev);
   double2* val = reinterpret_cast<double2*>(out_val);
   hipError_t e;
-  // <K, W>: W waves per proposal, K = 2^(rmax-6) / W outputs per lane.  Four
-  // waves for complex64 tables and eight for complex128 were the fastest
-  // measured at rank 12 (5.5 and 3.9 M proposals/s).
   if (precision == 64) {
-    if (rmax <= 6) e = launch_scan<float2, 1, 1>(plan_dev, u, bits_dev, count, val, out_exp, 6, st);
-    else if (rmax <= 8) e = launch_scan<float2, 1, 4>(plan_dev, u, bits_dev, count, val, out_exp, 8, st);
-    else if (rmax <= 10) e = launch_scan<float2, 4, 4>(plan_dev, u, bits_dev, count, val, out_exp, 10, st);
-    else if (rmax <= 12) e = launch_scan<float2, 16, 4>(plan_dev, u, bits_dev, count, val, out_exp, 12, st);
-    else if (rmax <= 13) e = launch_scan<float2, 32, 4>(plan_dev, u, bits_dev, count, val, out_exp, 13, st);
+    if (rmax <= 6) e = launch_rank<float2, 6, kWavesC64>(plan_dev, u, bits_dev, count, val, out_exp, st);
+    else if (rmax <= 8) e = launch_rank<float2, 8, kWavesC64>(plan_dev, u, bits_dev, count, val, out_exp, st);
+    else if (rmax <= 10) e = launch_rank<float2, 10, kWavesC64>(plan_dev, u, bits_dev, count, val, out_exp, st);
+    else if (rmax <= 12) e = launch_rank<float2, 12, kWavesC64>(plan_dev, u, bits_dev, count, val, out_exp, st);
+    else if (rmax <= 13) e = launch_rank<float2, 13, kWavesC64>(plan_dev, u, bits_dev, count, val, out_exp, st);
     else return fail(kInvalid, "scan_batch: complex64 batch limited to rmax <= 13");
   } else if (precision == 128) {
-    if (rmax <= 6) e = launch_scan<double2, 1, 1>(plan_dev, u, bits_dev, count, val, out_exp, 6, st);
-    else if (rmax <= 8) e = launch_scan<double2, 1, 4>(plan_dev, u, bits_dev, count, val, out_exp, 8, st);
-    else if (rmax <= 10) e = launch_scan<double2, 2, 8>(plan_dev, u, bits_dev, count, val, out_exp, 10, st);
-    else if (rmax <= 11) e = launch_scan<double2, 4, 8>(plan_dev, u, bits_dev, count, val, out_exp, 11, st);
-    else if (rmax <= 12) e = launch_scan<double2, 8, 8>(plan_dev, u, bits_dev, count, val, out_exp, 12, st);
+    if (rmax <= 6) e = launch_rank<double2, 6, kWavesC128>(plan_dev, u, bits_dev, count, val, out_exp, st);
+    else if (rmax <= 8) e = launch_rank<double2, 8, kWavesC128>(plan_dev, u, bits_dev, count, val, out_exp, st);
+    else if (rmax <= 10) e = launch_rank<double2, 10, kWavesC128>(plan_dev, u, bits_dev, count, val, out_exp, st);
+    else if (rmax <= 11) e = launch_rank<double2, 11, kWavesC128>(plan_dev, u, bits_dev, count, val, out_exp, st);
+    else if (rmax <= 12) e = launch_rank<double2, 12, kWavesC128>(plan_dev, u, bits_dev, count, val, out_exp, st);
     else return fail(kInvalid, "scan_batch: complex128 batch limited to rmax <= 12");
   } else {
     return fail(kInvalid, "precision must be 64 (complex64 tables) or 128 (complex128)");

@@ -261,10 +261,19 @@ extern "C" int rgs_scan_order_search(const uint8_t* g, int n, const int32_t* sta
 
 // Packed device plan (uint64 words):
 //   [0] n  [1] rmax  [2 .. 2+n) offset of each step
-//   step: hdr = v | r_in << 8 | r_out << 16 | nker << 24, beta,
+//   step: hdr = v | r_in << 8 | r_out << 16 | nker << 24 | split << 32, beta,
 //         lo[64]  (p0 of the low six bits of c'),
 //         hi[2^max(0, r_out-6)] (p0 of the remaining bits),
 //         combos[2^nker]
+// The kernel basis is first reduced on the y bit (bit r_in of p): at most one
+// basis vector keeps it, and that one (kpiv) goes last.  When one does
+// (split = 1), every lo/hi entry with the y bit set is moved by kpiv, so each
+// p0 = lo ^ hi is the member of its coset p0 + kernel with y = 0; the
+// combinations then hold y = 0 in their first half and y = 1 in their second,
+// and every output is f0 * (sum of its first-half entries) + f1 * (signed sum
+// of its second half), with no per-lane choice of weight.  When none does
+// (split = 0), all terms of an output share y(p0).  The set of terms of each
+// output, and so its sum, is unchanged; only the order of summation is.
 extern "C" int rgs_scan_pack(const rgs_scan_step* steps, int n, uint64_t* blob, uint64_t* words) {
   if (!steps || n < 1 || n > 128 || !words) return fail(kInvalid, "scan_pack: bad arguments");
   uint64_t w = 2 + (uint64_t)n;
@@ -281,22 +290,34 @@ extern "C" int rgs_scan_pack(const rgs_scan_step* steps, int n, uint64_t* blob, 
       uint64_t* p = blob + off;
       p[0] = (uint64_t)st.v | ((uint64_t)st.r_in << 8) | ((uint64_t)st.r_out << 16) | ((uint64_t)st.nker << 24);
       p[1] = st.beta;
+      uint64_t kern[64];
+      for (int k = 0; k < st.nker; ++k) kern[k] = st.kern[k];
+      const uint64_t ybit = 1ull << st.r_in;
+      int piv = -1;
+      for (int k = 0; k < st.nker; ++k)
+        if (kern[k] & ybit) {
+          if (piv < 0) piv = k;
+          else kern[k] ^= kern[piv];
+        }
+      if (piv >= 0) std::swap(kern[piv], kern[st.nker - 1]);
+      const uint64_t kpiv = piv >= 0 ? kern[st.nker - 1] : 0;
+      p[0] |= (uint64_t)(piv >= 0) << 32;
       for (int c = 0; c < 64; ++c) {
         uint64_t x = 0;
         for (int i = 0; i < 6 && i < st.r_out; ++i)
           if ((c >> i) & 1) x ^= st.lcols[i];
-        p[2 + c] = x;
+        p[2 + c] = (x & ybit) ? x ^ kpiv : x;
       }
       for (uint64_t k = 0; k < hi; ++k) {
         uint64_t x = 0;
         for (int i = 6; i < st.r_out; ++i)
           if ((k >> (i - 6)) & 1) x ^= st.lcols[i];
-        p[66 + k] = x;
+        p[66 + k] = (x & ybit) ? x ^ kpiv : x;
       }
       uint64_t* cb = p + 66 + hi;
       cb[0] = 0;
       for (int k = 0; k < st.nker; ++k)
-        for (uint64_t t = 0; t < (1ull << k); ++t) cb[(1ull << k) + t] = cb[t] ^ st.kern[k];
+        for (uint64_t t = 0; t < (1ull << k); ++t) cb[(1ull << k) + t] = cb[t] ^ kern[k];
     }
   }
   if (blob) {

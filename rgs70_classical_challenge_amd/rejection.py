@@ -9,11 +9,16 @@ q(x) in place of 2^-n: accept iff ``u < min(p(x) / (cap q(x)), 1)``, output
 proportional to min(p, cap q).  Proposals are scored in batches of thousands
 by the device scan (complex64 tables); every decision is then re-scored with
 complex128 tables and its FP64 margin |u - acceptance| recorded
-(audit_all_proposals_fp64.py:43-50).
+(audit_all_proposals_fp64.py:43-50).  A decision that differs from the
+complex128 one is a flip; flips whose margin exceeds FLIP_TOLERANCE of the
+acceptance (the 1e-5 relative p(x) tolerance) are counted apart, since only
+those are errors of the scorer rather than ties within its precision.
 """
 import time
 
 import numpy as np
+
+FLIP_TOLERANCE = 1e-5
 
 
 def uniform_proposals(rng, n, count, schedule='reference'):
@@ -55,7 +60,8 @@ class CappedRejection:
         import torch
         n = self.n
         acc_bits, acc_p, acc_p64 = [], [], []
-        flips = 0
+        flips = beyond = 0
+        flip_margin = 0.0
         margins = []
         proposals = 0
         gpu_s = 0.0
@@ -74,7 +80,12 @@ class CappedRejection:
             acc, keep = decide(p, logq, u, self.cap, n)
             p64 = self.auditor.probabilities(bits)
             acc64, keep64 = decide(p64, logq, u, self.cap, n)
-            flips += int(np.sum(keep != keep64))
+            flip = keep != keep64
+            rel = np.abs(u - acc64) / acc64
+            flips += int(np.sum(flip))
+            beyond += int(np.sum(flip & (rel > FLIP_TOLERANCE)))
+            if flip.any():
+                flip_margin = max(flip_margin, float(rel[flip].max()))
             margins.append(np.abs(u - acc64))
             proposals += batch
             acc_bits.append(bits[keep])
@@ -92,6 +103,7 @@ class CappedRejection:
                     sample_seconds=gpu_s, wall_seconds=wall, accepted_per_s=len(bits) / gpu_s,
                     xeb=float(z.mean() - 1), xeb_se=float(z.std(ddof=1) / np.sqrt(len(z))),
                     decision_flips_fp64=flips, min_margin_fp64=float(m.min()),
+                    decision_flips_beyond_tolerance=beyond, max_flip_margin_rel=flip_margin,
                     rel_err_c64_vs_c128={'median': float(np.median(np.abs(p - p64) / p64)),
                                          'max': float(np.max(np.abs(p - p64) / p64))},
                     bits=bits, p=p, p_fp64=p64)
@@ -165,7 +177,8 @@ class DeviceCappedRejection:
         torch.cuda.synchronize(dev)
         seconds = time.perf_counter() - t0
         # audit: complex128 tables for every proposal, decisions recomputed in FP64
-        flips = 0
+        flips = beyond = 0
+        flip_margin = 0.0
         margins = []
         z = []
         for words, u, logq, keep in zip(kept_words, kept_u, kept_logq, kept_keep):
@@ -175,11 +188,17 @@ class DeviceCappedRejection:
             else:
                 acc64 = torch.clamp(p64 / (cap * torch.exp(logq)), max=1.0)
             keep64 = u < acc64
-            flips += int((keep64 != keep).sum().item())
+            flip = keep64 != keep
+            rel = (u - acc64).abs() / acc64
+            flips += int(flip.sum().item())
+            beyond += int((flip & (rel > FLIP_TOLERANCE)).sum().item())
+            if bool(flip.any()):
+                flip_margin = max(flip_margin, rel[flip].max().item())
             margins.append((u - acc64).abs().min().item())
             z.append((p64[keep] * 2.0 ** n).cpu().numpy())
         z = np.concatenate(z)
         return dict(accepted=int(accepted), proposals=int(proposals), acceptance=accepted / proposals,
                     seconds=seconds, accepted_per_s=accepted / seconds, proposals_per_s=proposals / seconds,
                     xeb=float(z.mean() - 1), xeb_se=float(z.std(ddof=1) / np.sqrt(len(z))),
-                    decision_flips_fp64=flips, min_margin_fp64=float(min(margins)), z=z)
+                    decision_flips_fp64=flips, min_margin_fp64=float(min(margins)),
+                    decision_flips_beyond_tolerance=beyond, max_flip_margin_rel=flip_margin, z=z)

@@ -215,3 +215,39 @@ class ScanOracle:
 
     def amplitude(self, bits):
         return complex(self.amplitudes(np.asarray(bits, np.uint8)[None, :])[0])
+
+
+def main(argv):
+    """``python -m rgs70_classical_challenge_amd.scan [count [precision [seed]]]``:
+    p(x) of ``count`` uniform proposals of the 70-vertex 3-regular instance
+    (seeded QR unitaries, annealed order) in one launch each, best of five;
+    prints one JSON line with proposals/s."""
+    import json
+    import torch
+    from .instances import instance
+    count = int(argv[0]) if argv else 65536
+    precision = int(argv[1]) if len(argv) > 1 else 64
+    seed = int(argv[2]) if len(argv) > 2 else 70
+    g, u = instance(70, seed)
+    order, _ = order_search(g, seconds=5.0, rmax_cap=28)
+    so = ScanOracle(g, u, order)
+    bits = np.random.default_rng(seed).integers(0, 2, (count, 70), dtype=np.uint8)
+    words = torch.as_tensor(bits_to_words(bits).view(np.int64)).to(so.device)
+    so.probabilities_device(words, precision)
+    torch.cuda.synchronize()
+    best = float('inf')
+    for _ in range(5):
+        e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        e0.record(so.stream)
+        so.probabilities_device(words, precision)
+        e1.record(so.stream)
+        torch.cuda.synchronize()
+        best = min(best, e0.elapsed_time(e1))
+    print(json.dumps({'workload': f'70-vertex 3-regular, seed {seed}, {count} uniform proposals per launch',
+                      'precision': precision, 'rmax': so.rmax, 'sum_states': so.plan.sum_states,
+                      'ms_per_launch': round(best, 3), 'proposals_per_s': round(count / best * 1e3, 1)}), flush=True)
+
+
+if __name__ == '__main__':
+    import sys
+    main(sys.argv[1:])

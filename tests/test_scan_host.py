@@ -48,11 +48,17 @@ def test_pack_layout():
         off = int(blob[2 + j])
         hdr = int(blob[off])
         assert hdr & 255 == st.v and (hdr >> 8) & 255 == st.r_in and (hdr >> 16) & 255 == st.r_out
+        # split steps store the y = 0 member of p0's coset: p0 moved by the
+        # kernel vector that carries y (the combination at the middle index)
+        nhi = 1 << max(0, st.r_out - 6)
+        kpiv = int(blob[off + 66 + nhi + (1 << st.nker) // 2]) if (hdr >> 32) & 1 else 0
         for c in range(min(64, 1 << st.r_out)):
             want = 0
             for i in range(st.r_out):
                 if i < 6 and (c >> i) & 1:
                     want ^= st.lcols[i]
+            if (want >> st.r_in) & 1:
+                want ^= kpiv
             assert int(blob[off + 2 + c]) == want
 
 
@@ -118,3 +124,77 @@ def test_cost_agrees_with_reference_optimizer(golden_dir):
         prof = scan.profile(g, order)
         assert int(base[3]) == sum(1 << r for r in prof)
         assert int(base[4]) == max(prof)
+
+
+def _packed_amplitude(blob, u, bits):
+    """<x| (x)u |G> by walking the packed device plan in complex128 as the
+    batch kernel does: per output c', p0 = lo[c' & 63] ^ hi[c' >> 6]; the
+    first half of the combinations carries y = y(p0), the second (split
+    steps) 1 - y(p0); each half is a signed sum of table entries times f_y.
+    On split steps y(p0) is 0 for every output."""
+    n = int(blob[0])
+    E = np.ones(1, np.complex128)
+    for j in range(n):
+        off = int(blob[2 + j])
+        hdr = int(blob[off])
+        v, r_in, r_out, nker = hdr & 255, (hdr >> 8) & 255, (hdr >> 16) & 255, (hdr >> 24) & 255
+        split = (hdr >> 32) & 1
+        beta = int(blob[off + 1])
+        nhi = 1 << max(0, r_out - 6)
+        lo = [int(x) for x in blob[off + 2: off + 66]]
+        hi = [int(x) for x in blob[off + 66: off + 66 + nhi]]
+        combos = [int(x) for x in blob[off + 66 + nhi: off + 66 + nhi + (1 << nker)]]
+        half = len(combos) // 2 if split else len(combos)
+        f = u[v, int(bits[v])]
+        out = np.zeros(1 << r_out, np.complex128)
+        for cp in range(1 << r_out):
+            p0 = lo[cp & 63] ^ hi[cp >> 6]
+            ya = (p0 >> r_in) & 1
+            assert not (split and ya)
+            for part, ts in ((ya, combos[:half]), (1 - ya, combos[half:])):
+                acc = 0j
+                for t in ts:
+                    c = (p0 ^ t) & ((1 << r_in) - 1)
+                    acc += -E[c] if part and bin(c & beta).count('1') & 1 else E[c]
+                out[cp] += acc * f[part]
+        E = out
+    return complex(E[0]) * 2.0 ** (-n / 2)
+
+
+def test_packed_plan_halves_and_sum():
+    """The packer reduces each kernel basis on the y bit: the combinations
+    still span the kernel, their y bit is 0 in the first half and 1 in the
+    second on split steps (0 throughout otherwise), and the halved signed
+    sums over the packed plan equal the oracle sweep."""
+    rng = np.random.default_rng(11)
+    for n, p_edge in ((12, 0.5), (14, 0.35), (34, None)):
+        if p_edge is None:
+            g, u = instance(n, 5)
+        else:
+            a = np.triu(rng.random((n, n)) < p_edge, 1).astype(np.uint8)
+            g = a | a.T
+            z = rng.normal(size=(n, 2, 2)) + 1j * rng.normal(size=(n, 2, 2))
+            u = np.stack([np.linalg.qr(m)[0] for m in z])
+        order = list(range(n)) if p_edge else scan.order_search(g, seconds=0.3)[0]
+        plan = scan.ScanPlan(g, order)
+        blob = plan.packed()
+        steps, _, _ = osc.compile_plan(g, order)
+        splits = 0
+        for j, ref in enumerate(steps):
+            off = int(blob[2 + j])
+            hdr = int(blob[off])
+            nker, r_in, r_out = (hdr >> 24) & 255, (hdr >> 8) & 255, (hdr >> 16) & 255
+            nhi = 1 << max(0, r_out - 6)
+            combos = [int(x) for x in blob[off + 66 + nhi: off + 66 + nhi + (1 << nker)]]
+            assert sorted(combos) == sorted(osc.kernel_combos(ref.kernel))
+            ybits = [(t >> r_in) & 1 for t in combos]
+            split = (hdr >> 32) & 1
+            splits += split
+            want = [0] * (len(combos) // 2) + [1] * (len(combos) // 2) if split else [0] * len(combos)
+            assert ybits == want
+        if p_edge:
+            assert splits > 0
+        for _ in range(3):
+            bits = rng.integers(0, 2, n)
+            if n <= 14:
+                assert _packed_amplitude(blob, u, bits) == pytest.approx(osc.amplitude(steps, u, bits), abs=1e-13)
