@@ -346,6 +346,132 @@ __global__ void __launch_bounds__(256) k_made_sample(const uint16_t* __restrict_
   }
 }
 
+// Sampler for the degree-blocked layout: H = G (n - 1) hidden units with G a
+// multiple of 16, so the units of each degree d = 1 .. n-1 are the G/16
+// aligned 16-unit blocks [(d-1) G/16, d G/16) of each hidden layer.  One
+// wavefront draws 16 proposals with no workgroup barrier.  Products are taken
+// transposed, D[unit][row] = sum_k W[unit][k] v[row][k], so a lane (row
+// r = lane % 16, quarter q = lane / 16) receives units 4q .. 4q+3 of its row:
+// exactly the B-operand fragment of v_mfma_f32_16x16x16_bf16 for that unit
+// block.  First-layer blocks stay in the wave's LDS slice in fragment order
+// (one 8-byte entry per lane and block); a second-layer block is used at once
+// to add its contribution to the logit blocks L[ob] (outputs 16 ob .. 16 ob
+// + 15) and is not kept.  At step o the degree-o blocks are formed (first
+// layer over inputs < o, second layer over the blocks of degree <= o), then
+// logit o = L[o / 16] + b3[o] decides bit o as k_sample_step does.  Every
+// sum runs over ascending 16-wide K blocks into one accumulator with the
+// epilogue of k_gemm, as the launch-per-step GEMM sequence does (weights
+// outside a unit's mask are exact zeros there); log q accumulates in step
+// order.  Weights are read as 8-byte fragments straight from L2.
+__device__ __forceinline__ f32x4 mfma16(bf16x4 a, bf16x4 b, f32x4 c) {
+  return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
+}
+
+__global__ void __launch_bounds__(64) k_made_sample16(const uint16_t* __restrict__ W1, const float* __restrict__ b1,
+                                                      const uint16_t* __restrict__ W2, const float* __restrict__ b2,
+                                                      const uint16_t* __restrict__ W3, const float* __restrict__ b3,
+                                                      const double* __restrict__ u, int B, int n, int H,
+                                                      uint64_t* __restrict__ words, uint16_t* __restrict__ xout,
+                                                      double* __restrict__ logq) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  bf16x4* h1 = reinterpret_cast<bf16x4*>(smem);   // [H / 16 blocks][64 lanes]
+  const int lane = threadIdx.x;
+  const int r = lane & 15, q = lane >> 4;
+  const int nb = H >> 4, g16 = nb / (n - 1), nob = (n + 15) >> 4;
+  const bf16x4 zero = {0, 0, 0, 0};
+  for (int base = blockIdx.x * 16; base < B; base += gridDim.x * 16) {
+    const int row = base + r;
+    const bool live = row < B;
+    uint64_t xb = 0;   // bits drawn so far for this lane's row
+    double lq = 0.0;
+    f32x4 L[4] = {{0.f, 0.f, 0.f, 0.f}, {0.f, 0.f, 0.f, 0.f}, {0.f, 0.f, 0.f, 0.f}, {0.f, 0.f, 0.f, 0.f}};
+    double un = live ? u[(size_t)row * n] : 0.0;   // uniform of the next draw, loaded a step ahead
+    for (int o = 0; o < n; ++o) {
+      const double uo = un;
+      if (o + 1 < n && live) un = u[(size_t)row * n + o + 1];
+      if (o > 0) {
+        const int g0 = (o - 1) * g16;
+        for (int j = 0; j < g16; ++j) {   // first layer: inputs 0 .. o-1
+          const int blk = g0 + j;
+          const uint16_t* w1r = W1 + (size_t)(blk * 16 + r) * n;
+          f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+          for (int kb = 0; kb * 16 < o; ++kb) {
+            bf16x4 a, b;
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+              const int k = kb * 16 + 4 * q + i;
+              a[i] = (short)(k < o ? w1r[k] : 0);
+              b[i] = (short)(((xb >> k) & 1u) ? 0x3f80 : 0);
+            }
+            acc = mfma16(a, b, acc);
+          }
+          bf16x4 h;
+#pragma unroll
+          for (int t = 0; t < 4; ++t) h[t] = (short)f32_to_bf16(fmaxf(1.0f * acc[t] + b1[blk * 16 + 4 * q + t], 0.f));
+          h1[blk * 64 + lane] = h;
+        }
+        const int nkb = g0 + g16;   // first-layer blocks of degree <= o
+        for (int j = 0; j < g16; ++j) {   // second layer, then its logit contributions
+          const int blk = g0 + j;
+          const uint16_t* w2r = W2 + (size_t)(blk * 16 + r) * H + 4 * q;
+          f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+          int kb = 0;
+          for (; kb + 8 <= nkb; kb += 8) {   // eight fragment loads in flight, then the MFMAs in K order
+            bf16x4 wa[8], hb[8];
+#pragma unroll
+            for (int i = 0; i < 8; ++i) {
+              wa[i] = load4(w2r + (kb + i) * 16);
+              hb[i] = h1[(kb + i) * 64 + lane];
+            }
+#pragma unroll
+            for (int i = 0; i < 8; ++i) acc = mfma16(wa[i], hb[i], acc);
+          }
+          for (; kb < nkb; ++kb) acc = mfma16(load4(w2r + kb * 16), h1[kb * 64 + lane], acc);
+          bf16x4 h;
+#pragma unroll
+          for (int t = 0; t < 4; ++t) h[t] = (short)f32_to_bf16(fmaxf(1.0f * acc[t] + b2[blk * 16 + 4 * q + t], 0.f));
+          const uint16_t* w3r = W3 + (size_t)r * H + blk * 16 + 4 * q;
+#pragma unroll
+          for (int ob = 0; ob < 4; ++ob)
+            if (ob < nob) L[ob] = mfma16(16 * ob + r < n ? load4(w3r + (size_t)16 * ob * H) : zero, h, L[ob]);
+        }
+      }
+      // logit o sits in component o % 4 of L[o / 16] in the lanes of quarter (o % 16) / 4
+      const int ob = o >> 4, qq = (o & 15) >> 2, t = o & 3;
+      f32x4 lb = L[0];
+#pragma unroll
+      for (int k = 1; k < 4; ++k)
+        if (ob == k) lb = L[k];
+      const float acc = t == 0 ? lb[0] : t == 1 ? lb[1] : t == 2 ? lb[2] : lb[3];
+      int one = 0;
+      double term = 0.0;
+      if (q == qq && live) {
+        const float lf = 1.0f * acc + b3[o];
+        const double l = (double)lf;
+        const double p = 1.0 / (1.0 + exp(-l));
+        one = uo < p;
+        const double sgn = one ? l : -l;
+        term = sgn >= 0.0 ? -log1p(exp(-sgn)) : sgn - log1p(exp(sgn));
+      }
+      one = __shfl(one, qq * 16 + r);
+      term = __shfl(term, qq * 16 + r);
+      lq += term;
+      xb |= (uint64_t)(one & 1) << o;
+    }
+    if (live) {
+      if (q == 0) {
+        if (words) {
+          words[2 * (size_t)row] = xb;
+          words[2 * (size_t)row + 1] = 0;
+        }
+        if (logq) logq[row] = lq;
+      }
+      if (xout)
+        for (int k = q; k < n; k += 4) xout[(size_t)row * n + k] = ((xb >> k) & 1u) ? (uint16_t)0x3f80 : (uint16_t)0;
+    }
+  }
+}
+
 __global__ void k_f32_to_bf16(const float* in, uint16_t* out, long long count) {
   long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < count) out[i] = f32_to_bf16(in[i]);
@@ -433,9 +559,21 @@ extern "C" int rgs_made_sample_step(const float* logit, int ldl, const double* u
 extern "C" int rgs_made_sample(const uint16_t* W1, const float* b1, const uint16_t* W2, const float* b2,
                                const uint16_t* W3, const float* b3, const int* kpref, const double* u, int B, int n,
                                int H, uint64_t* words, uint16_t* xout, double* logq, void* stream) {
-  if (!W1 || !b1 || !W2 || !b2 || !W3 || !b3 || !kpref || !u || B <= 0 || n < 1 || n > 64 || H < 16 || H > 512 ||
+  if (!W1 || !b1 || !W2 || !b2 || !W3 || !b3 || !kpref || !u || B <= 0 || n < 1 || n > 64 || H < 16 ||
       H % 16 != 0 || (!words && !xout))
-    return fail(kInvalid, "made_sample: bad arguments (n <= 64, H <= 512 and a multiple of 16)");
+    return fail(kInvalid, "made_sample: bad arguments (n <= 64, H a multiple of 16)");
+  if (n >= 2 && H % (16 * (n - 1)) == 0 && H <= 2048) {
+    // degree-blocked layout: one wave per 16 proposals, first-layer fragments in LDS
+    const size_t lds = (size_t)(H / 16) * 64 * 8;
+    const int groups = (B + 15) / 16;
+    const int per_cu = std::max(1, (int)((160 * 1024) / lds));
+    const int grid = std::min(groups, 256 * per_cu);
+    hipLaunchKernelGGL(k_made_sample16, dim3(grid), dim3(64), lds, reinterpret_cast<hipStream_t>(stream), W1, b1, W2,
+                       b2, W3, b3, u, B, n, H, words, xout, logq);
+    RGS_LAUNCH_CHECK("k_made_sample16");
+    return kOk;
+  }
+  if (H > 512) return fail(kInvalid, "made_sample: H above 512 needs the degree-blocked layout H = 16 m (n - 1)");
   const size_t HS = (size_t)H + 8;
   const size_t lds = (2 * kSampRows * HS + 16 * HS + (size_t)kSampRows * kSampXS) * sizeof(uint16_t);
   static bool attr = false;

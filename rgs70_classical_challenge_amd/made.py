@@ -1,7 +1,8 @@
 """Learned proposal: a MADE network trained and sampled with this build's kernels.
 
 Architecture (BF16 weights and activations, FP32 accumulation, sigmoid
-conditionals): n binary inputs -> 512 -> 512 -> n logits, masked so that
+conditionals): n binary inputs -> H -> H -> n logits (H = 16 (n - 1) in the
+learned-proposal runs, so that each degree owns one 16-unit block), masked so that
 logit o depends only on inputs before o in qubit order (MADE, Germain et al.,
 arXiv:1502.03509).  Training is Adam on the mean binary cross-entropy of
 bitstrings drawn by the exact sampler; every GEMM, the loss, the bias
@@ -221,8 +222,11 @@ class MADE:
 
     def sample_fused(self, b, bits=False):
         """Draw b proposals from the uniforms in sb['u'][:b] with the fused
-        sampling kernel (one launch: 64 proposals per workgroup, activations
-        in LDS, the same arithmetic as the incremental GEMM sequence).
+        sampling kernel in one launch, with the same arithmetic as the
+        incremental GEMM sequence: for widths H = 16 m (n - 1) (every degree
+        owns m aligned 16-unit blocks) one wavefront per 16 proposals with no
+        workgroup barrier, otherwise 64 proposals per workgroup with both
+        layers' activations in LDS.
         Returns device views (packed words int64 [b, 2] in the scan's layout,
         log q float64 [b]) and, with bits=True, also writes the bf16 0/1 bits
         to sb['x'][:b]."""

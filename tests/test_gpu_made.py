@@ -162,10 +162,13 @@ def test_incremental_sampler_equals_full_forward():
 
 def test_fused_sampler_equals_incremental_sequence():
     """One-launch sampler against the launch-per-step incremental sampler:
-    identical bits and identical log q (same arithmetic, same order)."""
+    identical bits and identical log q (same arithmetic, same order).  Widths
+    H = 16 m (n - 1) take the degree-blocked one-wave kernel (m = 1 and 2
+    below), the others the 64-proposal workgroup kernel."""
     from rgs70_classical_challenge_amd.made import MADE
     from rgs70_classical_challenge_amd.rejection import pack_words_device
-    for n, H, count in ((20, 128, 3000), (34, 512, 1000), (9, 64, 77)):
+    for n, H, count in ((20, 128, 3000), (34, 512, 1000), (9, 64, 77),
+                        (34, 528, 1000), (9, 128, 77), (12, 352, 500), (20, 304, 3000)):
         net = MADE(n, hidden=H, batch=256, seed=5)
         rng = np.random.default_rng(n)
         data = (rng.random((4096, n)) < np.linspace(0.15, 0.85, n)).astype(np.uint8)
