@@ -33,6 +33,11 @@ Reported beside the headline value:
                 70-qubit instance (m=35: one 2^35 complex64 slab), replaying
                 6 recorded samples: seconds per sample, index matches and p
                 error against the recorded complex128 values.
+  learned       learned (MADE, H = 16 (n - 1)) against uniform proposals for
+                capped rejection at n=34: the MADE is trained on the GPU from
+                exact samples, both proposal streams are judged by the exact
+                scan and audited in complex128; accepted samples/s of each at
+                FP64-audited linear XEB equal within one standard error.
   cpu_baseline  the oracle (NumPy restatement of the reference's CPU
                 full-state path) on one host core: n=26 state + mass pyramid +
                 10,000 draws, extrapolated to n=34 by 2^8 * 34/26.
@@ -247,6 +252,21 @@ def run_scan_rgs70(torch, count=8):
     return out
 
 
+def run_learned(torch, n=34):
+    """Learned versus uniform proposals for capped rejection (accepted
+    samples/s at equal audited XEB); the full run's record minus the traces."""
+    from rgs70_classical_challenge_amd.learned import run
+    r = run(n=n, caps_learned=(1.6, 1.8, 2.0, 2.2, 2.5, 2.8), log=log)
+    keep = ('heldout_gain_nats', 'heldout_ceiling_nats', 'train_s', 'hidden', 'steps', 'scan_rmax')
+    out = {k: r[k] for k in keep}
+    out['uniform'] = {k: r['uniform'][k] for k in ('accepted_per_s', 'proposals_per_s', 'acceptance', 'xeb',
+                                                   'xeb_se', 'decision_flips_fp64', 'decision_flips_beyond_tolerance')}
+    out['learned'] = {c: {k: v[k] for k in ('accepted_per_s', 'proposals_per_s', 'acceptance', 'xeb', 'xeb_se',
+                                            'decision_flips_fp64')} for c, v in r['learned'].items()}
+    out['equal_xeb_comparison'] = r['equal_xeb_comparison']
+    return out
+
+
 def run_production(torch, count=6):
     """The reference's production workload on one device: the 70-qubit
     instance's 2^35-amplitude conditional state per sample, replaying its
@@ -384,6 +404,12 @@ def main():
         except Exception as e:
             out['production'] = {'error': repr(e)}
         log('[bench] production replay done')
+        torch.cuda.empty_cache()
+        try:
+            out['learned'] = run_learned(torch)
+        except Exception as e:
+            out['learned'] = {'error': repr(e)}
+        log('[bench] learned vs uniform done')
     if not args.no_cpu:
         out['cpu_baseline'] = cpu_baseline(args.seed)
         log('[bench] cpu baseline done')

@@ -140,21 +140,34 @@ __global__ void k_bce(const float* logits, const uint16_t* x, int B, int n, floa
   if (nll) nll[r] = acc;
 }
 
-// out[c] (+)= sum_r in[r][c]  (bias gradients)
-__global__ void k_colsum(const float* in, int rows, int cols, int ld, float* out, int accumulate) {
-  int c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= cols) return;
-  float s = 0.f;
-  for (int r = 0; r < rows; ++r) s += in[(size_t)r * ld + c];
-  out[c] = accumulate ? out[c] + s : s;
-}
+// out[c] (+)= sum_r in[r][c]  (bias gradients).  A workgroup takes 64
+// columns; its 16 row lanes sum rows r = lane, lane + 16, ... and the partial
+// sums are added in row-lane order, so the result does not depend on timing.
+constexpr int kColRowLanes = 16;
 
-__global__ void k_colsum_bf16(const uint16_t* in, int rows, int cols, int ld, float* out) {
-  int c = blockIdx.x * blockDim.x + threadIdx.x;
-  if (c >= cols) return;
+template <typename T>
+__device__ __forceinline__ float col_value(const T* p);
+template <>
+__device__ __forceinline__ float col_value<float>(const float* p) { return *p; }
+template <>
+__device__ __forceinline__ float col_value<uint16_t>(const uint16_t* p) { return bf16_to_f32(*p); }
+
+template <typename T>
+__global__ void __launch_bounds__(64 * kColRowLanes) k_colsum(const T* in, int rows, int cols, int ld, float* out,
+                                                              int accumulate) {
+  __shared__ float part[kColRowLanes][64];
+  const int cl = threadIdx.x & 63, rl = threadIdx.x >> 6;
+  const int c = blockIdx.x * 64 + cl;
   float s = 0.f;
-  for (int r = 0; r < rows; ++r) s += bf16_to_f32(in[(size_t)r * ld + c]);
-  out[c] = s;
+  if (c < cols)
+    for (int r = rl; r < rows; r += kColRowLanes) s += col_value<T>(in + (size_t)r * ld + c);
+  part[rl][cl] = s;
+  __syncthreads();
+  if (rl == 0 && c < cols) {
+    float t = part[0][cl];
+    for (int k = 1; k < kColRowLanes; ++k) t += part[k][cl];
+    out[c] = accumulate ? out[c] + t : t;
+  }
 }
 
 __global__ void k_step_counter(int* t) { t[0] += 1; }
@@ -504,16 +517,16 @@ extern "C" int rgs_made_bce(const float* logits, const uint16_t* x, int B, int n
 
 extern "C" int rgs_colsum(const float* in, int rows, int cols, int ld, float* out, int accumulate, void* stream) {
   if (!in || !out || rows <= 0 || cols <= 0) return fail(kInvalid, "colsum: bad arguments");
-  hipLaunchKernelGGL(k_colsum, dim3((cols + 255) / 256), dim3(256), 0, reinterpret_cast<hipStream_t>(stream), in,
-                     rows, cols, ld, out, accumulate);
+  hipLaunchKernelGGL(k_colsum<float>, dim3((cols + 63) / 64), dim3(64 * kColRowLanes), 0,
+                     reinterpret_cast<hipStream_t>(stream), in, rows, cols, ld, out, accumulate);
   RGS_LAUNCH_CHECK("k_colsum");
   return kOk;
 }
 
 extern "C" int rgs_colsum_bf16(const uint16_t* in, int rows, int cols, int ld, float* out, void* stream) {
   if (!in || !out || rows <= 0 || cols <= 0) return fail(kInvalid, "colsum: bad arguments");
-  hipLaunchKernelGGL(k_colsum_bf16, dim3((cols + 255) / 256), dim3(256), 0, reinterpret_cast<hipStream_t>(stream),
-                     in, rows, cols, ld, out);
+  hipLaunchKernelGGL(k_colsum<uint16_t>, dim3((cols + 63) / 64), dim3(64 * kColRowLanes), 0,
+                     reinterpret_cast<hipStream_t>(stream), in, rows, cols, ld, out, 0);
   RGS_LAUNCH_CHECK("k_colsum_bf16");
   return kOk;
 }

@@ -245,10 +245,12 @@ __global__ void __launch_bounds__(64 * W) k_scan_batch(const uint64_t* __restric
       // lanes past 2^r_out (steps of rank below six) computed in-range values
       // that are neither counted nor stored
       mxh = wave_max_u32(lane_ok ? mxh : 0u);
-      if (lane == 0) wmx[wave] = mxh;
+      if (W > 1 && lane == 0) wmx[wave] = mxh;
       __syncthreads();  // every read of E done; the waves' maxima visible
+      if (W > 1) {
 #pragma unroll
-      for (int w2 = 0; w2 < W; ++w2) mxh = max(mxh, wmx[w2]);
+        for (int w2 = 0; w2 < W; ++w2) mxh = max(mxh, wmx[w2]);
+      }
       const int e = scale_exponent(mxh);
       if (lane_ok) {
 #pragma unroll
@@ -276,7 +278,10 @@ template <typename S, int K, int W>
 static hipError_t launch_scan(const uint64_t* plan, const double2* u, const uint64_t* bits, int count,
                                 double2* val, int32_t* ex, int rmax, hipStream_t st) {
   size_t shmem = ((size_t)1 << rmax) * sizeof(S);
-  int grid = std::min(count, 256 * 16);
+  // at least two rounds of the workgroups that fit a CU at once (160 KiB of
+  // LDS, 32 waves), and at least 16 per CU (even shares of large batches)
+  const int per_cu = std::max(1, std::min((int)((160 * 1024) / shmem), 32 / W));
+  const int grid = std::min(count, 256 * std::max(16, 2 * per_cu));
   static bool attr = false;
   if (!attr) {
     // dynamic LDS up to the CU's 160 KiB less the kernel's own static array
@@ -292,13 +297,7 @@ static hipError_t launch_scan(const uint64_t* plan, const double2* u, const uint
 // Wavefronts per proposal for each table precision (the fastest measured at
 // rank 12); ranks with fewer 64-entry output blocks than that use one wave
 // per block.
-#ifndef RGS_SCAN_WAVES_C64
-#define RGS_SCAN_WAVES_C64 4
-#endif
-#ifndef RGS_SCAN_WAVES_C128
-#define RGS_SCAN_WAVES_C128 8
-#endif
-constexpr int kWavesC64 = RGS_SCAN_WAVES_C64, kWavesC128 = RGS_SCAN_WAVES_C128;
+constexpr int kWavesC64 = 4, kWavesC128 = 8;
 
 // Tables of rank up to R: 2^(R-6) output blocks of 64 entries, shared by
 // min(W, blocks) waves with K blocks each.

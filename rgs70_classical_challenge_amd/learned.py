@@ -87,16 +87,37 @@ def run(n=34, seed=20261205, train_samples=200_000, steps=20_000, batch=512, hid
         f'+- {uni["xeb_se"]:.4f}')
     out['learned'] = {}
     zs = {}
-    for c in caps_learned:
+
+    def learned_at(c):
         r = DeviceCappedRejection(scorer, auditor, n, c, proposal=net).run(seed + 3, accept_target,
                                                                            batch=proposal_batch)
         out['learned'][str(c)] = summary(r)
         zs[float(c)] = r['z']
         log(f'[learned] cap {c}: {r["accepted_per_s"]:.1f} accepted/s, XEB {r["xeb"]:.4f} +- {r["xeb_se"]:.4f}')
+        return r
+
+    def matches(r):
+        return abs(r['xeb'] - uni['xeb']) <= math.hypot(r['xeb_se'], uni['xeb_se'])
+
+    for c in caps_learned:
+        learned_at(c)
+    # XEB rises with the cap: when no grid cap matches the uniform XEB within
+    # one standard error, interpolate between the caps that bracket it
+    for _ in range(4):
+        if any(matches(r) for r in out['learned'].values()):
+            break
+        below = [(float(c), r['xeb']) for c, r in out['learned'].items() if r['xeb'] < uni['xeb']]
+        above = [(float(c), r['xeb']) for c, r in out['learned'].items() if r['xeb'] > uni['xeb']]
+        if not below or not above:
+            break
+        (c0, x0), (c1, x1) = max(below), min(above)
+        c = round(c0 + (uni['xeb'] - x0) * (c1 - c0) / (x1 - x0), 3)
+        if str(c) in out['learned'] or not c0 < c < c1:
+            break
+        learned_at(c)
     # learned caps whose XEB matches the uniform run within one standard error;
     # the fastest of them is the comparison
-    match = [(r['accepted_per_s'], float(c)) for c, r in out['learned'].items()
-             if abs(r['xeb'] - uni['xeb']) <= math.hypot(r['xeb_se'], uni['xeb_se'])]
+    match = [(r['accepted_per_s'], float(c)) for c, r in out['learned'].items() if matches(r)]
     if match:
         best = max(match)
         lr_ = out['learned'][str(best[1])]
