@@ -242,7 +242,13 @@ int rgs_scan_profile(const uint8_t *g, int n, const int32_t *order, int32_t *pro
 int rgs_scan_order_search(const uint8_t *g, int n, const int32_t *start, double seconds, uint64_t seed,
                           int rmax_cap, int32_t *best_order, uint64_t *best_cost);
 
-/* Pack compiled steps into the device plan (uint64 words).  Call with
+/* Pack compiled steps into the device plan (uint64 words): per step a header
+ * (v | r_in << 8 | r_out << 16 | nker << 24 | split << 32), beta, p0 over the
+ * low six output bits (64 words) and over the rest (2^(r_out-6) words), and
+ * the 2^nker kernel combinations.  The kernel basis is reduced on the y bit
+ * (bit r_in); when one basis vector keeps it (split = 1) that vector goes
+ * last and every p0 is stored as the y = 0 member of its coset, so the
+ * combinations' first half has y = 0 and the second y = 1.  Call with
  * blob == NULL to get the size in *words.  Ranks must be <= 28.  Replaces the
  * per-step byte_tables / kernel_combos uploads of GPUMixin.alloc
  * (upstream_scan/scan_plan.py:133-152, gpu_kernel.py:111-123). */
@@ -252,7 +258,9 @@ int rgs_scan_pack(const rgs_scan_step *steps, int n, uint64_t *blob, uint64_t *w
  * u_dev: complex128 [n][2][2] measurement unitaries (f_v = u[v][x_v][:]);
  * bits_dev: uint64 [count][2] bitstrings (qubit q = bit q).  precision 64
  * stores the tables as complex64 (rmax <= 13), 128 as complex128
- * (rmax <= 12); sums are FP64 in both.  Outputs: out_val complex128
+ * (rmax <= 12); sums are FP64 in both.  The plan must come from
+ * rgs_scan_pack (its split flag and y = 0 coset representatives are what the
+ * kernel's one-weight-per-output sums rely on).  Outputs: out_val complex128
  * [count] and out_exp int32 [count] with amplitude
  * = out_val * 2^out_exp * 2^{-n/2}.  Replaces ScanOracle.amplitude /
  * GPUMixin._sweep / scan_step_scaled (work/rgs_attack/scan_oracle.py:25-29,
@@ -329,14 +337,17 @@ int rgs_sample_col(const float *logits, const double *u, int B, int n, int o, ui
 int rgs_made_sample_step(const float *logit, int ldl, const double *u, int B, int n, int o, uint16_t *x,
                          double *logq, int first, void *stream);
 
-/* All n steps of incremental sampling in one launch (n <= 64, H <= 512, a
- * multiple of 16): 64 proposals per workgroup with their activations in LDS,
- * the same arithmetic as the sequence of rgs_gemm_bf16 / rgs_made_sample_step
- * calls (bit-identical results).  W1 bf16 [H][n], W2 [H][H], W3 [n][H] (rows
- * masked), biases FP32, kpref dev int32[n] = number of hidden units of degree
- * <= o (degrees sorted), u FP64 [B][n].  Outputs (words and/or xout
- * required): words uint64 [B][2] (qubit q = bit q % 64 of word q / 64), xout
- * bf16 0/1 [B][n], logq FP64 [B]. */
+/* All n steps of incremental sampling in one launch (n <= 64, H a multiple
+ * of 16), with the same arithmetic as the sequence of rgs_gemm_bf16 /
+ * rgs_made_sample_step calls (bit-identical results).  Widths H = 16 m (n-1)
+ * up to 2048 (every degree owns m aligned 16-unit blocks) run one wavefront
+ * per 16 proposals with transposed MFMA products and no workgroup barrier;
+ * other widths up to 512 run 64 proposals per workgroup with both layers'
+ * activations in LDS.  W1 bf16 [H][n], W2 [H][H], W3 [n][H] (rows masked),
+ * biases FP32, kpref dev int32[n] = number of hidden units of degree <= o
+ * (degrees sorted), u FP64 [B][n].  Outputs (words and/or xout required):
+ * words uint64 [B][2] (qubit q = bit q % 64 of word q / 64), xout bf16 0/1
+ * [B][n], logq FP64 [B]. */
 int rgs_made_sample(const uint16_t *W1, const float *b1, const uint16_t *W2, const float *b2, const uint16_t *W3,
                     const float *b3, const int *kpref, const double *u, int B, int n, int H, uint64_t *words,
                     uint16_t *xout, double *logq, void *stream);
