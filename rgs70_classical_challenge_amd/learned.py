@@ -24,7 +24,7 @@ def index_to_bits(idx, n):
 
 def run(n=34, seed=20261205, train_samples=200_000, steps=20_000, batch=512, hidden=None, cap=8.0,
         caps_learned=(1.0, 1.2, 1.4, 1.5, 1.6, 1.7, 1.8, 1.9, 2.0, 2.2, 2.5, 4.0, 8.0), accept_target=100_000,
-        proposal_batch=65536, lr=1e-3, order_seconds=5.0, analysis_resamples=2000, log=print):
+        proposal_batch=65536, lr=2e-3, order_seconds=5.0, analysis_resamples=2000, log=print):
     import torch
     from .engine import GraphStateSampler
     from .records import capped_statistics
@@ -102,17 +102,23 @@ def run(n=34, seed=20261205, train_samples=200_000, steps=20_000, batch=512, hid
     for c in caps_learned:
         learned_at(c)
     # XEB rises with the cap: when no grid cap matches the uniform XEB within
-    # one standard error, interpolate between the caps that bracket it
-    for _ in range(4):
+    # one standard error, widen the grid until it brackets that XEB, then
+    # interpolate between the bracketing caps
+    for _ in range(8):
         if any(matches(r) for r in out['learned'].values()):
             break
         below = [(float(c), r['xeb']) for c, r in out['learned'].items() if r['xeb'] < uni['xeb']]
         above = [(float(c), r['xeb']) for c, r in out['learned'].items() if r['xeb'] > uni['xeb']]
-        if not below or not above:
-            break
-        (c0, x0), (c1, x1) = max(below), min(above)
-        c = round(c0 + (uni['xeb'] - x0) * (c1 - c0) / (x1 - x0), 3)
-        if str(c) in out['learned'] or not c0 < c < c1:
+        if not above:
+            c = round(max(below)[0] * 1.5, 3)
+        elif not below:
+            c = round(max(1.0, min(above)[0] / 1.5), 3)
+        else:
+            (c0, x0), (c1, x1) = max(below), min(above)
+            c = round(c0 + (uni['xeb'] - x0) * (c1 - c0) / (x1 - x0), 3)
+            if not c0 < c < c1:
+                break
+        if str(c) in out['learned']:
             break
         learned_at(c)
     # learned caps whose XEB matches the uniform run within one standard error;
