@@ -257,7 +257,7 @@ def run_learned(torch, n=34):
     samples/s at equal audited XEB); the full run's record minus the traces."""
     from rgs70_classical_challenge_amd.learned import run
     r = run(n=n, caps_learned=(1.6, 1.8, 2.0, 2.2, 2.5, 2.8), log=log)
-    keep = ('heldout_gain_nats', 'heldout_ceiling_nats', 'train_s', 'hidden', 'steps', 'scan_rmax')
+    keep = ('heldout_gain_nats', 'heldout_ceiling_nats', 'train_s', 'hidden', 'steps', 'scan_rmax', 'scan_sum_states')
     out = {k: r[k] for k in keep}
     out['uniform'] = {k: r['uniform'][k] for k in ('accepted_per_s', 'proposals_per_s', 'acceptance', 'xeb',
                                                    'xeb_se', 'decision_flips_fp64', 'decision_flips_beyond_tolerance')}

@@ -190,14 +190,29 @@ __global__ void k_adam(float* w, const float* grad, const float* mask, float* m,
   wb[i] = f32_to_bf16(wi);
 }
 
-// Autoregressive sampling step: x[r][o] = (u[r][o] * (1 + e^{-l}) < 1), i.e.
-// u < sigmoid(l), for the column o just computed.
+// One draw from a float logit l, in FP64: bit = [u < sigmoid(l)] and *term =
+// log sigmoid(+-l) of the chosen bit.  With e = exp(-|l|), sigmoid(l) is
+// 1 / (1 + e) for l >= 0 and e / (1 + e) otherwise, and log sigmoid(s) =
+// min(s, 0) - log1p(e) for s = +-l, so one exponential serves both.  Every
+// sampler (per step and fused) draws through this function, so they agree
+// bit for bit.
+__device__ __forceinline__ int made_bit(float lf, double u, double* term) {
+  const double l = (double)lf;
+  const double e = exp(-fabs(l));
+  const double d = 1.0 + e;
+  const double p = l >= 0.0 ? 1.0 / d : e / d;
+  const int one = u < p;
+  if (term) *term = fmin(one ? l : -l, 0.0) - log1p(e);
+  return one;
+}
+
+// Autoregressive sampling step: x[r][o] = (u[r][o] < sigmoid(l)) for the
+// column o just computed.
 __global__ void k_sample_col(const float* logits, const double* u, int B, int n, int o, uint16_t* x) {
   int r = blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= B) return;
-  float l = logits[(size_t)r * n + o];
-  double p = 1.0 / (1.0 + exp(-(double)l));
-  x[(size_t)r * n + o] = u[(size_t)r * n + o] < p ? (uint16_t)0x3f80 : (uint16_t)0;  // bf16 1.0 / 0.0
+  const int one = made_bit(logits[(size_t)r * n + o], u[(size_t)r * n + o], nullptr);
+  x[(size_t)r * n + o] = one ? (uint16_t)0x3f80 : (uint16_t)0;  // bf16 1.0 / 0.0
 }
 
 // One autoregressive step: x[b][o] = (u[b][o] < sigmoid(l[b])), and log q
@@ -207,12 +222,8 @@ __global__ void k_sample_step(const float* logit, int ldl, const double* u, int 
                               double* logq, int first) {
   int r = blockIdx.x * blockDim.x + threadIdx.x;
   if (r >= B) return;
-  const double l = (double)logit[(size_t)r * ldl];
-  const double p = 1.0 / (1.0 + exp(-l));
-  const bool one = u[(size_t)r * n + o] < p;
-  // log sigmoid(s) = -log1p(exp(-s)) for s >= 0, s - log1p(exp(s)) otherwise
-  const double s = one ? l : -l;
-  const double ls = s >= 0.0 ? -log1p(exp(-s)) : s - log1p(exp(s));
+  double ls;
+  const int one = made_bit(logit[(size_t)r * ldl], u[(size_t)r * n + o], &ls);
   x[(size_t)r * n + o] = one ? (uint16_t)0x3f80 : (uint16_t)0;
   logq[r] = first ? ls : logq[r] + ls;
 }
@@ -325,12 +336,9 @@ __global__ void __launch_bounds__(256) k_made_sample(const uint16_t* __restrict_
       for (int t = 0; t < 4; ++t) {
         const int r = row0 + rw + kq + t;
         if (r < B) {
-          const float lf = 1.0f * acc[t] + b3[o];
-          const double l = (double)lf;
-          const double p = 1.0 / (1.0 + exp(-l));
-          const bool one = u[(size_t)r * n + o] < p;
-          const double sgn = one ? l : -l;
-          lq[t] += sgn >= 0.0 ? -log1p(exp(-sgn)) : sgn - log1p(exp(sgn));
+          double term;
+          const int one = made_bit(1.0f * acc[t] + b3[o], u[(size_t)r * n + o], &term);
+          lq[t] += term;
           xs[(rw + kq + t) * kSampXS + o] = one ? (uint16_t)0x3f80 : (uint16_t)0;
         }
       }
@@ -380,6 +388,76 @@ __device__ __forceinline__ f32x4 mfma16(bf16x4 a, bf16x4 b, f32x4 c) {
   return __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, c, 0, 0, 0);
 }
 
+// Logit o sits in component o % 4 of L[o / 16] in the lanes of quarter
+// (o % 16) / 4; those lanes draw bit o as k_sample_step does and the bit and
+// its log-probability term are broadcast to the row's four lanes.
+// The four logit blocks are separate variables so that no dynamic index
+// turns them into a scratch array.
+struct LogitBlocks {
+  f32x4 b0, b1, b2, b3;
+};
+
+__device__ __forceinline__ void made_logits(LogitBlocks& L, int nob, const bf16x4 (&w3)[4], bf16x4 h) {
+  L.b0 = mfma16(w3[0], h, L.b0);
+  if (nob > 1) L.b1 = mfma16(w3[1], h, L.b1);
+  if (nob > 2) L.b2 = mfma16(w3[2], h, L.b2);
+  if (nob > 3) L.b3 = mfma16(w3[3], h, L.b3);
+}
+
+__device__ __forceinline__ void made_draw(const LogitBlocks& L, const float* __restrict__ b3, int o, int q, int r,
+                                          bool live, double uo, uint64_t& xb, double& lq) {
+  const int ob = o >> 4, qq = (o & 15) >> 2, t = o & 3;
+  const f32x4 lb = ob == 0 ? L.b0 : ob == 1 ? L.b1 : ob == 2 ? L.b2 : L.b3;
+  const float acc = t == 0 ? lb[0] : t == 1 ? lb[1] : t == 2 ? lb[2] : lb[3];
+  int one = 0;
+  double term = 0.0;
+  if (q == qq && live) one = made_bit(1.0f * acc + b3[o], uo, &term);
+  one = __shfl(one, qq * 16 + r);
+  term = __shfl(term, qq * 16 + r);
+  lq += term;
+  xb |= (uint64_t)(one & 1) << o;
+}
+
+__device__ __forceinline__ bf16x4 made_xfrag(uint64_t xb, int kb, int q) {
+  bf16x4 b;
+#pragma unroll
+  for (int i = 0; i < 4; ++i) b[i] = (short)(((xb >> (kb * 16 + 4 * q + i)) & 1u) ? 0x3f80 : 0);
+  return b;
+}
+
+__device__ __forceinline__ bf16x4 made_relu_bf16(f32x4 acc, const float* __restrict__ bias) {
+  bf16x4 h;
+#pragma unroll
+  for (int t = 0; t < 4; ++t) h[t] = (short)f32_to_bf16(fmaxf(1.0f * acc[t] + bias[t], 0.f));
+  return h;
+}
+
+constexpr int kW2Ahead = 8;   // second-layer fragments of the next step loaded a step ahead
+
+// Weight fragments of the block that serves step st (one block per degree):
+// first layer over inputs < st, the first kW2Ahead second-layer K blocks, and
+// the logit rows.
+__device__ __forceinline__ void made_prefetch(const uint16_t* __restrict__ W1, const uint16_t* __restrict__ W2,
+                                              const uint16_t* __restrict__ W3, int n, int H, int nob, int st, int r,
+                                              int q, bf16x4 (&w1n)[4], bf16x4 (&w2n)[kW2Ahead], bf16x4 (&w3n)[4]) {
+  const bf16x4 zero = {0, 0, 0, 0};
+  const int blk = st - 1;
+  const uint16_t* w1r = W1 + (size_t)(blk * 16 + r) * n;
+#pragma unroll
+  for (int kb = 0; kb < 4; ++kb) {
+    const int k = kb * 16 + 4 * q;
+    const bf16x4 a = {(short)(k < st ? w1r[k] : 0), (short)(k + 1 < st ? w1r[k + 1] : 0),
+                      (short)(k + 2 < st ? w1r[k + 2] : 0), (short)(k + 3 < st ? w1r[k + 3] : 0)};
+    w1n[kb] = a;
+  }
+  const uint16_t* w2r = W2 + (size_t)(blk * 16 + r) * H + 4 * q;
+#pragma unroll
+  for (int kb = 0; kb < kW2Ahead; ++kb) w2n[kb] = kb < st ? load4(w2r + kb * 16) : zero;
+  const uint16_t* w3r = W3 + (size_t)r * H + blk * 16 + 4 * q;
+#pragma unroll
+  for (int ob = 0; ob < 4; ++ob) w3n[ob] = (ob < nob && 16 * ob + r < n) ? load4(w3r + (size_t)16 * ob * H) : zero;
+}
+
 __global__ void __launch_bounds__(64) k_made_sample16(const uint16_t* __restrict__ W1, const float* __restrict__ b1,
                                                       const uint16_t* __restrict__ W2, const float* __restrict__ b2,
                                                       const uint16_t* __restrict__ W3, const float* __restrict__ b3,
@@ -397,79 +475,91 @@ __global__ void __launch_bounds__(64) k_made_sample16(const uint16_t* __restrict
     const bool live = row < B;
     uint64_t xb = 0;   // bits drawn so far for this lane's row
     double lq = 0.0;
-    f32x4 L[4] = {{0.f, 0.f, 0.f, 0.f}, {0.f, 0.f, 0.f, 0.f}, {0.f, 0.f, 0.f, 0.f}, {0.f, 0.f, 0.f, 0.f}};
+    LogitBlocks L = {{0.f, 0.f, 0.f, 0.f}, {0.f, 0.f, 0.f, 0.f}, {0.f, 0.f, 0.f, 0.f}, {0.f, 0.f, 0.f, 0.f}};
     double un = live ? u[(size_t)row * n] : 0.0;   // uniform of the next draw, loaded a step ahead
-    for (int o = 0; o < n; ++o) {
-      const double uo = un;
-      if (o + 1 < n && live) un = u[(size_t)row * n + o + 1];
-      if (o > 0) {
-        const int g0 = (o - 1) * g16;
-        for (int j = 0; j < g16; ++j) {   // first layer: inputs 0 .. o-1
-          const int blk = g0 + j;
-          const uint16_t* w1r = W1 + (size_t)(blk * 16 + r) * n;
+    if (g16 == 1) {
+      // One block per degree: block s-1 serves step s.  Its weight fragments
+      // (first layer, the first kW2Ahead second-layer K blocks, the logit
+      // rows) are loaded during the previous step, before its draw, so their
+      // L2 latency overlaps the draw's FP64 work.
+      bf16x4 w1n[4], w2n[kW2Ahead], w3n[4];
+      if (n > 1) made_prefetch(W1, W2, W3, n, H, nob, 1, r, q, w1n, w2n, w3n);
+      for (int o = 0; o < n; ++o) {
+        const double uo = un;
+        if (o + 1 < n && live) un = u[(size_t)row * n + o + 1];
+        if (o > 0) {
+          const int blk = o - 1;
           f32x4 acc = {0.f, 0.f, 0.f, 0.f};
-          for (int kb = 0; kb * 16 < o; ++kb) {
-            bf16x4 a, b;
 #pragma unroll
-            for (int i = 0; i < 4; ++i) {
-              const int k = kb * 16 + 4 * q + i;
-              a[i] = (short)(k < o ? w1r[k] : 0);
-              b[i] = (short)(((xb >> k) & 1u) ? 0x3f80 : 0);
-            }
-            acc = mfma16(a, b, acc);
-          }
-          bf16x4 h;
+          for (int kb = 0; kb < 4; ++kb)
+            if (kb * 16 < o) acc = mfma16(w1n[kb], made_xfrag(xb, kb, q), acc);
+          h1[blk * 64 + lane] = made_relu_bf16(acc, b1 + blk * 16 + 4 * q);
+          acc = (f32x4){0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-          for (int t = 0; t < 4; ++t) h[t] = (short)f32_to_bf16(fmaxf(1.0f * acc[t] + b1[blk * 16 + 4 * q + t], 0.f));
-          h1[blk * 64 + lane] = h;
-        }
-        const int nkb = g0 + g16;   // first-layer blocks of degree <= o
-        for (int j = 0; j < g16; ++j) {   // second layer, then its logit contributions
-          const int blk = g0 + j;
+          for (int kb = 0; kb < kW2Ahead; ++kb)
+            if (kb < o) acc = mfma16(w2n[kb], h1[kb * 64 + lane], acc);
           const uint16_t* w2r = W2 + (size_t)(blk * 16 + r) * H + 4 * q;
-          f32x4 acc = {0.f, 0.f, 0.f, 0.f};
-          int kb = 0;
-          for (; kb + 8 <= nkb; kb += 8) {   // eight fragment loads in flight, then the MFMAs in K order
-            bf16x4 wa[8], hb[8];
+          for (int kb = kW2Ahead; kb < o; kb += kW2Ahead) {   // later K blocks, kW2Ahead loads in flight
+            bf16x4 wa[kW2Ahead];
 #pragma unroll
-            for (int i = 0; i < 8; ++i) {
-              wa[i] = load4(w2r + (kb + i) * 16);
-              hb[i] = h1[(kb + i) * 64 + lane];
-            }
+            for (int i = 0; i < kW2Ahead; ++i) wa[i] = kb + i < o ? load4(w2r + (kb + i) * 16) : zero;
 #pragma unroll
-            for (int i = 0; i < 8; ++i) acc = mfma16(wa[i], hb[i], acc);
+            for (int i = 0; i < kW2Ahead; ++i)
+              if (kb + i < o) acc = mfma16(wa[i], h1[(kb + i) * 64 + lane], acc);
           }
-          for (; kb < nkb; ++kb) acc = mfma16(load4(w2r + kb * 16), h1[kb * 64 + lane], acc);
-          bf16x4 h;
-#pragma unroll
-          for (int t = 0; t < 4; ++t) h[t] = (short)f32_to_bf16(fmaxf(1.0f * acc[t] + b2[blk * 16 + 4 * q + t], 0.f));
-          const uint16_t* w3r = W3 + (size_t)r * H + blk * 16 + 4 * q;
-#pragma unroll
-          for (int ob = 0; ob < 4; ++ob)
-            if (ob < nob) L[ob] = mfma16(16 * ob + r < n ? load4(w3r + (size_t)16 * ob * H) : zero, h, L[ob]);
+          made_logits(L, nob, w3n, made_relu_bf16(acc, b2 + blk * 16 + 4 * q));
         }
+        if (o + 1 < n) made_prefetch(W1, W2, W3, n, H, nob, o + 1, r, q, w1n, w2n, w3n);
+        made_draw(L, b3, o, q, r, live, uo, xb, lq);
       }
-      // logit o sits in component o % 4 of L[o / 16] in the lanes of quarter (o % 16) / 4
-      const int ob = o >> 4, qq = (o & 15) >> 2, t = o & 3;
-      f32x4 lb = L[0];
+    } else {
+      for (int o = 0; o < n; ++o) {
+        const double uo = un;
+        if (o + 1 < n && live) un = u[(size_t)row * n + o + 1];
+        if (o > 0) {
+          const int g0 = (o - 1) * g16;
+          for (int j = 0; j < g16; ++j) {   // first layer: inputs 0 .. o-1
+            const int blk = g0 + j;
+            const uint16_t* w1r = W1 + (size_t)(blk * 16 + r) * n;
+            f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+            for (int kb = 0; kb * 16 < o; ++kb) {
+              bf16x4 a;
 #pragma unroll
-      for (int k = 1; k < 4; ++k)
-        if (ob == k) lb = L[k];
-      const float acc = t == 0 ? lb[0] : t == 1 ? lb[1] : t == 2 ? lb[2] : lb[3];
-      int one = 0;
-      double term = 0.0;
-      if (q == qq && live) {
-        const float lf = 1.0f * acc + b3[o];
-        const double l = (double)lf;
-        const double p = 1.0 / (1.0 + exp(-l));
-        one = uo < p;
-        const double sgn = one ? l : -l;
-        term = sgn >= 0.0 ? -log1p(exp(-sgn)) : sgn - log1p(exp(sgn));
+              for (int i = 0; i < 4; ++i) {
+                const int k = kb * 16 + 4 * q + i;
+                a[i] = (short)(k < o ? w1r[k] : 0);
+              }
+              acc = mfma16(a, made_xfrag(xb, kb, q), acc);
+            }
+            h1[blk * 64 + lane] = made_relu_bf16(acc, b1 + blk * 16 + 4 * q);
+          }
+          const int nkb = g0 + g16;   // first-layer blocks of degree <= o
+          for (int j = 0; j < g16; ++j) {   // second layer, then its logit contributions
+            const int blk = g0 + j;
+            const uint16_t* w2r = W2 + (size_t)(blk * 16 + r) * H + 4 * q;
+            f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+            int kb = 0;
+            for (; kb + 8 <= nkb; kb += 8) {   // eight fragment loads in flight, then the MFMAs in K order
+              bf16x4 wa[8], hb[8];
+#pragma unroll
+              for (int i = 0; i < 8; ++i) {
+                wa[i] = load4(w2r + (kb + i) * 16);
+                hb[i] = h1[(kb + i) * 64 + lane];
+              }
+#pragma unroll
+              for (int i = 0; i < 8; ++i) acc = mfma16(wa[i], hb[i], acc);
+            }
+            for (; kb < nkb; ++kb) acc = mfma16(load4(w2r + kb * 16), h1[kb * 64 + lane], acc);
+            const uint16_t* w3r = W3 + (size_t)r * H + blk * 16 + 4 * q;
+            bf16x4 w3[4];
+#pragma unroll
+            for (int ob = 0; ob < 4; ++ob)
+              w3[ob] = (ob < nob && 16 * ob + r < n) ? load4(w3r + (size_t)16 * ob * H) : zero;
+            made_logits(L, nob, w3, made_relu_bf16(acc, b2 + blk * 16 + 4 * q));
+          }
+        }
+        made_draw(L, b3, o, q, r, live, uo, xb, lq);
       }
-      one = __shfl(one, qq * 16 + r);
-      term = __shfl(term, qq * 16 + r);
-      lq += term;
-      xb |= (uint64_t)(one & 1) << o;
     }
     if (live) {
       if (q == 0) {

@@ -14,6 +14,7 @@ complex128 one is a flip; flips whose margin exceeds FLIP_TOLERANCE of the
 acceptance (the 1e-5 relative p(x) tolerance) are counted apart, since only
 those are errors of the scorer rather than ties within its precision.
 """
+import math
 import time
 
 import numpy as np
@@ -123,10 +124,30 @@ def pack_words_device(x, n):
     return words
 
 
+def uniform_words_device(count, n, gen, dev):
+    """count uniform n-bit proposals drawn directly as int64 [count, 2] words
+    (qubit q at bit q of word q // 64): each word is one uniform integer
+    below 2^bits, so every bit is an independent fair coin."""
+    import torch
+    words = torch.zeros((count, 2), dtype=torch.int64, device=dev)
+    for w in range(2):
+        bits = min(64, n - 64 * w)
+        if bits <= 0:
+            continue
+        if bits <= 62:
+            words[:, w] = torch.randint(0, 1 << bits, (count,), generator=gen, device=dev, dtype=torch.int64)
+        else:
+            lo = torch.randint(0, 1 << 32, (count,), generator=gen, device=dev, dtype=torch.int64)
+            hi = torch.randint(0, 1 << (bits - 32), (count,), generator=gen, device=dev, dtype=torch.int64)
+            words[:, w] = lo | (hi << 32)
+    return words
+
+
 class DeviceCappedRejection:
     """Capped rejection with every proposal, uniform and decision on the
-    device (seeded torch generator): uniform proposals (bits and acceptance
-    uniforms) or MADE proposals (incremental sampler, captured graph).  The
+    device (seeded torch generator): uniform proposals (packed words drawn as
+    uniform integers, and acceptance uniforms) or MADE proposals (the fused
+    sampler).  The
     timed loop draws, scores (complex64-table scan) and decides; the
     complex128 audit of every decision runs afterwards, untimed."""
 
@@ -146,34 +167,42 @@ class DeviceCappedRejection:
         kept_words, kept_u, kept_logq, kept_keep = [], [], [], []
         accepted = 0
         proposals = 0
+        batches = 0
         torch.cuda.synchronize(dev)
         t0 = time.perf_counter()
-        for _ in range(max_batches):
-            if net is None:
-                x = torch.randint(0, 2, (batch, n), generator=gen, device=dev, dtype=torch.uint8)
-                logq = None
-            else:
-                sb['u'][:batch].uniform_(generator=gen)
-                words, logq = net.sample_fused(batch)
-                words = words.clone()
-                x = None
-            u = torch.rand(batch, generator=gen, device=dev, dtype=torch.float64)
-            if x is not None:
-                words = pack_words_device(x, n)
-            p = self.scorer.probabilities_device(words)
-            if logq is None:
-                acc = torch.clamp(p * (2.0 ** n / cap), max=1.0)
-            else:
-                acc = torch.clamp(p / (cap * torch.exp(logq)), max=1.0)
-            keep = u < acc
-            accepted += int(keep.sum().item())
-            proposals += batch
-            kept_words.append(words)
-            kept_u.append(u)
-            kept_logq.append(None if logq is None else logq.clone())
-            kept_keep.append(keep)
+        # The host reads the accepted count after the first batch, then
+        # enqueues as many batches as that rate says the target needs before
+        # reading again, so the device is not left idle between batches.
+        ahead = 1
+        while batches < max_batches:
+            counts = []
+            for _ in range(min(ahead, max_batches - batches)):
+                if net is None:
+                    words = uniform_words_device(batch, n, gen, dev)
+                    logq = None
+                else:
+                    sb['u'][:batch].uniform_(generator=gen)
+                    words, logq = net.sample_fused(batch)
+                    words = words.clone()
+                u = torch.rand(batch, generator=gen, device=dev, dtype=torch.float64)
+                p = self.scorer.probabilities_device(words)
+                if logq is None:
+                    acc = torch.clamp(p * (2.0 ** n / cap), max=1.0)
+                else:
+                    acc = torch.clamp(p / (cap * torch.exp(logq)), max=1.0)
+                keep = u < acc
+                counts.append(keep.sum())
+                proposals += batch
+                batches += 1
+                kept_words.append(words)
+                kept_u.append(u)
+                kept_logq.append(None if logq is None else logq.clone())
+                kept_keep.append(keep)
+            accepted += int(torch.stack(counts).sum().item())
             if accepted >= accept_target:
                 break
+            rate = max(accepted / proposals, 1e-9)
+            ahead = max(1, math.ceil((accept_target - accepted) / (rate * batch)))
         torch.cuda.synchronize(dev)
         seconds = time.perf_counter() - t0
         # audit: complex128 tables for every proposal, decisions recomputed in FP64
