@@ -262,7 +262,8 @@ def run_learned(torch, n=34):
     out['uniform'] = {k: r['uniform'][k] for k in ('accepted_per_s', 'proposals_per_s', 'acceptance', 'xeb',
                                                    'xeb_se', 'decision_flips_fp64', 'decision_flips_beyond_tolerance')}
     out['learned'] = {c: {k: v[k] for k in ('accepted_per_s', 'proposals_per_s', 'acceptance', 'xeb', 'xeb_se',
-                                            'decision_flips_fp64')} for c, v in r['learned'].items()}
+                                            'decision_flips_fp64', 'decision_flips_beyond_tolerance')}
+                      for c, v in r['learned'].items()}
     out['equal_xeb_comparison'] = r['equal_xeb_comparison']
     out['analysis'] = r.get('analysis')
     return out
