@@ -339,11 +339,13 @@ int rgs_made_sample_step(const float *logit, int ldl, const double *u, int B, in
 
 /* All n steps of incremental sampling in one launch (n <= 64, H a multiple
  * of 16), with the same arithmetic as the sequence of rgs_gemm_bf16 /
- * rgs_made_sample_step calls (bit-identical results).  Widths H = 16 m (n-1)
- * up to 2048 (every degree owns m aligned 16-unit blocks) run one wavefront
- * per 16 proposals with transposed MFMA products and no workgroup barrier;
- * other widths up to 512 run 64 proposals per workgroup with both layers'
- * activations in LDS.  W1 bf16 [H][n], W2 [H][H], W3 [n][H] (rows masked),
+ * rgs_made_sample_step calls (bit-identical results), for widths up to 2048
+ * whose degrees own whole aligned 16-unit blocks (H = 16 m (n-1), or
+ * H = 16 k with k < n-1 and degrees 1..k, as hidden_degrees assigns them):
+ * one wavefront per 16 proposals with transposed MFMA products and no
+ * workgroup barrier.  Other widths return an invalid-argument error (the step
+ * sequence serves them).  kpref is checked for presence only: the degree
+ * blocks follow from n and H.  W1 bf16 [H][n], W2 [H][H], W3 [n][H] (rows masked),
  * biases FP32, kpref dev int32[n] = number of hidden units of degree <= o
  * (degrees sorted), u FP64 [B][n].  Outputs (words and/or xout required):
  * words uint64 [B][2] (qubit q = bit q % 64 of word q / 64), xout bf16 0/1
@@ -351,6 +353,22 @@ int rgs_made_sample_step(const float *logit, int ldl, const double *u, int B, in
 int rgs_made_sample(const uint16_t *W1, const float *b1, const uint16_t *W2, const float *b2, const uint16_t *W3,
                     const float *b3, const int *kpref, const double *u, int B, int n, int H, uint64_t *words,
                     uint16_t *xout, double *logq, void *stream);
+
+/* Fragment-major copy of the sampler's weights for rgs_made_sample_packed
+ * (degree-blocked widths only): first layer [H/16][ceil(n/16)] tiles with
+ * inputs at or past each block's degree zeroed, second layer [H/16][H/16]
+ * tiles, logit rows [ceil(n/16)][H/16] tiles (rows past n zeroed); each
+ * 16 x 16 tile is 64 lanes x 4 bf16 in the MFMA A-operand order, so the
+ * sampler's fragment loads are contiguous whatever the row stride.  Call
+ * with packed == NULL to get the size in *bytes.  No reference counterpart
+ * (the learned proposal is new). */
+int rgs_made_sample_pack(const uint16_t *W1, const uint16_t *W2, const uint16_t *W3, int n, int H, void *packed,
+                         uint64_t *bytes, void *stream);
+
+/* rgs_made_sample reading the packed weights of rgs_made_sample_pack:
+ * bit-identical outputs. */
+int rgs_made_sample_packed(const void *packed, const float *b1, const float *b2, const float *b3, const double *u,
+                           int B, int n, int H, uint64_t *words, uint16_t *xout, double *logq, void *stream);
 
 /* bf16 copy (round to nearest even) of an FP32 array. */
 int rgs_f32_to_bf16(const float *in, uint16_t *out, long long count, void *stream);

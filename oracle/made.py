@@ -14,10 +14,10 @@ import numpy as np
 
 
 def made_masks(n, hidden, seed=0):
-    """Degrees: inputs 1..n in order, hidden units take 1..n-1 evenly in
-    sorted order (``seed`` is kept for the signature; the assignment is
-    deterministic); a hidden unit of degree d sees inputs of degree <= d and
-    output o (degree o+1) sees hidden units of degree < o+1."""
+    """Degrees: inputs 1..n in order, hidden units take degrees in 1..n-1 in
+    sorted order (hidden_degrees; ``seed`` is kept for the signature, the
+    assignment is deterministic); a hidden unit of degree d sees inputs of
+    degree <= d and output o (degree o+1) sees hidden units of degree < o+1."""
     deg_in = np.arange(1, n + 1)
     d1 = hidden_degrees(n, hidden)
     d2 = hidden_degrees(n, hidden)
@@ -40,8 +40,19 @@ def _exact(a):
 
 
 def hidden_degrees(n, hidden):
-    """Sorted hidden-unit degrees: unit i has degree floor(i (n-1) / hidden) + 1."""
-    return (np.arange(hidden) * max(1, n - 1)) // hidden + 1
+    """Sorted hidden-unit degrees in 1 .. n-1.  Widths that are whole 16-unit
+    blocks give each degree whole blocks: m blocks per degree when the block
+    count is m (n-1), one block per degree 1 .. k when there are k < n-1
+    blocks.  Other widths spread the degrees evenly:
+    floor(i (n-1) / hidden) + 1."""
+    d = max(1, n - 1)
+    units = np.arange(hidden)
+    if hidden % 16 == 0 and (hidden // 16) % d == 0:
+        per = 16 * ((hidden // 16) // d)
+        return np.array([u // per + 1 for u in units])
+    if hidden % 16 == 0 and hidden // 16 < d:
+        return np.array([u // 16 + 1 for u in units])
+    return (units * d) // hidden + 1
 
 
 def forward(params, masks, x, store=_exact):

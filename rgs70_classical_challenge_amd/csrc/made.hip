@@ -228,148 +228,13 @@ __global__ void k_sample_step(const float* logit, int ldl, const double* u, int 
   logq[r] = first ? ls : logq[r] + ls;
 }
 
-// Fused incremental sampler: one workgroup of four wavefronts draws 64
-// proposals through all n steps.  Hidden degrees are sorted, so at step o the
-// units of degree <= o are the prefix [0, kpref[o]) of each hidden layer and
-// the new units are [kpref[o-1], kpref[o]) (at most 16).  Each wavefront owns
-// 16 proposals and keeps their BF16 activations of both layers in LDS; the
-// step's slice of W2 (new rows x prefix columns) is staged in LDS once for the
-// four wavefronts.  Per step: the new first-layer units (K = o), the new
-// second-layer units (K = kpref[o]) and logit o (K = kpref[o]) as
-// v_mfma_f32_16x16x16_bf16 sweeps over K in ascending blocks of 16 with the
-// epilogue of k_gemm (alpha 1, + bias, ReLU, BF16 round-to-nearest-even), so
-// every value equals the unfused sequence of rgs_gemm_bf16 calls bit for bit;
-// then the bit x_o = [u < sigmoid(l)] and log q += log sigmoid(+-l) in FP64 as
-// k_sample_step does.  Outputs: the bits packed as uint64 words [B][2] (qubit
-// q = bit q % 64 of word q / 64), optionally as bf16 0/1 [B][n], and log q.
-constexpr int kSampRows = 64;   // proposals per workgroup (16 per wavefront)
-constexpr int kSampXS = 72;     // LDS row stride of the bits (n <= 64, padded)
-
 // Four consecutive bf16 of an 8-byte aligned LDS or global address.
 __device__ __forceinline__ bf16x4 load4(const uint16_t* p) { return *reinterpret_cast<const bf16x4*>(p); }
 
-__global__ void __launch_bounds__(256) k_made_sample(const uint16_t* __restrict__ W1, const float* __restrict__ b1,
-                                                     const uint16_t* __restrict__ W2, const float* __restrict__ b2,
-                                                     const uint16_t* __restrict__ W3, const float* __restrict__ b3,
-                                                     const int* __restrict__ kpref, const double* __restrict__ u,
-                                                     int B, int n, int H, uint64_t* __restrict__ words,
-                                                     uint16_t* __restrict__ xout, double* __restrict__ logq) {
-  extern __shared__ __align__(16) uint16_t sm[];
-  const int HS = H + 8;                               // row stride: 2H + 16 bytes, a multiple of 8
-  uint16_t* h1 = sm;                                  // [64][HS]
-  uint16_t* h2 = h1 + kSampRows * HS;                 // [64][HS]
-  uint16_t* w2s = h2 + kSampRows * HS;                // [16][HS]
-  uint16_t* xs = w2s + 16 * HS;                       // [64][kSampXS]
-  const int tid = threadIdx.x;
-  const int wave = tid >> 6, lane = tid & 63;
-  const int rw = wave * 16;                           // first LDS row of this wavefront
-  const int row0 = blockIdx.x * kSampRows;
-  const int c = lane & 15, kq = 4 * (lane >> 4);
-  // Everything starts at zero: activations and staged weights beyond the
-  // current prefix K stay zero, and masked MADE weights are exactly zero, so
-  // the K loops below may run to the end of their 16-wide block without
-  // masking and still add exact zeros, as the zero-padded GEMM tiles do.
-  {
-    const int words16 = (2 * kSampRows * HS + 16 * HS + kSampRows * kSampXS) / 8;   // 16-byte units
-    uint4* z = reinterpret_cast<uint4*>(sm);
-    for (int i = tid; i < words16; i += 256) z[i] = make_uint4(0u, 0u, 0u, 0u);
-  }
-  double lq[4] = {0.0, 0.0, 0.0, 0.0};
-  __syncthreads();
-  for (int o = 0; o < n; ++o) {
-    const int k0 = o ? kpref[o - 1] : 0, k1 = kpref[o];
-    const int dk = k1 - k0;
-    if (dk > 0) {
-      // stage the new rows of W2 (whole rows: columns >= k1 are masked zeros)
-      for (int rr = 0; rr < 16; ++rr)
-        for (int cc = 4 * tid; cc < H; cc += 4 * 256) {
-          bf16x4 v = {0, 0, 0, 0};
-          if (rr < dk) v = load4(W2 + (size_t)(k0 + rr) * H + cc);
-          *reinterpret_cast<bf16x4*>(w2s + rr * HS + cc) = v;
-        }
-      // new first-layer units: [16 rows] x [K = o] x [16 units]
-      f32x4 acc = {0.f, 0.f, 0.f, 0.f};
-      for (int kb = 0; kb < o; kb += 16) {
-        const bf16x4 a = load4(xs + (rw + c) * kSampXS + kb + kq);   // bits >= o are still zero
-        bf16x4 b;
-#pragma unroll
-        for (int i = 0; i < 4; ++i) {
-          const int k = kb + kq + i;
-          b[i] = (short)((c < dk && k < o) ? W1[(size_t)(k0 + c) * n + k] : 0);
-        }
-        acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc, 0, 0, 0);
-      }
-      if (c < dk) {
-#pragma unroll
-        for (int t = 0; t < 4; ++t) {
-          float v = 1.0f * acc[t] + b1[k0 + c];
-          h1[(rw + kq + t) * HS + k0 + c] = f32_to_bf16(fmaxf(v, 0.f));
-        }
-      }
-      __syncthreads();
-      // new second-layer units: [16 rows] x [K = k1] x [16 units]
-      acc = (f32x4){0.f, 0.f, 0.f, 0.f};
-      for (int kb = 0; kb < k1; kb += 16) {
-        const bf16x4 a = load4(h1 + (rw + c) * HS + kb + kq);
-        const bf16x4 b = load4(w2s + c * HS + kb + kq);
-        acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc, 0, 0, 0);
-      }
-      if (c < dk) {
-#pragma unroll
-        for (int t = 0; t < 4; ++t) {
-          float v = 1.0f * acc[t] + b2[k0 + c];
-          h2[(rw + kq + t) * HS + k0 + c] = f32_to_bf16(fmaxf(v, 0.f));
-        }
-      }
-      __syncthreads();
-    }
-    // logit o: [16 rows] x [K = k1] x [1 column] (other columns zero)
-    f32x4 acc = {0.f, 0.f, 0.f, 0.f};
-    for (int kb = 0; kb < k1; kb += 16) {
-      const bf16x4 a = load4(h2 + (rw + c) * HS + kb + kq);
-      bf16x4 b = {0, 0, 0, 0};
-      if (c == 0) b = load4(W3 + (size_t)o * H + kb + kq);       // units of degree > o are masked zeros
-      acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc, 0, 0, 0);
-    }
-    if (c == 0) {
-#pragma unroll
-      for (int t = 0; t < 4; ++t) {
-        const int r = row0 + rw + kq + t;
-        if (r < B) {
-          double term;
-          const int one = made_bit(1.0f * acc[t] + b3[o], u[(size_t)r * n + o], &term);
-          lq[t] += term;
-          xs[(rw + kq + t) * kSampXS + o] = one ? (uint16_t)0x3f80 : (uint16_t)0;
-        }
-      }
-    }
-    __syncthreads();
-  }
-  if (c == 0) {
-#pragma unroll
-    for (int t = 0; t < 4; ++t) {
-      const int lr = rw + kq + t, r = row0 + lr;
-      if (r >= B) continue;
-      uint64_t w0 = 0, w1 = 0;
-      for (int q = 0; q < n; ++q) {
-        const uint64_t bit = xs[lr * kSampXS + q] != 0;
-        if (q < 64) w0 |= bit << q;
-        else w1 |= bit << (q - 64);
-      }
-      if (words) {
-        words[2 * (size_t)r] = w0;
-        words[2 * (size_t)r + 1] = w1;
-      }
-      if (logq) logq[r] = lq[t];
-      if (xout)
-        for (int q = 0; q < n; ++q) xout[(size_t)r * n + q] = xs[lr * kSampXS + q];
-    }
-  }
-}
-
-// Sampler for the degree-blocked layout: H = G (n - 1) hidden units with G a
-// multiple of 16, so the units of each degree d = 1 .. n-1 are the G/16
-// aligned 16-unit blocks [(d-1) G/16, d G/16) of each hidden layer.  One
+// Sampler for the degree-blocked layout: every degree d owns g16 whole
+// aligned 16-unit blocks [(d-1) g16, d g16) of each hidden layer, either
+// H = 16 g16 (n - 1) or H = 16 k with k < n - 1 (one block for each of the
+// degrees 1 .. k, none above; steps past k add no units).  One
 // wavefront draws 16 proposals with no workgroup barrier.  Products are taken
 // transposed, D[unit][row] = sum_k W[unit][k] v[row][k], so a lane (row
 // r = lane % 16, quarter q = lane / 16) receives units 4q .. 4q+3 of its row:
@@ -434,41 +299,68 @@ __device__ __forceinline__ bf16x4 made_relu_bf16(f32x4 acc, const float* __restr
 
 constexpr int kW2Ahead = 8;   // second-layer fragments of the next step loaded a step ahead
 
+// Weight fragments as the MFMA A operand reads them: lane (r, q) holds
+// W[row block * 16 + r][K block * 16 + 4q + i], i = 0..3.  Raw row-major
+// weights are read in place; the packed copy (rgs_made_sample_pack) stores
+// each 16 x 16 tile as 64 consecutive 8-byte fragments in lane order, so a
+// fragment load is one contiguous 512-byte read whatever H is (row strides of
+// a power-of-two bytes otherwise put a load's 16 rows in the same cache sets).
+// First-layer fragments are zero at inputs k >= the block's degree d (the
+// unit sees inputs < d), as the step sequence's K = d product pads them.
+struct MadeWeights {
+  const uint16_t* W1;   // [H][n]
+  const uint16_t* W2;   // [H][H]
+  const uint16_t* W3;   // [n][H]
+  const bf16x4* P;      // packed copy, or null
+  int n, H, nb, nkb1;   // nb = H / 16 row blocks, nkb1 = ceil(n / 16) input blocks
+
+  __device__ __forceinline__ bf16x4 w1(int blk, int kb, int d, int r, int q, int lane) const {
+    if (P) return P[((size_t)blk * nkb1 + kb) * 64 + lane];
+    const uint16_t* w1r = W1 + (size_t)(blk * 16 + r) * n;
+    const int k = kb * 16 + 4 * q;
+    const bf16x4 a = {(short)(k < d ? w1r[k] : 0), (short)(k + 1 < d ? w1r[k + 1] : 0),
+                      (short)(k + 2 < d ? w1r[k + 2] : 0), (short)(k + 3 < d ? w1r[k + 3] : 0)};
+    return a;
+  }
+  __device__ __forceinline__ bf16x4 w2(int blk, int kb, int r, int q, int lane) const {
+    if (P) return P[((size_t)nb * nkb1 + (size_t)blk * nb + kb) * 64 + lane];
+    return load4(W2 + (size_t)(blk * 16 + r) * H + kb * 16 + 4 * q);
+  }
+  __device__ __forceinline__ bf16x4 w3(int ob, int blk, int r, int q, int lane) const {
+    if (P) return P[((size_t)nb * nkb1 + (size_t)nb * nb + (size_t)ob * nb + blk) * 64 + lane];
+    const bf16x4 zero = {0, 0, 0, 0};
+    return 16 * ob + r < n ? load4(W3 + (size_t)(16 * ob + r) * H + blk * 16 + 4 * q) : zero;
+  }
+};
+
 // Weight fragments of the block that serves step st (one block per degree):
 // first layer over inputs < st, the first kW2Ahead second-layer K blocks, and
 // the logit rows.
-__device__ __forceinline__ void made_prefetch(const uint16_t* __restrict__ W1, const uint16_t* __restrict__ W2,
-                                              const uint16_t* __restrict__ W3, int n, int H, int nob, int st, int r,
-                                              int q, bf16x4 (&w1n)[4], bf16x4 (&w2n)[kW2Ahead], bf16x4 (&w3n)[4]) {
+__device__ __forceinline__ void made_prefetch(const MadeWeights& wt, int nob, int st, int r, int q, int lane,
+                                              bf16x4 (&w1n)[4], bf16x4 (&w2n)[kW2Ahead], bf16x4 (&w3n)[4]) {
   const bf16x4 zero = {0, 0, 0, 0};
   const int blk = st - 1;
-  const uint16_t* w1r = W1 + (size_t)(blk * 16 + r) * n;
 #pragma unroll
-  for (int kb = 0; kb < 4; ++kb) {
-    const int k = kb * 16 + 4 * q;
-    const bf16x4 a = {(short)(k < st ? w1r[k] : 0), (short)(k + 1 < st ? w1r[k + 1] : 0),
-                      (short)(k + 2 < st ? w1r[k + 2] : 0), (short)(k + 3 < st ? w1r[k + 3] : 0)};
-    w1n[kb] = a;
-  }
-  const uint16_t* w2r = W2 + (size_t)(blk * 16 + r) * H + 4 * q;
+  for (int kb = 0; kb < 4; ++kb) w1n[kb] = kb < wt.nkb1 ? wt.w1(blk, kb, st, r, q, lane) : zero;
 #pragma unroll
-  for (int kb = 0; kb < kW2Ahead; ++kb) w2n[kb] = kb < st ? load4(w2r + kb * 16) : zero;
-  const uint16_t* w3r = W3 + (size_t)r * H + blk * 16 + 4 * q;
+  for (int kb = 0; kb < kW2Ahead; ++kb) w2n[kb] = kb < st ? wt.w2(blk, kb, r, q, lane) : zero;
 #pragma unroll
-  for (int ob = 0; ob < 4; ++ob) w3n[ob] = (ob < nob && 16 * ob + r < n) ? load4(w3r + (size_t)16 * ob * H) : zero;
+  for (int ob = 0; ob < 4; ++ob) w3n[ob] = ob < nob ? wt.w3(ob, blk, r, q, lane) : zero;
 }
 
-__global__ void __launch_bounds__(64) k_made_sample16(const uint16_t* __restrict__ W1, const float* __restrict__ b1,
-                                                      const uint16_t* __restrict__ W2, const float* __restrict__ b2,
-                                                      const uint16_t* __restrict__ W3, const float* __restrict__ b3,
-                                                      const double* __restrict__ u, int B, int n, int H,
+__global__ void __launch_bounds__(64) k_made_sample16(MadeWeights wt, const float* __restrict__ b1,
+                                                      const float* __restrict__ b2, const float* __restrict__ b3,
+                                                      const double* __restrict__ u, int B,
                                                       uint64_t* __restrict__ words, uint16_t* __restrict__ xout,
                                                       double* __restrict__ logq) {
+  const int n = wt.n, H = wt.H;
   extern __shared__ __align__(16) unsigned char smem[];
   bf16x4* h1 = reinterpret_cast<bf16x4*>(smem);   // [H / 16 blocks][64 lanes]
   const int lane = threadIdx.x;
   const int r = lane & 15, q = lane >> 4;
-  const int nb = H >> 4, g16 = nb / (n - 1), nob = (n + 15) >> 4;
+  // g16 blocks per degree for degrees 1 .. ndeg (ndeg = n - 1, or fewer when
+  // the width has fewer blocks than degrees: one block each, none above)
+  const int nb = H >> 4, g16 = nb >= n - 1 ? nb / (n - 1) : 1, ndeg = nb / g16, nob = (n + 15) >> 4;
   const bf16x4 zero = {0, 0, 0, 0};
   for (int base = blockIdx.x * 16; base < B; base += gridDim.x * 16) {
     const int row = base + r;
@@ -483,11 +375,11 @@ __global__ void __launch_bounds__(64) k_made_sample16(const uint16_t* __restrict
       // rows) are loaded during the previous step, before its draw, so their
       // L2 latency overlaps the draw's FP64 work.
       bf16x4 w1n[4], w2n[kW2Ahead], w3n[4];
-      if (n > 1) made_prefetch(W1, W2, W3, n, H, nob, 1, r, q, w1n, w2n, w3n);
+      if (ndeg >= 1) made_prefetch(wt, nob, 1, r, q, lane, w1n, w2n, w3n);
       for (int o = 0; o < n; ++o) {
         const double uo = un;
         if (o + 1 < n && live) un = u[(size_t)row * n + o + 1];
-        if (o > 0) {
+        if (o > 0 && o <= ndeg) {
           const int blk = o - 1;
           f32x4 acc = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
@@ -498,18 +390,17 @@ __global__ void __launch_bounds__(64) k_made_sample16(const uint16_t* __restrict
 #pragma unroll
           for (int kb = 0; kb < kW2Ahead; ++kb)
             if (kb < o) acc = mfma16(w2n[kb], h1[kb * 64 + lane], acc);
-          const uint16_t* w2r = W2 + (size_t)(blk * 16 + r) * H + 4 * q;
           for (int kb = kW2Ahead; kb < o; kb += kW2Ahead) {   // later K blocks, kW2Ahead loads in flight
             bf16x4 wa[kW2Ahead];
 #pragma unroll
-            for (int i = 0; i < kW2Ahead; ++i) wa[i] = kb + i < o ? load4(w2r + (kb + i) * 16) : zero;
+            for (int i = 0; i < kW2Ahead; ++i) wa[i] = kb + i < o ? wt.w2(blk, kb + i, r, q, lane) : zero;
 #pragma unroll
             for (int i = 0; i < kW2Ahead; ++i)
               if (kb + i < o) acc = mfma16(wa[i], h1[(kb + i) * 64 + lane], acc);
           }
           made_logits(L, nob, w3n, made_relu_bf16(acc, b2 + blk * 16 + 4 * q));
         }
-        if (o + 1 < n) made_prefetch(W1, W2, W3, n, H, nob, o + 1, r, q, w1n, w2n, w3n);
+        if (o + 1 <= ndeg) made_prefetch(wt, nob, o + 1, r, q, lane, w1n, w2n, w3n);
         made_draw(L, b3, o, q, r, live, uo, xb, lq);
       }
     } else {
@@ -520,41 +411,29 @@ __global__ void __launch_bounds__(64) k_made_sample16(const uint16_t* __restrict
           const int g0 = (o - 1) * g16;
           for (int j = 0; j < g16; ++j) {   // first layer: inputs 0 .. o-1
             const int blk = g0 + j;
-            const uint16_t* w1r = W1 + (size_t)(blk * 16 + r) * n;
             f32x4 acc = {0.f, 0.f, 0.f, 0.f};
-            for (int kb = 0; kb * 16 < o; ++kb) {
-              bf16x4 a;
-#pragma unroll
-              for (int i = 0; i < 4; ++i) {
-                const int k = kb * 16 + 4 * q + i;
-                a[i] = (short)(k < o ? w1r[k] : 0);
-              }
-              acc = mfma16(a, made_xfrag(xb, kb, q), acc);
-            }
+            for (int kb = 0; kb * 16 < o; ++kb) acc = mfma16(wt.w1(blk, kb, o, r, q, lane), made_xfrag(xb, kb, q), acc);
             h1[blk * 64 + lane] = made_relu_bf16(acc, b1 + blk * 16 + 4 * q);
           }
           const int nkb = g0 + g16;   // first-layer blocks of degree <= o
           for (int j = 0; j < g16; ++j) {   // second layer, then its logit contributions
             const int blk = g0 + j;
-            const uint16_t* w2r = W2 + (size_t)(blk * 16 + r) * H + 4 * q;
             f32x4 acc = {0.f, 0.f, 0.f, 0.f};
             int kb = 0;
             for (; kb + 8 <= nkb; kb += 8) {   // eight fragment loads in flight, then the MFMAs in K order
               bf16x4 wa[8], hb[8];
 #pragma unroll
               for (int i = 0; i < 8; ++i) {
-                wa[i] = load4(w2r + (kb + i) * 16);
+                wa[i] = wt.w2(blk, kb + i, r, q, lane);
                 hb[i] = h1[(kb + i) * 64 + lane];
               }
 #pragma unroll
               for (int i = 0; i < 8; ++i) acc = mfma16(wa[i], hb[i], acc);
             }
-            for (; kb < nkb; ++kb) acc = mfma16(load4(w2r + kb * 16), h1[kb * 64 + lane], acc);
-            const uint16_t* w3r = W3 + (size_t)r * H + blk * 16 + 4 * q;
+            for (; kb < nkb; ++kb) acc = mfma16(wt.w2(blk, kb, r, q, lane), h1[kb * 64 + lane], acc);
             bf16x4 w3[4];
 #pragma unroll
-            for (int ob = 0; ob < 4; ++ob)
-              w3[ob] = (ob < nob && 16 * ob + r < n) ? load4(w3r + (size_t)16 * ob * H) : zero;
+            for (int ob = 0; ob < 4; ++ob) w3[ob] = ob < nob ? wt.w3(ob, blk, r, q, lane) : zero;
             made_logits(L, nob, w3, made_relu_bf16(acc, b2 + blk * 16 + 4 * q));
           }
         }
@@ -578,6 +457,64 @@ __global__ void __launch_bounds__(64) k_made_sample16(const uint16_t* __restrict
 __global__ void k_f32_to_bf16(const float* in, uint16_t* out, long long count) {
   long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (i < count) out[i] = f32_to_bf16(in[i]);
+}
+
+// Fragment-major copy of the sampler's weights (see MadeWeights): first
+// layer [nb][nkb1] tiles (zero at inputs >= the block's degree), second layer
+// [nb][nb] tiles, logit rows [nob][nb] tiles (zero rows past n); each tile is
+// 64 lanes x 4 bf16.
+__global__ void k_made_pack(const uint16_t* __restrict__ W1, const uint16_t* __restrict__ W2,
+                            const uint16_t* __restrict__ W3, int n, int H, int nb, int nkb1, int g16,
+                            bf16x4* __restrict__ out, long long fragments) {
+  const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= fragments * 64) return;
+  const long long f = t >> 6;
+  const int lane = (int)(t & 63), r = lane & 15, q = lane >> 4;
+  bf16x4 v = {0, 0, 0, 0};
+  if (f < (long long)nb * nkb1) {
+    const int blk = (int)(f / nkb1), kb = (int)(f % nkb1);
+    const int d = blk / g16 + 1;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      const int k = kb * 16 + 4 * q + i;
+      v[i] = (short)(k < d && k < n ? W1[(size_t)(blk * 16 + r) * n + k] : 0);
+    }
+  } else if (f < (long long)nb * nkb1 + (long long)nb * nb) {
+    const long long g = f - (long long)nb * nkb1;
+    const int blk = (int)(g / nb), kb = (int)(g % nb);
+    v = load4(W2 + (size_t)(blk * 16 + r) * H + kb * 16 + 4 * q);
+  } else {
+    const long long h = f - (long long)nb * nkb1 - (long long)nb * nb;
+    const int ob = (int)(h / nb), blk = (int)(h % nb);
+    if (16 * ob + r < n) v = load4(W3 + (size_t)(16 * ob + r) * H + blk * 16 + 4 * q);
+  }
+  out[t] = v;
+}
+
+static bool made_blocked(int n, int H) {
+  return n >= 2 && n <= 64 && H >= 16 && H % 16 == 0 && H <= 2048 && ((H / 16) % (n - 1) == 0 || H / 16 < n - 1);
+}
+
+static long long made_pack_fragments(int n, int H) {
+  const long long nb = H / 16, nkb1 = (n + 15) / 16, nob = (n + 15) / 16;
+  return nb * nkb1 + nb * nb + nob * nb;
+}
+
+static int launch_made_sample16(const MadeWeights& wt, const float* b1, const float* b2, const float* b3,
+                                const double* u, int B, uint64_t* words, uint16_t* xout, double* logq,
+                                hipStream_t stream) {
+  const size_t lds = (size_t)wt.nb * 64 * 8;
+  const int groups = (B + 15) / 16;
+  // one round of the workgroups that fit a CU at once (registers and LDS),
+  // so every wave takes an even share of the 16-row groups
+  int per_cu = 0;
+  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(&k_made_sample16), 64,
+                                                   lds) != hipSuccess || per_cu < 1)
+    per_cu = std::max(1, (int)((160 * 1024) / lds));
+  const int grid = std::min(groups, 256 * per_cu);
+  hipLaunchKernelGGL(k_made_sample16, dim3(grid), dim3(64), lds, stream, wt, b1, b2, b3, u, B, words, xout, logq);
+  RGS_LAUNCH_CHECK("k_made_sample16");
+  return kOk;
 }
 
 }  // namespace rgs
@@ -665,30 +602,38 @@ extern "C" int rgs_made_sample(const uint16_t* W1, const float* b1, const uint16
   if (!W1 || !b1 || !W2 || !b2 || !W3 || !b3 || !kpref || !u || B <= 0 || n < 1 || n > 64 || H < 16 ||
       H % 16 != 0 || (!words && !xout))
     return fail(kInvalid, "made_sample: bad arguments (n <= 64, H a multiple of 16)");
-  if (n >= 2 && H % (16 * (n - 1)) == 0 && H <= 2048) {
-    // degree-blocked layout: one wave per 16 proposals, first-layer fragments in LDS
-    const size_t lds = (size_t)(H / 16) * 64 * 8;
-    const int groups = (B + 15) / 16;
-    const int per_cu = std::max(1, (int)((160 * 1024) / lds));
-    const int grid = std::min(groups, 256 * per_cu);
-    hipLaunchKernelGGL(k_made_sample16, dim3(grid), dim3(64), lds, reinterpret_cast<hipStream_t>(stream), W1, b1, W2,
-                       b2, W3, b3, u, B, n, H, words, xout, logq);
-    RGS_LAUNCH_CHECK("k_made_sample16");
-    return kOk;
-  }
-  if (H > 512) return fail(kInvalid, "made_sample: H above 512 needs the degree-blocked layout H = 16 m (n - 1)");
-  const size_t HS = (size_t)H + 8;
-  const size_t lds = (2 * kSampRows * HS + 16 * HS + (size_t)kSampRows * kSampXS) * sizeof(uint16_t);
-  static bool attr = false;
-  if (!attr) {
-    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_made_sample),
-                                       hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024);
-    if (e != hipSuccess) return fail(kHip, std::string("made_sample: ") + hipGetErrorString(e));
-    attr = true;
-  }
-  hipLaunchKernelGGL(k_made_sample, dim3((B + kSampRows - 1) / kSampRows), dim3(256), lds,
-                     reinterpret_cast<hipStream_t>(stream), W1, b1, W2, b2, W3, b3, kpref, u, B, n, H, words, xout,
-                     logq);
-  RGS_LAUNCH_CHECK("k_made_sample");
+  if (!made_blocked(n, H))
+    return fail(kInvalid,
+                "made_sample: the fused sampler needs every degree to own whole aligned 16-unit blocks "
+                "(H = 16 m (n - 1) or H = 16 k with k < n - 1, H <= 2048); use the step sequence for other widths");
+  const MadeWeights wt{W1, W2, W3, nullptr, n, H, H / 16, (n + 15) / 16};
+  return launch_made_sample16(wt, b1, b2, b3, u, B, words, xout, logq, reinterpret_cast<hipStream_t>(stream));
+}
+
+extern "C" int rgs_made_sample_pack(const uint16_t* W1, const uint16_t* W2, const uint16_t* W3, int n, int H,
+                                    void* packed, uint64_t* bytes, void* stream) {
+  if (!bytes) return fail(kInvalid, "made_sample_pack: null size");
+  if (!made_blocked(n, H)) return fail(kInvalid, "made_sample_pack: width without aligned degree blocks");
+  const long long fragments = made_pack_fragments(n, H);
+  *bytes = (uint64_t)fragments * 64 * 8;
+  if (!packed) return kOk;
+  if (!W1 || !W2 || !W3) return fail(kInvalid, "made_sample_pack: null weights");
+  const int nb = H / 16, g16 = nb >= n - 1 ? nb / (n - 1) : 1;
+  const long long threads = fragments * 64;
+  hipLaunchKernelGGL(k_made_pack, dim3((unsigned)((threads + 255) / 256)), dim3(256), 0,
+                     reinterpret_cast<hipStream_t>(stream), W1, W2, W3, n, H, nb, (n + 15) / 16, g16,
+                     reinterpret_cast<bf16x4*>(packed), fragments);
+  RGS_LAUNCH_CHECK("k_made_pack");
   return kOk;
+}
+
+extern "C" int rgs_made_sample_packed(const void* packed, const float* b1, const float* b2, const float* b3,
+                                      const double* u, int B, int n, int H, uint64_t* words, uint16_t* xout,
+                                      double* logq, void* stream) {
+  if (!packed || !b1 || !b2 || !b3 || !u || B <= 0 || (!words && !xout))
+    return fail(kInvalid, "made_sample_packed: bad arguments");
+  if (!made_blocked(n, H)) return fail(kInvalid, "made_sample_packed: width without aligned degree blocks");
+  const MadeWeights wt{nullptr, nullptr, nullptr, reinterpret_cast<const bf16x4*>(packed), n, H, H / 16,
+                       (n + 15) / 16};
+  return launch_made_sample16(wt, b1, b2, b3, u, B, words, xout, logq, reinterpret_cast<hipStream_t>(stream));
 }

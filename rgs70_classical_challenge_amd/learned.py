@@ -2,7 +2,7 @@
 
 1. Draw training bitstrings with the exact sampler (full state, device
    qubit-by-qubit draws) of a seeded n-vertex 3-regular instance.
-2. Train a MADE (n -> H -> H -> n with H = 16 (n - 1), BF16 weights, FP32 accumulation) with
+2. Train a MADE (n -> 512 -> 512 -> n, BF16 weights, FP32 accumulation) with
    Adam on them, entirely with this build's kernels.
 3. Run capped rejection with uniform proposals (the reference's rule,
    work/rgs_attack/rejection_sampler.py:19-24) and with the MADE as proposal,
@@ -22,9 +22,9 @@ def index_to_bits(idx, n):
     return ((idx[:, None] >> np.arange(n)) & 1).astype(np.uint8)
 
 
-def run(n=34, seed=20261205, train_samples=200_000, steps=20_000, batch=512, hidden=None, cap=8.0,
+def run(n=34, seed=20261205, train_samples=200_000, steps=20_000, batch=512, hidden=512, cap=8.0,
         caps_learned=(1.0, 1.2, 1.4, 1.5, 1.6, 1.7, 1.8, 1.9, 2.0, 2.2, 2.5, 4.0, 8.0), accept_target=300_000,
-        proposal_batch=65536, lr=2e-3, order_seconds=5.0, analysis_resamples=500, log=print):
+        proposal_batch=65536, lr=1e-3, order_seconds=5.0, analysis_resamples=500, log=print):
     import torch
     from .engine import GraphStateSampler
     from .records import capped_statistics
@@ -33,7 +33,6 @@ def run(n=34, seed=20261205, train_samples=200_000, steps=20_000, batch=512, hid
     from .rejection import DeviceCappedRejection
     from .scan import ScanOracle, order_search
 
-    hidden = hidden or 16 * (n - 1)   # one 16-unit block per degree: the one-wave sampler
     out = {'n': n, 'seed': seed, 'cap': cap, 'train_samples': train_samples, 'steps': steps, 'batch': batch,
            'hidden': hidden}
     g, u = instance(n, seed)

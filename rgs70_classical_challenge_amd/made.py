@@ -1,8 +1,8 @@
 """Learned proposal: a MADE network trained and sampled with this build's kernels.
 
 Architecture (BF16 weights and activations, FP32 accumulation, sigmoid
-conditionals): n binary inputs -> H -> H -> n logits (H = 16 (n - 1) in the
-learned-proposal runs, so that each degree owns one 16-unit block), masked so that
+conditionals): n binary inputs -> H -> H -> n logits (H = 512 in the
+learned-proposal runs: 32 degrees of one aligned 16-unit block each), masked so that
 logit o depends only on inputs before o in qubit order (MADE, Germain et al.,
 arXiv:1502.03509).  Training is Adam on the mean binary cross-entropy of
 bitstrings drawn by the exact sampler; every GEMM, the loss, the bias
@@ -31,18 +31,32 @@ _lib.register('rgs_sample_col', _I, [_V, _V, _I, _I, _I, _V, _V])
 _lib.register('rgs_f32_to_bf16', _I, [_V, _V, ctypes.c_longlong, _V])
 _lib.register('rgs_made_sample_step', _I, [_V, _I, _V, _I, _I, _I, _V, _V, _I, _V])
 _lib.register('rgs_made_sample', _I, [_V, _V, _V, _V, _V, _V, _V, _V, _I, _I, _I, _V, _V, _V, _V])
+_lib.register('rgs_made_sample_pack', _I, [_V, _V, _V, _I, _I, _V, ctypes.POINTER(ctypes.c_uint64), _V])
+_lib.register('rgs_made_sample_packed', _I, [_V, _V, _V, _V, _V, _I, _I, _I, _V, _V, _V, _V])
 
 
 def hidden_degrees(n, hidden):
-    """Sorted hidden-unit degrees: unit i has degree floor(i (n-1) / hidden) + 1,
-    so the units of degree <= d form a prefix (incremental sampling relies on
-    it)."""
-    return (np.arange(hidden) * max(1, n - 1)) // hidden + 1
+    """Sorted hidden-unit degrees in 1 .. n-1, so the units of degree <= d form
+    a prefix (incremental sampling relies on it).  When the width allows,
+    every degree owns whole aligned 16-unit blocks (the one-wave sampler
+    needs it): with hidden = 16 m (n-1), unit i has degree i // (16 m) + 1;
+    with hidden = 16 k and k < n-1 blocks, unit i has degree i // 16 + 1
+    (degrees above k get no units; output n-1 still sees every unit).
+    Otherwise unit i has degree floor(i (n-1) / hidden) + 1."""
+    i = np.arange(hidden)
+    d = max(1, n - 1)
+    if hidden % 16 == 0:
+        blocks = hidden // 16
+        if blocks % d == 0:
+            return i // (16 * (blocks // d)) + 1
+        if blocks < d:
+            return i // 16 + 1
+    return (i * d) // hidden + 1
 
 
 def made_masks(n, hidden, seed=0):
-    """Degrees: inputs 1..n; hidden degrees 1..n-1, evenly and in sorted order
-    (``seed`` is kept for the signature; the assignment is deterministic).
+    """Degrees: inputs 1..n; hidden degrees from hidden_degrees, in sorted
+    order (``seed`` is kept for the signature; the assignment is deterministic).
     Returns float masks [H,n], [H,H], [n,H]."""
     deg_in = np.arange(1, n + 1)
     d1 = hidden_degrees(n, hidden)
@@ -159,6 +173,7 @@ class MADE:
     def adam(self):
         call('rgs_adam', ptr(self.w), ptr(self.g), ptr(self.mask), ptr(self.m), ptr(self.v), ptr(self.wb), self.P,
              ptr(self.t), self.lr, 0.9, 0.999, 1e-8, self.stream.cuda_stream)
+        self._packed_stale = True
 
     def train_step(self):
         self.forward()
@@ -223,13 +238,31 @@ class MADE:
     def sample_fused(self, b, bits=False):
         """Draw b proposals from the uniforms in sb['u'][:b] with the fused
         sampling kernel in one launch, with the same arithmetic as the
-        incremental GEMM sequence: for widths H = 16 m (n - 1) (every degree
-        owns m aligned 16-unit blocks) one wavefront per 16 proposals with no
-        workgroup barrier, otherwise 64 proposals per workgroup with both
-        layers' activations in LDS.
-        Returns device views (packed words int64 [b, 2] in the scan's layout,
-        log q float64 [b]) and, with bits=True, also writes the bf16 0/1 bits
-        to sb['x'][:b]."""
+        incremental GEMM sequence, for widths whose degrees own whole aligned
+        16-unit blocks (H = 16 m (n - 1), or H = 16 k with k < n - 1; see
+        hidden_degrees): one wavefront per 16 proposals, no workgroup
+        barrier.  The weights are read from a fragment-major copy, made again
+        after any training step.  Returns device views (packed words int64
+        [b, 2] in the scan's layout, log q float64 [b]) and, with bits=True,
+        also writes the bf16 0/1 bits to sb['x'][:b]."""
+        sb = self._sbuf
+        wb, w = self.wb, self.w
+        s = self.stream.cuda_stream
+        if getattr(self, '_packed', None) is None or getattr(self, '_packed_stale', True):
+            size = ctypes.c_uint64()
+            call('rgs_made_sample_pack', None, None, None, self.n, self.H, None, ctypes.byref(size), s)
+            if getattr(self, '_packed', None) is None:
+                self._packed = self.torch.empty(int(size.value), dtype=self.torch.uint8, device=self.dev)
+            call('rgs_made_sample_pack', self._p(wb, 0), self._p(wb, 2), self._p(wb, 4), self.n, self.H,
+                 ptr(self._packed), ctypes.byref(size), s)
+            self._packed_stale = False
+        call('rgs_made_sample_packed', ptr(self._packed), self._p(w, 1), self._p(w, 3), self._p(w, 5), ptr(sb['u']),
+             b, self.n, self.H, ptr(sb['words']), ptr(sb['x']) if bits else None, ptr(sb['logq']), s)
+        return sb['words'][:b], sb['logq'][:b]
+
+    def sample_fused_raw(self, b, bits=False):
+        """sample_fused reading the row-major weights in place
+        (rgs_made_sample); the same results."""
         sb = self._sbuf
         if not hasattr(self, '_kpref'):
             k = np.searchsorted(hidden_degrees(self.n, self.H), np.arange(self.n), side='right').astype(np.int32)

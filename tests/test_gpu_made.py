@@ -163,8 +163,9 @@ def test_incremental_sampler_equals_full_forward():
 def test_fused_sampler_equals_incremental_sequence():
     """One-launch sampler against the launch-per-step incremental sampler:
     identical bits and identical log q (same arithmetic, same order).  Widths
-    H = 16 m (n - 1) take the degree-blocked one-wave kernel (m = 1 and 2
-    below), the others the 64-proposal workgroup kernel."""
+    of whole 16-unit blocks per degree (H = 16 m (n - 1), m = 1 and 2, or
+    H = 16 k with k < n - 1) take the one-wave kernel; a width without such
+    blocks (H = 160 at n = 9: 20 units per degree) is refused."""
     from rgs70_classical_challenge_amd.made import MADE
     from rgs70_classical_challenge_amd.rejection import pack_words_device
     for n, H, count in ((20, 128, 3000), (34, 512, 1000), (9, 64, 77),
@@ -185,3 +186,12 @@ def test_fused_sampler_equals_incremental_sequence():
         assert torch.equal(words, words_inc)
         assert torch.equal(sb['x'][:count], x_inc)
         assert torch.equal(lq, lq_inc)
+        # the same draws from the row-major weights read in place
+        words_raw, lq_raw = net.sample_fused_raw(count)
+        assert torch.equal(words_raw, words_inc) and torch.equal(lq_raw, lq_inc)
+    net = MADE(9, hidden=160, batch=64, seed=5)
+    net._sampling_buffers(64)
+    with pytest.raises(ValueError, match='aligned'):
+        net.sample_fused(64)
+    with pytest.raises(ValueError, match='aligned 16-unit blocks'):
+        net.sample_fused_raw(64)

@@ -45,9 +45,29 @@ def test_gradients_by_finite_differences():
 
 def test_product_masks_equal_oracle():
     from rgs70_classical_challenge_amd.made import made_masks
-    for n, H in ((8, 32), (34, 512), (70, 512)):
+    for n, H in ((8, 32), (34, 512), (70, 512), (34, 528), (9, 160), (20, 100)):
         for a, b in zip(made_masks(n, H, 5), om.made_masks(n, H, 5)):
             assert np.array_equal(a.astype(np.float64), b)
+
+
+def test_masks_are_autoregressive_and_block_aligned():
+    """Output o reaches (through the two masked layers) exactly the inputs
+    i < min(o, k), k the highest hidden degree: autoregressive, and complete
+    when every degree 1 .. n-1 has units.  Widths of whole 16-unit blocks give
+    each degree whole aligned blocks."""
+    for n, H in ((8, 32), (9, 128), (9, 160), (12, 352), (20, 304), (34, 512), (34, 528), (20, 100)):
+        m1, m2, m3 = om.made_masks(n, H)
+        reach = (m3 @ m2 @ m1) > 0                     # [n outputs, n inputs]
+        d = om.hidden_degrees(n, H)
+        o, i = np.indices((n, n))
+        assert np.array_equal(reach, i < np.minimum(o, d.max())), (n, H)
+        assert np.all(np.diff(d) >= 0) and d.min() >= 1 and d.max() <= n - 1
+        if H % 16 == 0 and ((H // 16) % (n - 1) == 0 or H // 16 < n - 1):
+            starts = np.flatnonzero(np.diff(np.concatenate([[0], d])) != 0)
+            assert np.all(starts % 16 == 0), (n, H)   # every degree group starts on a block
+    # 512 units over 33 degrees: degrees 1..32, sixteen units each
+    d = om.hidden_degrees(34, 512)
+    assert d.max() == 32 and np.all(np.bincount(d)[1:] == 16)
 
 
 def test_bf16_round_known_values():
