@@ -507,10 +507,17 @@ static int launch_made_sample16(const MadeWeights& wt, const float* b1, const fl
   const int groups = (B + 15) / 16;
   // one round of the workgroups that fit a CU at once (registers and LDS),
   // so every wave takes an even share of the 16-row groups
-  int per_cu = 0;
-  if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(&k_made_sample16), 64,
-                                                   lds) != hipSuccess || per_cu < 1)
-    per_cu = std::max(1, (int)((160 * 1024) / lds));
+  static size_t cached_lds = 0;
+  static int cached_per_cu = 0;
+  if (lds != cached_lds) {
+    int per_cu = 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(&k_made_sample16), 64,
+                                                     lds) != hipSuccess || per_cu < 1)
+      per_cu = std::max(1, (int)((160 * 1024) / lds));
+    cached_lds = lds;
+    cached_per_cu = per_cu;
+  }
+  const int per_cu = cached_per_cu;
   const int grid = std::min(groups, 256 * per_cu);
   hipLaunchKernelGGL(k_made_sample16, dim3(grid), dim3(64), lds, stream, wt, b1, b2, b3, u, B, words, xout, logq);
   RGS_LAUNCH_CHECK("k_made_sample16");

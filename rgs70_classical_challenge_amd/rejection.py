@@ -155,7 +155,7 @@ class DeviceCappedRejection:
         self.scorer, self.auditor, self.n, self.cap = scorer, auditor, n, cap
         self.proposal = proposal
 
-    def run(self, seed, accept_target, batch=65536, max_batches=10000):
+    def run(self, seed, accept_target, batch=65536, max_batches=10000, max_ahead=64):
         import torch
         n, cap = self.n, self.cap
         dev = self.scorer.device
@@ -171,8 +171,9 @@ class DeviceCappedRejection:
         torch.cuda.synchronize(dev)
         t0 = time.perf_counter()
         # The host reads the accepted count after the first batch, then
-        # enqueues as many batches as that rate says the target needs before
-        # reading again, so the device is not left idle between batches.
+        # enqueues as many batches as that rate says the target needs (at most
+        # max_ahead) before reading again, so the device is not left idle
+        # between batches.
         ahead = 1
         while batches < max_batches:
             counts = []
@@ -202,7 +203,7 @@ class DeviceCappedRejection:
             if accepted >= accept_target:
                 break
             rate = max(accepted / proposals, 1e-9)
-            ahead = max(1, math.ceil((accept_target - accepted) / (rate * batch)))
+            ahead = min(max_ahead, max(1, math.ceil((accept_target - accepted) / (rate * batch))))
         torch.cuda.synchronize(dev)
         seconds = time.perf_counter() - t0
         # audit: complex128 tables for every proposal, decisions recomputed in FP64
