@@ -78,8 +78,9 @@ struct ScanStepArgs {
 // order g(kk) = kk ^ (kk >> 1), block k = wave + W g(kk), so that each block's
 // p0 = lo ^ hi[k] differs from the previous one by a single uniform word
 // hstep[ctz(kk)] = hi[W 2^ctz(kk)]; lo_w = lo ^ hi[wave] is per lane.  Blocks
-// kk < nkw are live (nkw uniform).  Term counts 2^NKER are compile-time
-// constants, so the table reads of all outputs are straight-line loads.
+// kk < nkw are live (nkw uniform).  Term counts 2^NKER (NKER <= 2 in every
+// packed plan) are compile-time constants, so the table reads of all outputs
+// are straight-line loads.
 // Split steps: every p0 has y = 0; the first half of the combinations keeps
 // y = 0 (weight f0, no sign), the second has y = 1 (weight f1, sign
 // (-1)^(c . beta)).  Plain steps: every term of an output has y = y(p0),
@@ -141,42 +142,6 @@ __device__ __forceinline__ uint32_t scan_outputs(const S* E, const ScanStepArgs&
   return mxh;
 }
 
-// Any kernel size, with run-time loops over the terms (steps with nker > 2).
-template <typename S, int K>
-__device__ __forceinline__ uint32_t scan_outputs_any(const S* E, const ScanStepArgs& a, bool split, uint32_t lo_w,
-                                                     const uint32_t* hstep, int nkw, S (&outv)[K]) {
-  const int ncomb = 1 << a.nker, half = split ? ncomb >> 1 : ncomb;
-  uint32_t mxh = 0, hs = 0;
-#pragma unroll
-  for (int kk = 0; kk < K; ++kk) {
-    if (kk) hs ^= hstep[__builtin_ctz(kk)];
-    if (kk < nkw) {
-      const uint32_t p0 = lo_w ^ hs;
-      const uint32_t y = (p0 >> a.r_in) & 1u;   // 0 on split steps
-      const uint32_t bm = y ? a.beta : 0u;
-      double s0r = 0.0, s0i = 0.0, s1r = 0.0, s1i = 0.0;
-      for (int t = 0; t < half; ++t) {
-        const uint32_t c = (p0 ^ (uint32_t)a.combos[t]) & a.mask;
-        const double2 e = load_signed(E[c], (uint32_t)__popc(c & bm) << 31);
-        s0r += e.x;
-        s0i += e.y;
-      }
-      for (int t = half; t < ncomb; ++t) {
-        const uint32_t c = (p0 ^ (uint32_t)a.combos[t]) & a.mask;
-        const double2 e = load_signed(E[c], (uint32_t)__popc(c & a.beta) << 31);
-        s1r += e.x;
-        s1i += e.y;
-      }
-      const double2 f = y ? a.f1 : a.f0;
-      const double orr = (s0r * f.x - s0i * f.y) + (s1r * a.f1.x - s1i * a.f1.y);
-      const double oi = (s0r * f.y + s0i * f.x) + (s1r * a.f1.y + s1i * a.f1.x);
-      mxh = max(mxh, sq_hi(orr, oi));
-      outv[kk] = make_s<S>(orr, oi);
-    }
-  }
-  return mxh;
-}
-
 // Wave w of the workgroup computes the output blocks k = w + W g of a step
 // (at most K of them per lane).  Output c' sums E[c] over the kernel
 // combinations t of p = p0(c') ^ combos[t] = (c, y), each term weighted by
@@ -228,19 +193,14 @@ __global__ void __launch_bounds__(64 * W) k_scan_batch(const uint64_t* __restric
       for (int b = 0; b < KB; ++b) hstep[b] = (W << b) < nk ? (uint32_t)st[66 + (W << b)] : 0u;
       S outv[K];
       uint32_t mxh;
+      // the packed plan has nker <= 2, and split steps nker >= 1
       if (split) {
-        switch (a.nker) {
-          case 1: mxh = scan_outputs<S, K, 1, true>(E, a, lo_w, hstep, nkw, outv); break;
-          case 2: mxh = scan_outputs<S, K, 2, true>(E, a, lo_w, hstep, nkw, outv); break;
-          default: mxh = scan_outputs_any<S, K>(E, a, true, lo_w, hstep, nkw, outv); break;
-        }
+        mxh = a.nker == 1 ? scan_outputs<S, K, 1, true>(E, a, lo_w, hstep, nkw, outv)
+                          : scan_outputs<S, K, 2, true>(E, a, lo_w, hstep, nkw, outv);
       } else {
-        switch (a.nker) {
-          case 0: mxh = scan_outputs<S, K, 0, false>(E, a, lo_w, hstep, nkw, outv); break;
-          case 1: mxh = scan_outputs<S, K, 1, false>(E, a, lo_w, hstep, nkw, outv); break;
-          case 2: mxh = scan_outputs<S, K, 2, false>(E, a, lo_w, hstep, nkw, outv); break;
-          default: mxh = scan_outputs_any<S, K>(E, a, false, lo_w, hstep, nkw, outv); break;
-        }
+        mxh = a.nker == 0   ? scan_outputs<S, K, 0, false>(E, a, lo_w, hstep, nkw, outv)
+              : a.nker == 1 ? scan_outputs<S, K, 1, false>(E, a, lo_w, hstep, nkw, outv)
+                            : scan_outputs<S, K, 2, false>(E, a, lo_w, hstep, nkw, outv);
       }
       // lanes past 2^r_out (steps of rank below six) computed in-range values
       // that are neither counted nor stored

@@ -280,7 +280,11 @@ extern "C" int rgs_scan_pack(const rgs_scan_step* steps, int n, uint64_t* blob, 
   int rmax = 0;
   for (int j = 0; j < n; ++j) {
     const rgs_scan_step& st = steps[j];
-    if (st.r_out > 28 || st.nker > 16) return fail(kInvalid, "scan_pack: rank above 28 or kernel above 16");
+    // A step clears one coordinate of r_in basis vectors (rank drops by at
+    // most one) and adds one vector, so r_out >= r_in - 1 and the kernel of
+    // p = (c, y) -> c' has dimension r_in + 1 - r_out <= 2; the batch kernel
+    // has straight-line paths for nker = 0, 1, 2 only.
+    if (st.r_out > 28 || st.nker > 2) return fail(kInvalid, "scan_pack: rank above 28 or kernel dimension above 2");
     rmax = std::max(rmax, st.r_out);
     uint64_t hi = 1ull << std::max(0, st.r_out - 6);
     uint64_t off = w;

@@ -198,3 +198,24 @@ def test_packed_plan_halves_and_sum():
             bits = rng.integers(0, 2, n)
             if n <= 14:
                 assert _packed_amplitude(blob, u, bits) == pytest.approx(osc.amplitude(steps, u, bits), abs=1e-13)
+
+
+def test_kernel_dimension_at_most_two(golden_dir):
+    """Each step clears one coordinate of the r_in basis vectors (rank drops
+    by at most one) and adds one vector, so r_out >= r_in - 1 and the kernel
+    of p = (c, y) -> c' has dimension r_in + 1 - r_out <= 2; the packer
+    relies on it.  Checked on dense and sparse random graphs in random
+    orders, and on the RGS70 graph with its optimised order."""
+    d = np.load(f'{golden_dir}/rgs_graph.npz')
+    plan = scan.ScanPlan(d['graph'], np.load(f'{golden_dir}/scan_order_optimized.npy').tolist())
+    assert max(st.nker for st in plan.steps) <= 2
+    rng = np.random.default_rng(21)
+    seen = set()
+    for _ in range(120):
+        n = int(rng.integers(8, 19))
+        a = np.triu(rng.random((n, n)) < rng.uniform(0.1, 0.95), 1).astype(np.uint8)
+        steps, prof, _ = osc.compile_plan(a | a.T, rng.permutation(n).tolist())
+        for st in steps:
+            assert st.r_out >= st.r_in - 1 and len(st.kernel) == st.r_in + 1 - st.r_out <= 2
+            seen.add(len(st.kernel))
+    assert seen == {0, 1, 2}
