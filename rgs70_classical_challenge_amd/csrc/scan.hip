@@ -290,11 +290,17 @@ __global__ void k_scan_init(S* e0, unsigned long long* mx, int n) {
   for (int j = t; j < n; j += blockDim.x) mx[j] = 0ull;
 }
 
+// One step over the 2^r_out outputs.  Output c' sums E[c] over the kernel
+// combinations of p = p0(c') ^ combos[t] = (c, y), weighted by f_y and
+// negated when y = 1 and c . beta is odd.  When d != 0 (the twin of c' is
+// c' ^ d, whose terms are the same entries with y flipped, see twin_of), a
+// thread takes an output and its twin: it reads each entry once and weights
+// it into both, halving the step's reads.
 template <typename S>
 __global__ void __launch_bounds__(256) k_scan_step(const uint64_t* __restrict__ st, const S* __restrict__ ein,
                                                    S* __restrict__ eout, double2 f0, double2 f1,
                                                    const unsigned long long* __restrict__ mx_prev,
-                                                   unsigned long long* __restrict__ mx_out) {
+                                                   unsigned long long* __restrict__ mx_out, uint32_t d) {
   __shared__ uint64_t tab[4 * 256];
   __shared__ double red[4];
   const uint64_t hdr = st[0];
@@ -307,34 +313,77 @@ __global__ void __launch_bounds__(256) k_scan_step(const uint64_t* __restrict__ 
   const int ncomb = 1 << nker;
   const double insc = mx_prev ? inscale_of(mx_prev) : 1.0;
   const uint64_t mask = (1ull << r_in) - 1ull;
-  const uint64_t total = 1ull << r_out;
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
   double lmax = 0.0;
-  for (uint64_t cp = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; cp < total;
-       cp += (uint64_t)gridDim.x * blockDim.x) {
-    uint64_t p0 = 0;
-    for (int b = 0; b < nbytes; ++b) p0 ^= tab[(b << 8) | ((cp >> (8 * b)) & 255u)];
-    double ar = 0.0, ai = 0.0;
-    for (int t = 0; t < ncomb; ++t) {
-      const uint64_t p = p0 ^ combos[t];
-      const uint64_t c = p & mask;
-      const uint32_t y = (uint32_t)(p >> r_in) & 1u;
-      const uint32_t neg = y & ((uint32_t)__popcll(c & beta) & 1u);
-      const double2 w = y ? f1 : f0;
-      const double2 e = load_c(ein[c]);
-      const double pr = e.x * w.x - e.y * w.y;
-      const double pi = e.x * w.y + e.y * w.x;
-      if (neg) {
-        ar -= pr;
-        ai -= pi;
-      } else {
-        ar += pr;
-        ai += pi;
+  if (d == 0) {
+    const uint64_t total = 1ull << r_out;
+    for (uint64_t cp = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; cp < total; cp += stride) {
+      uint64_t p0 = 0;
+      for (int b = 0; b < nbytes; ++b) p0 ^= tab[(b << 8) | ((cp >> (8 * b)) & 255u)];
+      double ar = 0.0, ai = 0.0;
+      for (int t = 0; t < ncomb; ++t) {
+        const uint64_t p = p0 ^ combos[t];
+        const uint64_t c = p & mask;
+        const uint32_t y = (uint32_t)(p >> r_in) & 1u;
+        const uint32_t neg = y & ((uint32_t)__popcll(c & beta) & 1u);
+        const double2 w = y ? f1 : f0;
+        const double2 e = load_c(ein[c]);
+        const double pr = e.x * w.x - e.y * w.y;
+        const double pi = e.x * w.y + e.y * w.x;
+        if (neg) {
+          ar -= pr;
+          ai -= pi;
+        } else {
+          ar += pr;
+          ai += pi;
+        }
       }
+      ar *= insc;
+      ai *= insc;
+      lmax = fmax(lmax, ar * ar + ai * ai);
+      eout[cp] = make_s<S>(ar, ai);
     }
-    ar *= insc;
-    ai *= insc;
-    lmax = fmax(lmax, ar * ar + ai * ai);
-    eout[cp] = make_s<S>(ar, ai);
+  } else {
+    // outputs c' with bit hb of d clear (hb its highest set bit) and their twins
+    const int hb = 31 - __clz(d);
+    const uint64_t low = (1ull << hb) - 1ull, half = 1ull << (r_out - 1);
+    for (uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; i < half; i += stride) {
+      const uint64_t cp = ((i >> hb) << (hb + 1)) | (i & low);
+      uint64_t p0 = 0;
+      for (int b = 0; b < nbytes; ++b) p0 ^= tab[(b << 8) | ((cp >> (8 * b)) & 255u)];
+      double ar = 0.0, ai = 0.0, br = 0.0, bi = 0.0;
+      for (int t = 0; t < ncomb; ++t) {
+        const uint64_t p = p0 ^ combos[t];
+        const uint64_t c = p & mask;
+        const uint32_t y = (uint32_t)(p >> r_in) & 1u;
+        const uint32_t par = (uint32_t)__popcll(c & beta) & 1u;
+        const double2 e = load_c(ein[c]);
+        const double2 wa = y ? f1 : f0, wb = y ? f0 : f1;   // the twin sees the entry with y flipped
+        const double pra = e.x * wa.x - e.y * wa.y, pia = e.x * wa.y + e.y * wa.x;
+        const double prb = e.x * wb.x - e.y * wb.y, pib = e.x * wb.y + e.y * wb.x;
+        if (y & par) {
+          ar -= pra;
+          ai -= pia;
+        } else {
+          ar += pra;
+          ai += pia;
+        }
+        if ((y ^ 1u) & par) {
+          br -= prb;
+          bi -= pib;
+        } else {
+          br += prb;
+          bi += pib;
+        }
+      }
+      ar *= insc;
+      ai *= insc;
+      br *= insc;
+      bi *= insc;
+      lmax = fmax(lmax, fmax(ar * ar + ai * ai, br * br + bi * bi));
+      eout[cp] = make_s<S>(ar, ai);
+      eout[cp ^ d] = make_s<S>(br, bi);
+    }
   }
   for (int off = 32; off > 0; off >>= 1) lmax = fmax(lmax, __shfl_xor(lmax, off));
   if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = lmax;
@@ -363,6 +412,41 @@ __global__ void k_scan_finish(const S* elast, const unsigned long long* mx, int 
   out_exp[0] = esum;
 }
 
+// The twin offset of a step: p0(c' ^ d) = p0(c') ^ e_y ^ k with k in the
+// kernel, so the terms of output c' ^ d are those of c' with y flipped.  The
+// lcols and the kernel basis together form a basis of p-space (r_in + 1
+// vectors), so e_y = L d ^ K z has exactly one solution; d is its L part
+// (0 when e_y lies in the kernel, and then outputs have no twin).
+static uint32_t twin_of(const rgs_scan_step& st) {
+  const int m = st.r_out + st.nker;
+  if (st.r_out < 1 || m > 64) return 0;
+  uint64_t vec[64], comb[64];
+  int piv[64], nb = 0;
+  for (int i = 0; i < m; ++i) {
+    uint64_t v = i < st.r_out ? st.lcols[i] : st.kern[i - st.r_out];
+    uint64_t c = 1ull << i;
+    for (int k = 0; k < nb; ++k)
+      if ((v >> piv[k]) & 1ull) {
+        v ^= vec[k];
+        c ^= comb[k];
+      }
+    if (v) {
+      vec[nb] = v;
+      comb[nb] = c;
+      piv[nb] = 63 - __builtin_clzll(v);
+      ++nb;
+    }
+  }
+  uint64_t t = 1ull << st.r_in, tc = 0;
+  for (int k = 0; k < nb; ++k)
+    if ((t >> piv[k]) & 1ull) {
+      t ^= vec[k];
+      tc ^= comb[k];
+    }
+  if (t) return 0;   // not in the span: leave the step to the one-output path
+  return (uint32_t)(tc & ((st.r_out >= 32 ? 0ull : (1ull << st.r_out)) - 1ull));
+}
+
 template <typename S>
 static int scan_large(const rgs_scan_step* steps, int n, const uint64_t* plan_host_offsets, const uint64_t* plan,
                       const double2* u, const uint64_t* bits, void* buf0, void* buf1,
@@ -375,10 +459,11 @@ static int scan_large(const rgs_scan_step* steps, int n, const uint64_t* plan_ho
     const int v = s.v;
     const int xv = (int)((v < 64 ? (bits[0] >> v) : (bits[1] >> (v - 64))) & 1u);
     const double2 f0 = u[(v * 2 + xv) * 2 + 0], f1 = u[(v * 2 + xv) * 2 + 1];
-    const uint64_t outs = 1ull << s.r_out;
+    const uint32_t d = twin_of(s);
+    const uint64_t outs = d ? 1ull << (s.r_out - 1) : 1ull << s.r_out;   // threads: output pairs or outputs
     const uint64_t blocks = std::min<uint64_t>((outs + 255) / 256, 256ull * 64);
     hipLaunchKernelGGL((k_scan_step<S>), dim3((unsigned)blocks), dim3(256), 0, st, plan + plan_host_offsets[j],
-                       b[j & 1], b[(j + 1) & 1], f0, f1, j ? mx + (j - 1) : nullptr, mx + j);
+                       b[j & 1], b[(j + 1) & 1], f0, f1, j ? mx + (j - 1) : nullptr, mx + j, d);
     RGS_LAUNCH_CHECK("k_scan_step");
   }
   hipLaunchKernelGGL((k_scan_finish<S>), dim3(1), dim3(64), 0, st, b[n & 1], mx, n, out_val, out_exp);

@@ -219,3 +219,68 @@ def test_kernel_dimension_at_most_two(golden_dir):
             assert st.r_out >= st.r_in - 1 and len(st.kernel) == st.r_in + 1 - st.r_out <= 2
             seen.add(len(st.kernel))
     assert seen == {0, 1, 2}
+
+
+def _twin(st):
+    """d with e_y = L d ^ K z (the lcols and kernel vectors span p-space), or 0."""
+    cols = list(st.lcols) + list(st.kernel)
+    basis = []                       # (vector, combination mask), distinct top bits
+    for i, v in enumerate(cols):
+        c = 1 << i
+        for bv, bc in basis:
+            if v ^ bv < v:           # bv's top bit is set in v
+                v, c = v ^ bv, c ^ bc
+        if v:
+            basis.append((v, c))
+            basis.sort(key=lambda e: -e[0])
+    t, tc = 1 << st.r_in, 0
+    for bv, bc in basis:
+        if t ^ bv < t:
+            t, tc = t ^ bv, tc ^ bc
+    return 0 if t else tc & ((1 << st.r_out) - 1)
+
+
+def test_twin_outputs_share_entries():
+    """Output c' ^ d reads the same table entries as output c' with y flipped,
+    so a sweep that produces each output pair from one read of its entries
+    equals the oracle sweep."""
+    rng = np.random.default_rng(8)
+    twins = 0
+    for n in (10, 12, 14):
+        for _ in range(4):
+            a = np.triu(rng.random((n, n)) < rng.uniform(0.2, 0.8), 1).astype(np.uint8)
+            g = a | a.T
+            z = rng.normal(size=(n, 2, 2)) + 1j * rng.normal(size=(n, 2, 2))
+            u = np.stack([np.linalg.qr(m)[0] for m in z])
+            order = rng.permutation(n).tolist()
+            steps, _, _ = osc.compile_plan(g, order)
+            bits = rng.integers(0, 2, n)
+            E = np.ones(1, np.complex128)
+            for st in steps:
+                f = u[st.v, int(bits[st.v])]
+                d = _twin(st)
+                combos = osc.kernel_combos(st.kernel)
+                out = np.zeros(1 << st.r_out, np.complex128)
+                done = np.zeros(1 << st.r_out, bool)
+                for cp in range(1 << st.r_out):
+                    if done[cp]:
+                        continue
+                    p0 = 0
+                    for i in range(st.r_out):
+                        if (cp >> i) & 1:
+                            p0 ^= st.lcols[i]
+                    acc = [0j, 0j]
+                    for t in combos:
+                        p = p0 ^ t
+                        c, y = p & ((1 << st.r_in) - 1), (p >> st.r_in) & 1
+                        par = bin(c & st.beta).count('1') & 1
+                        acc[0] += E[c] * f[y] * (-1 if y and par else 1)
+                        acc[1] += E[c] * f[1 - y] * (-1 if (1 - y) and par else 1)
+                    out[cp], done[cp] = acc[0], True
+                    if d:
+                        out[cp ^ d], done[cp ^ d] = acc[1], True
+                        twins += 1
+                E = out
+            want = osc.amplitude(steps, u, bits)
+            assert complex(E[0]) * 2.0 ** (-n / 2) == pytest.approx(want, abs=1e-12)
+    assert twins > 0
