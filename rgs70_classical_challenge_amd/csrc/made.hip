@@ -314,38 +314,106 @@ struct MadeWeights {
   const bf16x4* P;      // packed copy, or null
   int n, H, nb, nkb1;   // nb = H / 16 row blocks, nkb1 = ceil(n / 16) input blocks
 
+  // kP: the packed copy is known to be present (no raw-layout code emitted)
+  template <bool kP = false>
   __device__ __forceinline__ bf16x4 w1(int blk, int kb, int d, int r, int q, int lane) const {
-    if (P) return P[((size_t)blk * nkb1 + kb) * 64 + lane];
+    if (kP || P) return P[((size_t)blk * nkb1 + kb) * 64 + lane];
     const uint16_t* w1r = W1 + (size_t)(blk * 16 + r) * n;
     const int k = kb * 16 + 4 * q;
     const bf16x4 a = {(short)(k < d ? w1r[k] : 0), (short)(k + 1 < d ? w1r[k + 1] : 0),
                       (short)(k + 2 < d ? w1r[k + 2] : 0), (short)(k + 3 < d ? w1r[k + 3] : 0)};
     return a;
   }
+  template <bool kP = false>
   __device__ __forceinline__ bf16x4 w2(int blk, int kb, int r, int q, int lane) const {
-    if (P) return P[((size_t)nb * nkb1 + (size_t)blk * nb + kb) * 64 + lane];
+    if (kP || P) return P[((size_t)nb * nkb1 + (size_t)blk * nb + kb) * 64 + lane];
     return load4(W2 + (size_t)(blk * 16 + r) * H + kb * 16 + 4 * q);
   }
+  template <bool kP = false>
   __device__ __forceinline__ bf16x4 w3(int ob, int blk, int r, int q, int lane) const {
-    if (P) return P[((size_t)nb * nkb1 + (size_t)nb * nb + (size_t)ob * nb + blk) * 64 + lane];
+    if (kP || P) return P[((size_t)nb * nkb1 + (size_t)nb * nb + (size_t)ob * nb + blk) * 64 + lane];
     const bf16x4 zero = {0, 0, 0, 0};
     return 16 * ob + r < n ? load4(W3 + (size_t)(16 * ob + r) * H + blk * 16 + 4 * q) : zero;
   }
 };
 
 // Weight fragments of the block that serves step st (one block per degree):
-// first layer over inputs < st, the first kW2Ahead second-layer K blocks, and
+// first layer over inputs < st, the first AHEAD second-layer K blocks, and
 // the logit rows.
+template <bool kP, int AHEAD>
 __device__ __forceinline__ void made_prefetch(const MadeWeights& wt, int nob, int st, int r, int q, int lane,
-                                              bf16x4 (&w1n)[4], bf16x4 (&w2n)[kW2Ahead], bf16x4 (&w3n)[4]) {
+                                              bf16x4 (&w1n)[4], bf16x4 (&w2n)[AHEAD], bf16x4 (&w3n)[4]) {
   const bf16x4 zero = {0, 0, 0, 0};
   const int blk = st - 1;
 #pragma unroll
-  for (int kb = 0; kb < 4; ++kb) w1n[kb] = kb < wt.nkb1 ? wt.w1(blk, kb, st, r, q, lane) : zero;
+  for (int kb = 0; kb < 4; ++kb) w1n[kb] = kb < wt.nkb1 ? wt.w1<kP>(blk, kb, st, r, q, lane) : zero;
 #pragma unroll
-  for (int kb = 0; kb < kW2Ahead; ++kb) w2n[kb] = kb < st ? wt.w2(blk, kb, r, q, lane) : zero;
+  for (int kb = 0; kb < AHEAD; ++kb) w2n[kb] = kb < st ? wt.w2<kP>(blk, kb, r, q, lane) : zero;
 #pragma unroll
-  for (int ob = 0; ob < 4; ++ob) w3n[ob] = ob < nob ? wt.w3(ob, blk, r, q, lane) : zero;
+  for (int ob = 0; ob < 4; ++ob) w3n[ob] = ob < nob ? wt.w3<kP>(ob, blk, r, q, lane) : zero;
+}
+
+// The n draws of one 16-row group when every degree owns one block (block
+// s-1 serves step s).  A step's weight fragments (first layer, the first
+// AHEAD second-layer K blocks, the logit rows) are loaded during the previous
+// step, before its draw, so their L2 latency overlaps the draw's FP64 work;
+// later second-layer K blocks are read AHEAD at a time.  The AHEAD grouping
+// changes only when loads are issued: the MFMAs run in ascending K order
+// into one accumulator whatever it is.
+template <bool kP, int AHEAD>
+__device__ __forceinline__ void made_rows_one(const MadeWeights& wt, const float* __restrict__ b1,
+                                              const float* __restrict__ b2, const float* __restrict__ b3,
+                                              const double* __restrict__ u, int row, bool live, int r, int q,
+                                              int lane, int nob, int ndeg, bf16x4* h1, uint64_t& xb, double& lq) {
+  const int n = wt.n;
+  const bf16x4 zero = {0, 0, 0, 0};
+  LogitBlocks L = {{0.f, 0.f, 0.f, 0.f}, {0.f, 0.f, 0.f, 0.f}, {0.f, 0.f, 0.f, 0.f}, {0.f, 0.f, 0.f, 0.f}};
+  double un = live ? u[(size_t)row * n] : 0.0;   // uniform of the next draw, loaded a step ahead
+  bf16x4 w1n[4], w2n[AHEAD], w3n[4];
+  if (ndeg >= 1) made_prefetch<kP, AHEAD>(wt, nob, 1, r, q, lane, w1n, w2n, w3n);
+  for (int o = 0; o < n; ++o) {
+    const double uo = un;
+    if (o + 1 < n && live) un = u[(size_t)row * n + o + 1];
+    if (o > 0 && o <= ndeg) {
+      const int blk = o - 1;
+      f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+      for (int kb = 0; kb < 4; ++kb)
+        if (kb * 16 < o) acc = mfma16(w1n[kb], made_xfrag(xb, kb, q), acc);
+      h1[blk * 64 + lane] = made_relu_bf16(acc, b1 + blk * 16 + 4 * q);
+      acc = (f32x4){0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+      for (int kb = 0; kb < AHEAD; ++kb)
+        if (kb < o) acc = mfma16(w2n[kb], h1[kb * 64 + lane], acc);
+      for (int kb = AHEAD; kb < o; kb += AHEAD) {
+        bf16x4 wa[AHEAD];
+#pragma unroll
+        for (int i = 0; i < AHEAD; ++i) wa[i] = kb + i < o ? wt.w2<kP>(blk, kb + i, r, q, lane) : zero;
+#pragma unroll
+        for (int i = 0; i < AHEAD; ++i)
+          if (kb + i < o) acc = mfma16(wa[i], h1[(kb + i) * 64 + lane], acc);
+      }
+      made_logits(L, nob, w3n, made_relu_bf16(acc, b2 + blk * 16 + 4 * q));
+    }
+    if (o + 1 <= ndeg) made_prefetch<kP, AHEAD>(wt, nob, o + 1, r, q, lane, w1n, w2n, w3n);
+    made_draw(L, b3, o, q, r, live, uo, xb, lq);
+  }
+}
+
+// A row's results: packed words (the scan's two-word layout), log q, and
+// optionally the bf16 0/1 bits.
+__device__ __forceinline__ void made_store(int n, int row, int q, uint64_t xb, double lq,
+                                           uint64_t* __restrict__ words, uint16_t* __restrict__ xout,
+                                           double* __restrict__ logq) {
+  if (q == 0) {
+    if (words) {
+      words[2 * (size_t)row] = xb;
+      words[2 * (size_t)row + 1] = 0;
+    }
+    if (logq) logq[row] = lq;
+  }
+  if (xout)
+    for (int k = q; k < n; k += 4) xout[(size_t)row * n + k] = ((xb >> k) & 1u) ? (uint16_t)0x3f80 : (uint16_t)0;
 }
 
 __global__ void __launch_bounds__(64) k_made_sample16(MadeWeights wt, const float* __restrict__ b1,
@@ -367,43 +435,11 @@ __global__ void __launch_bounds__(64) k_made_sample16(MadeWeights wt, const floa
     const bool live = row < B;
     uint64_t xb = 0;   // bits drawn so far for this lane's row
     double lq = 0.0;
-    LogitBlocks L = {{0.f, 0.f, 0.f, 0.f}, {0.f, 0.f, 0.f, 0.f}, {0.f, 0.f, 0.f, 0.f}, {0.f, 0.f, 0.f, 0.f}};
-    double un = live ? u[(size_t)row * n] : 0.0;   // uniform of the next draw, loaded a step ahead
     if (g16 == 1) {
-      // One block per degree: block s-1 serves step s.  Its weight fragments
-      // (first layer, the first kW2Ahead second-layer K blocks, the logit
-      // rows) are loaded during the previous step, before its draw, so their
-      // L2 latency overlaps the draw's FP64 work.
-      bf16x4 w1n[4], w2n[kW2Ahead], w3n[4];
-      if (ndeg >= 1) made_prefetch(wt, nob, 1, r, q, lane, w1n, w2n, w3n);
-      for (int o = 0; o < n; ++o) {
-        const double uo = un;
-        if (o + 1 < n && live) un = u[(size_t)row * n + o + 1];
-        if (o > 0 && o <= ndeg) {
-          const int blk = o - 1;
-          f32x4 acc = {0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-          for (int kb = 0; kb < 4; ++kb)
-            if (kb * 16 < o) acc = mfma16(w1n[kb], made_xfrag(xb, kb, q), acc);
-          h1[blk * 64 + lane] = made_relu_bf16(acc, b1 + blk * 16 + 4 * q);
-          acc = (f32x4){0.f, 0.f, 0.f, 0.f};
-#pragma unroll
-          for (int kb = 0; kb < kW2Ahead; ++kb)
-            if (kb < o) acc = mfma16(w2n[kb], h1[kb * 64 + lane], acc);
-          for (int kb = kW2Ahead; kb < o; kb += kW2Ahead) {   // later K blocks, kW2Ahead loads in flight
-            bf16x4 wa[kW2Ahead];
-#pragma unroll
-            for (int i = 0; i < kW2Ahead; ++i) wa[i] = kb + i < o ? wt.w2(blk, kb + i, r, q, lane) : zero;
-#pragma unroll
-            for (int i = 0; i < kW2Ahead; ++i)
-              if (kb + i < o) acc = mfma16(wa[i], h1[(kb + i) * 64 + lane], acc);
-          }
-          made_logits(L, nob, w3n, made_relu_bf16(acc, b2 + blk * 16 + 4 * q));
-        }
-        if (o + 1 <= ndeg) made_prefetch(wt, nob, o + 1, r, q, lane, w1n, w2n, w3n);
-        made_draw(L, b3, o, q, r, live, uo, xb, lq);
-      }
+      made_rows_one<false, kW2Ahead>(wt, b1, b2, b3, u, row, live, r, q, lane, nob, ndeg, h1, xb, lq);
     } else {
+      LogitBlocks L = {{0.f, 0.f, 0.f, 0.f}, {0.f, 0.f, 0.f, 0.f}, {0.f, 0.f, 0.f, 0.f}, {0.f, 0.f, 0.f, 0.f}};
+      double un = live ? u[(size_t)row * n] : 0.0;   // uniform of the next draw, loaded a step ahead
       for (int o = 0; o < n; ++o) {
         const double uo = un;
         if (o + 1 < n && live) un = u[(size_t)row * n + o + 1];
@@ -440,17 +476,33 @@ __global__ void __launch_bounds__(64) k_made_sample16(MadeWeights wt, const floa
         made_draw(L, b3, o, q, r, live, uo, xb, lq);
       }
     }
-    if (live) {
-      if (q == 0) {
-        if (words) {
-          words[2 * (size_t)row] = xb;
-          words[2 * (size_t)row + 1] = 0;
-        }
-        if (logq) logq[row] = lq;
-      }
-      if (xout)
-        for (int k = q; k < n; k += 4) xout[(size_t)row * n + k] = ((xb >> k) & 1u) ? (uint16_t)0x3f80 : (uint16_t)0;
-    }
+    if (live) made_store(n, row, q, xb, lq, words, xout, logq);
+  }
+}
+
+// The packed-weight, one-block-per-degree case (H = 16 k, k < n - 1, as the
+// 34 -> 512 -> 512 -> 34 proposal): the same draws as k_made_sample16, with
+// the raw-layout paths compiled out and the register budget held to WAVES
+// wavefronts per SIMD.  The draw chain of one 16-row group is latency-bound
+// (n dependent steps of L2 reads, MFMA chains and FP64 draws), so more
+// resident groups per SIMD hide more of it.
+template <int AHEAD, int WAVES>
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(WAVES, WAVES)))
+k_made_sample_one(MadeWeights wt, const float* __restrict__ b1, const float* __restrict__ b2,
+                  const float* __restrict__ b3, const double* __restrict__ u, int B, uint64_t* __restrict__ words,
+                  uint16_t* __restrict__ xout, double* __restrict__ logq) {
+  extern __shared__ __align__(16) unsigned char smem[];
+  bf16x4* h1 = reinterpret_cast<bf16x4*>(smem);   // [H / 16 blocks][64 lanes]
+  const int lane = threadIdx.x;
+  const int r = lane & 15, q = lane >> 4;
+  const int n = wt.n, ndeg = wt.nb, nob = (n + 15) >> 4;
+  for (int base = blockIdx.x * 16; base < B; base += gridDim.x * 16) {
+    const int row = base + r;
+    const bool live = row < B;
+    uint64_t xb = 0;
+    double lq = 0.0;
+    made_rows_one<true, AHEAD>(wt, b1, b2, b3, u, row, live, r, q, lane, nob, ndeg, h1, xb, lq);
+    if (live) made_store(n, row, q, xb, lq, words, xout, logq);
   }
 }
 
@@ -503,24 +555,29 @@ static long long made_pack_fragments(int n, int H) {
 static int launch_made_sample16(const MadeWeights& wt, const float* b1, const float* b2, const float* b3,
                                 const double* u, int B, uint64_t* words, uint16_t* xout, double* logq,
                                 hipStream_t stream) {
+  typedef void (*Kernel)(MadeWeights, const float*, const float*, const float*, const double*, int, uint64_t*,
+                         uint16_t*, double*);
+  constexpr int kOneAhead = 4, kOneWaves = 3;   // 165 VGPRs: three wavefronts per SIMD, no spills
+  const bool one = wt.P && wt.nb <= wt.n - 1;   // packed copy, one block per degree
+  const Kernel kern = one ? &k_made_sample_one<kOneAhead, kOneWaves> : &k_made_sample16;
   const size_t lds = (size_t)wt.nb * 64 * 8;
   const int groups = (B + 15) / 16;
   // one round of the workgroups that fit a CU at once (registers and LDS),
   // so every wave takes an even share of the 16-row groups
-  static size_t cached_lds = 0;
-  static int cached_per_cu = 0;
-  if (lds != cached_lds) {
+  static size_t cached_lds[2] = {0, 0};
+  static int cached_per_cu[2] = {0, 0};
+  if (lds != cached_lds[one]) {
     int per_cu = 0;
-    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(&k_made_sample16), 64,
-                                                     lds) != hipSuccess || per_cu < 1)
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, reinterpret_cast<const void*>(kern), 64, lds) !=
+            hipSuccess ||
+        per_cu < 1)
       per_cu = std::max(1, (int)((160 * 1024) / lds));
-    cached_lds = lds;
-    cached_per_cu = per_cu;
+    cached_lds[one] = lds;
+    cached_per_cu[one] = per_cu;
   }
-  const int per_cu = cached_per_cu;
-  const int grid = std::min(groups, 256 * per_cu);
-  hipLaunchKernelGGL(k_made_sample16, dim3(grid), dim3(64), lds, stream, wt, b1, b2, b3, u, B, words, xout, logq);
-  RGS_LAUNCH_CHECK("k_made_sample16");
+  const int grid = std::min(groups, 256 * cached_per_cu[one]);
+  hipLaunchKernelGGL(kern, dim3(grid), dim3(64), lds, stream, wt, b1, b2, b3, u, B, words, xout, logq);
+  RGS_LAUNCH_CHECK(one ? "k_made_sample_one" : "k_made_sample16");
   return kOk;
 }
 
