@@ -66,9 +66,10 @@ __device__ __forceinline__ int scale_exponent(uint32_t mxh) {
   return biased ? ((biased - 1023) >> 1) + 1 : 0;
 }
 
-// One step's uniform constants.
+// One step's uniform constants.  The step's 2^nker kernel combinations (y = 1
+// in the upper half when split) are passed beside them, from the plan in
+// global memory or from an LDS copy.
 struct ScanStepArgs {
-  const uint64_t* combos;  // 2^nker kernel combinations, y = 1 in the upper half when split
   uint32_t beta, mask;
   int r_in, nker;
   double2 f0, f1;
@@ -87,13 +88,13 @@ struct ScanStepArgs {
 // weight f_y and sign (-1)^(y (c . beta)); the lane's part of y is folded
 // into the per-lane weights fa / fb and sign masks ba / bb once per step.
 // Returns the lane's maximum high word of |out|^2.
-template <typename S, int K, int NKER, bool SPLIT>
-__device__ __forceinline__ uint32_t scan_outputs(const S* E, const ScanStepArgs& a, uint32_t lo_w,
+template <typename S, int K, int NKER, bool SPLIT, typename CP>
+__device__ __forceinline__ uint32_t scan_outputs(const S* E, const ScanStepArgs& a, CP combos, uint32_t lo_w,
                                                  const uint32_t* hstep, int nkw, S (&outv)[K]) {
   constexpr int NC = 1 << NKER, HALF = SPLIT ? NC / 2 : NC;
   uint32_t cmb[NC];
 #pragma unroll
-  for (int t = 0; t < NC; ++t) cmb[t] = (uint32_t)a.combos[t];
+  for (int t = 0; t < NC; ++t) cmb[t] = (uint32_t)combos[t];
   const uint32_t ylo = SPLIT ? 0u : (lo_w >> a.r_in) & 1u;
   const double2 fa = ylo ? a.f1 : a.f0, fb = ylo ? a.f0 : a.f1;   // weights for y(hi part) = 0 / 1
   const uint32_t ba = ylo ? a.beta : 0u, bb = a.beta ^ ba;
@@ -182,7 +183,7 @@ __global__ void __launch_bounds__(64 * W) k_scan_batch(const uint64_t* __restric
       const int xv = (int)((v < 64 ? (x0 >> v) : (x1 >> (v - 64))) & 1u);
       a.f0 = u[(v * 2 + xv) * 2 + 0];
       a.f1 = u[(v * 2 + xv) * 2 + 1];
-      a.combos = st + 66 + (1ull << (r_out > 6 ? r_out - 6 : 0));
+      const uint64_t* combos = st + 66 + (1ull << (r_out > 6 ? r_out - 6 : 0));
       a.mask = (1u << a.r_in) - 1u;
       const int nk = r_out > 6 ? 1 << (r_out - 6) : 1;
       const int nkw = nk >= W ? nk / W : (wave < nk ? 1 : 0);
@@ -195,12 +196,12 @@ __global__ void __launch_bounds__(64 * W) k_scan_batch(const uint64_t* __restric
       uint32_t mxh;
       // the packed plan has nker <= 2, and split steps nker >= 1
       if (split) {
-        mxh = a.nker == 1 ? scan_outputs<S, K, 1, true>(E, a, lo_w, hstep, nkw, outv)
-                          : scan_outputs<S, K, 2, true>(E, a, lo_w, hstep, nkw, outv);
+        mxh = a.nker == 1 ? scan_outputs<S, K, 1, true>(E, a, combos, lo_w, hstep, nkw, outv)
+                          : scan_outputs<S, K, 2, true>(E, a, combos, lo_w, hstep, nkw, outv);
       } else {
-        mxh = a.nker == 0   ? scan_outputs<S, K, 0, false>(E, a, lo_w, hstep, nkw, outv)
-              : a.nker == 1 ? scan_outputs<S, K, 1, false>(E, a, lo_w, hstep, nkw, outv)
-                            : scan_outputs<S, K, 2, false>(E, a, lo_w, hstep, nkw, outv);
+        mxh = a.nker == 0   ? scan_outputs<S, K, 0, false>(E, a, combos, lo_w, hstep, nkw, outv)
+              : a.nker == 1 ? scan_outputs<S, K, 1, false>(E, a, combos, lo_w, hstep, nkw, outv)
+                            : scan_outputs<S, K, 2, false>(E, a, combos, lo_w, hstep, nkw, outv);
       }
       // lanes past 2^r_out (steps of rank below six) computed in-range values
       // that are neither counted nor stored
@@ -267,6 +268,186 @@ static hipError_t launch_rank(const uint64_t* plan, const double2* u, const uint
   constexpr int NB = R > 6 ? 1 << (R - 6) : 1;
   constexpr int WE = W < NB ? W : NB;
   return launch_scan<S, NB / WE, WE>(plan, u, bits, count, val, ex, R, st);
+}
+
+// ---------------------------------------------------------------- rank <= 6
+// Plans of rank at most six have one output block of at most 64 entries per
+// step.  A step of the general kernel spends one full 64-lane pass per
+// proposal, though a step has only 2^r_out outputs (26 on average for a
+// rank-6 plan of 34 vertices), and at this size the pass is bound by
+// instruction issue (FP64 sums on every lane).  Here a wavefront owns P
+// proposals at once and gives each a slot of G = max(2^r_out, 64 / P) lanes,
+// so a step of 2^r_out <= 64 / P outputs serves all P proposals in one pass
+// (64 / G proposals per pass, P G / 64 passes).  Each proposal's exponent is
+// the maximum over its own slot, and every output is summed by scan_outputs
+// with K = 1, so values and exponents are those of the general kernel bit for
+// bit.  Each workgroup first copies every step's constants into LDS (header,
+// split flag, beta, the kernel combinations, the 64 lane words with the
+// step's block word folded in, and both weight pairs of the step's vertex);
+// after that one barrier its PW wavefronts run independently.
+constexpr int kSmallMaxSteps = 128;   // packed plans have at most 128 steps
+struct SmallStep {
+  uint32_t hdr, split, beta, pad;
+  uint32_t combos[4];
+  uint32_t lo[64];
+};
+constexpr int kSmallStepWords = (int)(sizeof(SmallStep) / 4);
+static_assert(kSmallStepWords == 72, "SmallStep layout");
+constexpr size_t kSmallPlanBytes = (size_t)kSmallMaxSteps * (sizeof(SmallStep) + 4 * sizeof(double2));
+constexpr int kSmallP = 4;   // proposals per wavefront
+
+// LDS of one wavefront: P tables of 64 entries, the P proposals' bit words,
+// and their exponent sums.
+template <typename S, int P>
+constexpr size_t small_wave_bytes() {
+  return (size_t)P * 64 * sizeof(S) + (size_t)P * 16 + 16;
+}
+
+// Orders this wavefront's LDS accesses across its lanes (a lane reads what
+// another lane of the same wavefront wrote): no workgroup barrier is needed
+// because a wavefront's LDS operations complete in program order.
+__device__ __forceinline__ void wave_lds_sync() {
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+}
+
+// Maximum over aligned segments of 2^lg lanes, lg in 4..6 (uniform): DPP
+// within each row of 16 lanes, then lane swaps across rows.  All 64 lanes
+// must be active.
+__device__ __forceinline__ uint32_t segment_max_u32(uint32_t v, int lg) {
+  v = max(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0xB1, 0xF, 0xF, false));    // quad_perm 1,0,3,2
+  v = max(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x4E, 0xF, 0xF, false));    // quad_perm 2,3,0,1
+  v = max(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x124, 0xF, 0xF, false));   // row_ror 4
+  v = max(v, (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, 0x128, 0xF, 0xF, false));   // row_ror 8
+  if (lg >= 5) v = max(v, (uint32_t)__shfl_xor((int)v, 16));
+  if (lg >= 6) v = max(v, (uint32_t)__shfl_xor((int)v, 32));
+  return v;
+}
+
+template <typename S, int PW, int P>
+__global__ void __launch_bounds__(64 * PW) k_scan_small(const uint64_t* __restrict__ plan,
+                                                       const double2* __restrict__ u,
+                                                       const uint64_t* __restrict__ bits, int count,
+                                                       double2* __restrict__ out_val, int32_t* __restrict__ out_exp) {
+  static_assert(P == 1 || P == 2 || P == 4, "P divides the wavefront into slots of at least 16 lanes");
+  constexpr int LOG2P = P == 4 ? 2 : P == 2 ? 1 : 0;
+  extern __shared__ __align__(16) unsigned char smem[];
+  SmallStep* ps = reinterpret_cast<SmallStep*>(smem);
+  uint32_t* psw = reinterpret_cast<uint32_t*>(smem);
+  double2* pf = reinterpret_cast<double2*>(smem + (size_t)kSmallMaxSteps * sizeof(SmallStep));
+  const int lane = threadIdx.x & 63;
+  const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  unsigned char* wb = smem + kSmallPlanBytes + (size_t)wave * small_wave_bytes<S, P>();
+  S* E = reinterpret_cast<S*>(wb);                                            // [P][64]
+  uint64_t* xw = reinterpret_cast<uint64_t*>(wb + (size_t)P * 64 * sizeof(S));  // [P][2]
+  int* es = reinterpret_cast<int*>(wb + (size_t)P * 64 * sizeof(S) + (size_t)P * 16);   // [P]
+  const int n = (int)plan[0];
+  // Step words: the plan layout at r_out <= 6 is header, beta, 64 lane words,
+  // one block word, then 2^nker combinations.
+  for (int i = threadIdx.x; i < n * kSmallStepWords; i += 64 * PW) {
+    const int j = i / kSmallStepWords, k = i - j * kSmallStepWords;
+    const uint64_t* st = plan + plan[2 + j];
+    const uint64_t hdr = st[0];
+    uint32_t w = 0;
+    if (k == 0) w = (uint32_t)hdr;
+    else if (k == 1) w = (uint32_t)(hdr >> 32) & 1u;
+    else if (k == 2) w = (uint32_t)st[1];
+    else if (k >= 4 && k < 8) w = (k - 4) < (1 << ((hdr >> 24) & 255u)) ? (uint32_t)st[67 + (k - 4)] : 0u;
+    else if (k >= 8) w = (uint32_t)st[2 + (k - 8)] ^ (uint32_t)st[66];
+    psw[i] = w;
+  }
+  // weights u[v][xv][0..1] of the step's vertex, both values of the bit
+  for (int i = threadIdx.x; i < n * 4; i += 64 * PW) {
+    const int v = (int)(plan[plan[2 + (i >> 2)]] & 255u);
+    pf[i] = u[v * 4 + (i & 3)];
+  }
+  __syncthreads();
+  for (int g = blockIdx.x * PW + wave; g * P < count; g += gridDim.x * PW) {
+    const int pbase = g * P;
+    if (lane < P) {
+      const bool in = pbase + lane < count;   // slots past the batch run on zero bits, unstored
+      xw[2 * lane] = in ? bits[2 * (size_t)(pbase + lane)] : 0ull;
+      xw[2 * lane + 1] = in ? bits[2 * (size_t)(pbase + lane) + 1] : 0ull;
+      es[lane] = 0;
+      E[64 * lane] = make_s<S>(1.0, 0.0);
+    }
+    wave_lds_sync();
+    for (int j = 0; j < n; ++j) {
+      const SmallStep& sp = ps[j];
+      const uint32_t hdr = __builtin_amdgcn_readfirstlane(sp.hdr);
+      const bool split = __builtin_amdgcn_readfirstlane(sp.split) != 0u;
+      const int v = (int)(hdr & 255u), r_out = (int)((hdr >> 16) & 255u);
+      ScanStepArgs a;
+      a.r_in = (int)((hdr >> 8) & 255u);
+      a.nker = (int)((hdr >> 24) & 255u);
+      a.beta = __builtin_amdgcn_readfirstlane(sp.beta);
+      a.mask = (1u << a.r_in) - 1u;
+      const int lg = r_out > 6 - LOG2P ? r_out : 6 - LOG2P;   // log2 of the slot width G
+      const int passes = P >> (6 - lg);
+      const int out = lane & ((1 << lg) - 1);
+      const bool lane_ok = out < (1 << r_out);
+      const uint32_t lo_w = sp.lo[out];
+      for (int k = 0; k < passes; ++k) {
+        const int p = (k << (6 - lg)) + (lane >> lg);
+        const uint64_t x0 = xw[2 * p], x1 = xw[2 * p + 1];
+        const int xv = (int)((v < 64 ? (x0 >> v) : (x1 >> (v - 64))) & 1u);
+        a.f0 = pf[4 * j + 2 * xv];
+        a.f1 = pf[4 * j + 2 * xv + 1];
+        S* Ep = E + 64 * p;
+        const uint32_t hstep[1] = {0u};
+        S outv[1];
+        uint32_t mxh;
+        if (split) {
+          mxh = a.nker == 1 ? scan_outputs<S, 1, 1, true>(Ep, a, sp.combos, lo_w, hstep, 1, outv)
+                            : scan_outputs<S, 1, 2, true>(Ep, a, sp.combos, lo_w, hstep, 1, outv);
+        } else {
+          mxh = a.nker == 0   ? scan_outputs<S, 1, 0, false>(Ep, a, sp.combos, lo_w, hstep, 1, outv)
+                : a.nker == 1 ? scan_outputs<S, 1, 1, false>(Ep, a, sp.combos, lo_w, hstep, 1, outv)
+                              : scan_outputs<S, 1, 2, false>(Ep, a, sp.combos, lo_w, hstep, 1, outv);
+        }
+        mxh = segment_max_u32(lane_ok ? mxh : 0u, lg);
+        const int e = scale_exponent(mxh);
+        wave_lds_sync();   // every lane's reads of the pass's inputs precede the writes
+        if (lane_ok) {
+          S val = outv[0];
+          val.x = ldexp(val.x, -e);
+          val.y = ldexp(val.y, -e);
+          Ep[out] = val;
+        }
+        if (out == 0) es[p] += e;
+        wave_lds_sync();
+      }
+    }
+    if (lane < P && pbase + lane < count) {
+      out_val[pbase + lane] = load_c(E[64 * lane]);
+      out_exp[pbase + lane] = es[lane];
+    }
+    wave_lds_sync();
+  }
+}
+
+// Wavefronts per workgroup: the workgroup's plan copy is fixed (128 steps),
+// so wider workgroups spread it over more proposals; two workgroups fit a CU
+// at each precision (complex64 at 16 waves, complex128 at 8).
+template <typename S>
+static hipError_t launch_small(const uint64_t* plan, const double2* u, const uint64_t* bits, int count, double2* val,
+                               int32_t* ex, hipStream_t st) {
+  constexpr int PW = sizeof(S) == 8 ? 16 : 8, P = kSmallP;
+  const size_t shmem = kSmallPlanBytes + (size_t)PW * small_wave_bytes<S, P>();
+  static bool attr = false;
+  if (!attr) {
+    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_scan_small<S, PW, P>),
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)shmem);
+    if (e != hipSuccess) return e;
+    attr = true;
+  }
+  // one round of the workgroups that fit a CU at once (LDS bound), each
+  // copying the plan once for its share of the proposals
+  const int per_cu = std::max(1, (int)((160 * 1024) / shmem));
+  const int grid = std::min((count + PW * P - 1) / (PW * P), 256 * per_cu);
+  hipLaunchKernelGGL((k_scan_small<S, PW, P>), dim3(grid), dim3(64 * PW), shmem, st, plan, u, bits, count, val, ex);
+  return hipGetLastError();
 }
 
 // ---------------------------------------------------------------- large rank
@@ -484,14 +665,14 @@ extern "C" int rgs_scan_batch(const uint64_t* plan_dev, int rmax, const double* 
   double2* val = reinterpret_cast<double2*>(out_val);
   hipError_t e;
   if (precision == 64) {
-    if (rmax <= 6) e = launch_rank<float2, 6, kWavesC64>(plan_dev, u, bits_dev, count, val, out_exp, st);
+    if (rmax <= 6) e = launch_small<float2>(plan_dev, u, bits_dev, count, val, out_exp, st);
     else if (rmax <= 8) e = launch_rank<float2, 8, kWavesC64>(plan_dev, u, bits_dev, count, val, out_exp, st);
     else if (rmax <= 10) e = launch_rank<float2, 10, kWavesC64>(plan_dev, u, bits_dev, count, val, out_exp, st);
     else if (rmax <= 12) e = launch_rank<float2, 12, kWavesC64>(plan_dev, u, bits_dev, count, val, out_exp, st);
     else if (rmax <= 13) e = launch_rank<float2, 13, kWavesC64>(plan_dev, u, bits_dev, count, val, out_exp, st);
     else return fail(kInvalid, "scan_batch: complex64 batch limited to rmax <= 13");
   } else if (precision == 128) {
-    if (rmax <= 6) e = launch_rank<double2, 6, kWavesC128>(plan_dev, u, bits_dev, count, val, out_exp, st);
+    if (rmax <= 6) e = launch_small<double2>(plan_dev, u, bits_dev, count, val, out_exp, st);
     else if (rmax <= 8) e = launch_rank<double2, 8, kWavesC128>(plan_dev, u, bits_dev, count, val, out_exp, st);
     else if (rmax <= 10) e = launch_rank<double2, 10, kWavesC128>(plan_dev, u, bits_dev, count, val, out_exp, st);
     else if (rmax <= 11) e = launch_rank<double2, 11, kWavesC128>(plan_dev, u, bits_dev, count, val, out_exp, st);
@@ -500,7 +681,7 @@ extern "C" int rgs_scan_batch(const uint64_t* plan_dev, int rmax, const double* 
   } else {
     return fail(kInvalid, "precision must be 64 (complex64 tables) or 128 (complex128)");
   }
-  if (e != hipSuccess) return fail(kHip, std::string("k_scan_batch: ") + hipGetErrorString(e));
+  if (e != hipSuccess) return fail(kHip, std::string(rmax <= 6 ? "k_scan_small: " : "k_scan_batch: ") + hipGetErrorString(e));
   return kOk;
 }
 

@@ -77,3 +77,37 @@ def test_config3_instance_against_oracle_sweep():
     for k in range(6):
         ref = osc.amplitude(steps, u, bits[k].tolist())
         assert p128[k] == pytest.approx(abs(ref) ** 2, rel=1e-12)
+
+
+@pytest.mark.parametrize('count', [3, 4093])
+def test_rank6_kernel_matches_general_kernel(count):
+    """Plans of rank <= 6 run in the LDS-staged one-wave-per-proposal kernel;
+    the same plan passed with a larger rmax runs in the general batch kernel.
+    Both sum in the same order, so values and exponents agree bit for bit, in
+    both table precisions, for counts below and off a multiple of the
+    wavefronts per workgroup."""
+    from rgs70_classical_challenge_amd.scan import ScanOracle, order_search, bits_to_words
+    from rgs70_classical_challenge_amd._lib import call, ptr
+    g, u = instance(34, 20261205)
+    order, _ = order_search(g, seconds=2.0, rmax_cap=6)
+    rng = np.random.default_rng(count)
+    bits = rng.integers(0, 2, (count, 34), dtype=np.uint8)
+    for fp64 in (False, True):
+        so = ScanOracle(g, u, order, fp64=fp64)
+        assert so.rmax <= 6
+        prec = 128 if fp64 else 64
+        words = torch.as_tensor(bits_to_words(bits).view(np.int64)).to(so.device)
+        out = {}
+        for rmax in (so.rmax, 8):
+            print(f'case count={count} fp64={fp64} rmax={rmax}', flush=True)
+            val = torch.empty((count, 2), dtype=torch.float64, device=so.device)
+            ex = torch.empty(count, dtype=torch.int32, device=so.device)
+            call('rgs_scan_batch', ptr(so._plan), rmax, ptr(so._u), ptr(words), count, prec, ptr(val), ptr(ex),
+                 so.stream.cuda_stream)
+            so.stream.synchronize()
+            out[rmax] = (val.cpu().numpy(), ex.cpu().numpy())
+        assert np.array_equal(out[so.rmax][0], out[8][0])
+        assert np.array_equal(out[so.rmax][1], out[8][1])
+        steps, _, _ = osc.compile_plan(g, order)
+        amp = out[so.rmax][0][0] @ [1, 1j] * 2.0 ** (out[so.rmax][1][0] - 17)
+        assert amp == pytest.approx(osc.amplitude(steps, u, bits[0].tolist()), rel=1e-12 if fp64 else 2e-6)
