@@ -269,6 +269,18 @@ int rgs_scan_pack(const rgs_scan_step *steps, int n, uint64_t *blob, uint64_t *w
 int rgs_scan_batch(const uint64_t *plan_dev, int rmax, const double *u_dev, const uint64_t *bits_dev,
                    int count, int precision, double *out_val, int32_t *out_exp, void *stream);
 
+/* Capped-rejection decisions of `count` scored proposals, on the device:
+ * p = (re^2 + im^2) 2^(2 out_exp - n) from rgs_scan_batch's outputs (val
+ * complex128 [count], ex int32 [count]), then keep[i] = u[i] < min(p * scale,
+ * 1) for uniform proposals (logq == NULL, scale = 2^n / cap) or
+ * u[i] < min(p / (scale * exp(logq[i])), 1) for proposals of density q
+ * (scale = cap).  Each product, sum and quotient is one FP64 rounding, in
+ * that order.  keep: uint8 [count] (0 or 1); the number kept is added to
+ * *accepted (device uint64, not cleared here).  Replaces the accept test of
+ * rejection_sampler.py:18-24, one proposal and one host comparison per call. */
+int rgs_capped_decide(const double *val, const int32_t *ex, const double *u, const double *logq, int count, int n,
+                      double scale, uint8_t *keep, unsigned long long *accepted, void *stream);
+
 /* Byte-table form of a plan for rgs_scan_large (ranks <= 32): per step a
  * header, beta, ceil(r_out/8) x 256 words of XORs of the lcols and the kernel
  * combinations; blob[2 + j] is the word offset of step j.  Call with

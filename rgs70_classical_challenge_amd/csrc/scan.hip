@@ -652,6 +652,37 @@ static int scan_large(const rgs_scan_step* steps, int n, const uint64_t* plan_ho
   return kOk;
 }
 
+// ------------------------------------------------------------ decisions
+// Capped-rejection decisions from the batch scan's outputs, one proposal per
+// lane: p = (re^2 + im^2) 2^(2 e - n), acc = min(p * scale, 1) (uniform
+// proposals, scale = 2^n / cap) or min(p / (scale * exp(log q)), 1) (scale =
+// cap), keep = u < acc.  Every product, sum and quotient is rounded on its
+// own (no fused multiply-add), as the same expressions over device tensors
+// evaluate them; the power of two is exact.  Each wavefront adds its kept
+// count to *accepted with one atomic.
+__global__ void __launch_bounds__(256) k_capped_decide(const double2* __restrict__ val,
+                                                       const int32_t* __restrict__ ex, const double* __restrict__ u,
+                                                       const double* __restrict__ logq, int count, int n,
+                                                       double scale, uint8_t* __restrict__ keep,
+                                                       unsigned long long* __restrict__ accepted) {
+  // hipcc contracts a * b + c into one fused multiply-add by default (also
+  // through __dmul_rn / __dadd_rn once inlined); the decision must round the
+  // squares and their sum separately
+#pragma clang fp contract(off)
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  bool k = false;
+  if (i < count) {
+    const double2 v = val[i];
+    const double s = v.x * v.x + v.y * v.y;
+    const double p = ldexp(s, 2 * ex[i] - n);
+    const double acc = logq ? p / (scale * exp(logq[i])) : p * scale;
+    k = u[i] < fmin(acc, 1.0);
+    keep[i] = k ? 1 : 0;
+  }
+  const unsigned long long b = __ballot(k);
+  if ((threadIdx.x & 63) == 0 && b) atomicAdd(accepted, (unsigned long long)__popcll(b));
+}
+
 }  // namespace rgs
 
 using namespace rgs;
@@ -682,6 +713,19 @@ extern "C" int rgs_scan_batch(const uint64_t* plan_dev, int rmax, const double* 
     return fail(kInvalid, "precision must be 64 (complex64 tables) or 128 (complex128)");
   }
   if (e != hipSuccess) return fail(kHip, std::string(rmax <= 6 ? "k_scan_small: " : "k_scan_batch: ") + hipGetErrorString(e));
+  return kOk;
+}
+
+extern "C" int rgs_capped_decide(const double* val, const int32_t* ex, const double* u, const double* logq,
+                                 int count, int n, double scale, uint8_t* keep, unsigned long long* accepted,
+                                 void* stream) {
+  if (!val || !ex || !u || !keep || !accepted || count < 0 || n < 1 || n > 128 || !(scale > 0.0))
+    return fail(kInvalid, "capped_decide: bad arguments");
+  if (count == 0) return kOk;
+  hipLaunchKernelGGL(k_capped_decide, dim3((unsigned)((count + 255) / 256)), dim3(256), 0,
+                     reinterpret_cast<hipStream_t>(stream), reinterpret_cast<const double2*>(val), ex, u, logq,
+                     count, n, scale, keep, accepted);
+  RGS_LAUNCH_CHECK("k_capped_decide");
   return kOk;
 }
 

@@ -19,6 +19,8 @@ import time
 
 import numpy as np
 
+from ._lib import call, ptr
+
 FLIP_TOLERANCE = 1e-5
 
 
@@ -135,7 +137,7 @@ def uniform_words_device(count, n, gen, dev):
         if bits <= 0:
             continue
         if bits <= 62:
-            words[:, w] = torch.randint(0, 1 << bits, (count,), generator=gen, device=dev, dtype=torch.int64)
+            words[:, w].random_(0, 1 << bits, generator=gen)
         else:
             lo = torch.randint(0, 1 << 32, (count,), generator=gen, device=dev, dtype=torch.int64)
             hi = torch.randint(0, 1 << (bits - 32), (count,), generator=gen, device=dev, dtype=torch.int64)
@@ -175,8 +177,14 @@ class DeviceCappedRejection:
         # max_ahead) before reading again, so the device is not left idle
         # between batches.
         ahead = 1
+        # each batch is a few launches: proposals, acceptance uniforms, the
+        # scan, and one fused decision kernel (p, the capped acceptance
+        # ratio, u < ratio and the kept count), so the host keeps ahead of
+        # the device
+        scale = 2.0 ** n / cap if net is None else cap
+        count_dev = torch.zeros(1, dtype=torch.int64, device=dev)
         while batches < max_batches:
-            counts = []
+            count_dev.zero_()
             for _ in range(min(ahead, max_batches - batches)):
                 if net is None:
                     words = uniform_words_device(batch, n, gen, dev)
@@ -185,21 +193,19 @@ class DeviceCappedRejection:
                     sb['u'][:batch].uniform_(generator=gen)
                     words, logq = net.sample_fused(batch)
                     words = words.clone()
+                    logq = logq.clone()
                 u = torch.rand(batch, generator=gen, device=dev, dtype=torch.float64)
-                p = self.scorer.probabilities_device(words)
-                if logq is None:
-                    acc = torch.clamp(p * (2.0 ** n / cap), max=1.0)
-                else:
-                    acc = torch.clamp(p / (cap * torch.exp(logq)), max=1.0)
-                keep = u < acc
-                counts.append(keep.sum())
+                val, ex = self.scorer.scores_device(words)
+                keep = torch.empty(batch, dtype=torch.uint8, device=dev)
+                call('rgs_capped_decide', ptr(val), ptr(ex), ptr(u), None if logq is None else ptr(logq), batch, n,
+                     scale, ptr(keep), ptr(count_dev), self.scorer.stream.cuda_stream)
                 proposals += batch
                 batches += 1
                 kept_words.append(words)
                 kept_u.append(u)
-                kept_logq.append(None if logq is None else logq.clone())
-                kept_keep.append(keep)
-            accepted += int(torch.stack(counts).sum().item())
+                kept_logq.append(logq)
+                kept_keep.append(keep.view(torch.bool))
+            accepted += int(count_dev.item())
             if accepted >= accept_target:
                 break
             rate = max(accepted / proposals, 1e-9)

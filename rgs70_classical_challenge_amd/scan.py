@@ -29,6 +29,8 @@ _lib.register('rgs_scan_order_search', ctypes.c_int,
 _lib.register('rgs_scan_pack', ctypes.c_int, [_V, ctypes.c_int, _V, ctypes.POINTER(ctypes.c_uint64)])
 _lib.register('rgs_scan_batch', ctypes.c_int,
               [_V, ctypes.c_int, _V, _V, ctypes.c_int, ctypes.c_int, _V, _V, _V])
+_lib.register('rgs_capped_decide', ctypes.c_int,
+              [_V, _V, _V, _V, ctypes.c_int, ctypes.c_int, ctypes.c_double, _V, _V, _V])
 _lib.register('rgs_scan_pack_tables', ctypes.c_int, [_V, ctypes.c_int, _V, ctypes.POINTER(ctypes.c_uint64)])
 _lib.register('rgs_scan_large', ctypes.c_int,
               [_V, ctypes.c_int, _V, _V, _V, _V, ctypes.c_int, _V, _V, ctypes.c_uint64, _V, _V, _V, _V])
@@ -60,6 +62,16 @@ def order_search(g, start=None, seconds=5.0, seed=20261030, rmax_cap=62):
     call('rgs_scan_order_search', ptr(g), n, ptr(s), float(seconds), int(seed), int(rmax_cap), ptr(best),
          ctypes.byref(cost))
     return best.tolist(), int(cost.value)
+
+
+def probabilities_from_scores(val, ex, n):
+    """p = (re re + im im) 2^(2 ex - n) of the batch kernel's device outputs.
+    The power of two is assembled from its exponent bits, so it is exact
+    whatever a device exp2 or pow would round to (normal range: p of every
+    proposal of a normalised state lies far above 2^-1022)."""
+    import torch
+    pw =((2 * ex.to(torch.int64) - n + 1023) << 52).view(torch.float64)
+    return (val[:, 0] * val[:, 0] + val[:, 1] * val[:, 1]) * pw
 
 
 def bits_to_words(bits):
@@ -185,10 +197,12 @@ class ScanOracle:
                  ptr(val) + 16 * k, ptr(ex) + 4 * k, self.stream.cuda_stream)
         return val, ex, None
 
-    def probabilities_device(self, words_dev, precision=None):
-        """p(x) of proposals already on the device as int64 [count, 2] words
-        (bits_to_words layout); returns a device float64 tensor, nothing is
-        copied to the host (batch kernels only: rank within their LDS limit)."""
+    def scores_device(self, words_dev, precision=None):
+        """The batch kernel's outputs for proposals already on the device as
+        int64 [count, 2] words (bits_to_words layout): device tensors val
+        float64 [count, 2] (re, im) and ex int32 [count], amplitude =
+        (re + i im) 2^ex 2^(-n/2); nothing is copied to the host (batch
+        kernels only: rank within their LDS limit)."""
         torch = self.torch
         prec = precision or self.precision
         if self.rmax > self.batch_rmax[prec]:
@@ -198,7 +212,14 @@ class ScanOracle:
         ex = torch.empty(count, dtype=torch.int32, device=self.device)
         call('rgs_scan_batch', ptr(self._plan), self.rmax, ptr(self._u), ptr(words_dev), count, prec, ptr(val),
              ptr(ex), self.stream.cuda_stream)
-        return (val[:, 0] ** 2 + val[:, 1] ** 2) * torch.exp2(2.0 * ex.double() - self.n)
+        return val, ex
+
+    def probabilities_device(self, words_dev, precision=None):
+        """p(x) of proposals already on the device (scores_device) as a
+        device float64 tensor, with rgs_capped_decide's arithmetic: each
+        square and the sum rounded once, the power of two exact."""
+        val, ex = self.scores_device(words_dev, precision)
+        return probabilities_from_scores(val, ex, self.n)
 
     def amplitudes(self, bits, precision=None):
         val, ex, _ = self.raw(bits, precision)

@@ -57,3 +57,46 @@ def test_uniform_and_learned_rejection_match_closed_forms():
         print('learned', cap, r['acceptance'], acc, r['xeb'], xeb, r['xeb_se'], flush=True)
         assert abs(r['acceptance'] - acc) < 5 * np.sqrt(acc * (1 - acc) / r['proposals']) + 2e-3
         assert abs(r['xeb'] - xeb) < 5 * r['xeb_se'] + 2e-3
+
+
+@pytest.mark.parametrize('learned', [False, True])
+def test_capped_decide_rounds_as_tensor_expressions(learned):
+    """rgs_capped_decide against the same decision written as device tensor
+    expressions (p from probabilities_from_scores, which the complex128 audit
+    also uses): u is set exactly at the tensor-computed acceptance ratio for
+    half the proposals (not kept) and one ulp below it for the other half
+    (kept), so a ratio off by one rounding flips a decision.  The count is
+    not a multiple of the wavefront or workgroup size."""
+    torch = _torch()
+    from rgs70_classical_challenge_amd.scan import probabilities_from_scores
+    from rgs70_classical_challenge_amd._lib import call, ptr
+    dev = torch.device('cuda', 0)
+    n, cap, count = 34, 1.7 if learned else 8.0, 10007
+    g = torch.Generator(device=dev)
+    g.manual_seed(7 + learned)
+    val = torch.randn((count, 2), generator=g, device=dev, dtype=torch.float64) * 0.6
+    ex = torch.randint(-3, 4, (count,), generator=g, device=dev, dtype=torch.int32)
+    p = probabilities_from_scores(val, ex, n)
+    # the exact power of two, against an independent construction
+    assert torch.equal(p, (val[:, 0] * val[:, 0] + val[:, 1] * val[:, 1])
+                       * torch.tensor([2.0 ** k for k in (2 * ex.cpu().numpy() - n).tolist()], device=dev))
+    if learned:
+        logq = -n * np.log(2.0) + torch.randn(count, generator=g, device=dev, dtype=torch.float64)
+        acc = torch.clamp(p / (cap * torch.exp(logq)), max=1.0)
+        scale = cap
+    else:
+        logq = None
+        acc = torch.clamp(p * (2.0 ** n / cap), max=1.0)
+        scale = 2.0 ** n / cap
+    below = torch.nextafter(acc, torch.zeros_like(acc))
+    half = torch.arange(count, device=dev) % 2 == 1
+    u = torch.where(half, below, acc).contiguous()
+    keep = torch.empty(count, dtype=torch.uint8, device=dev)
+    accepted = torch.zeros(1, dtype=torch.int64, device=dev)
+    call('rgs_capped_decide', ptr(val), ptr(ex), ptr(u), None if logq is None else ptr(logq), count, n, scale,
+         ptr(keep), ptr(accepted), torch.cuda.current_stream(dev).cuda_stream)
+    torch.cuda.synchronize(dev)
+    expect = (u < acc)
+    assert torch.equal(keep.bool(), expect)
+    assert torch.equal(expect, half & (acc > 0))
+    assert int(accepted.item()) == int(expect.sum().item())
