@@ -255,6 +255,9 @@ int rgs_scan_order_search(const uint8_t *g, int n, const int32_t *start, double 
 int rgs_scan_pack(const rgs_scan_step *steps, int n, uint64_t *blob, uint64_t *words);
 
 /* p(x) of `count` proposals in one persistent launch.  plan_dev: packed plan;
+ * rmax: its largest rank, max(r_in, r_out) over the steps (the LDS tables and
+ * the kernel, one wavefront per four proposals at rmax <= 6, are chosen by
+ * it; a larger value is allowed, a smaller one is not);
  * u_dev: complex128 [n][2][2] measurement unitaries (f_v = u[v][x_v][:]);
  * bits_dev: uint64 [count][2] bitstrings (qubit q = bit q).  precision 64
  * stores the tables as complex64 (rmax <= 13), 128 as complex128
