@@ -193,7 +193,7 @@ def run_scan(torch, count=65536, seed=70, reps=5):
     from rgs70_classical_challenge_amd.instances import instance
     from rgs70_classical_challenge_amd.scan import ScanOracle, order_search, bits_to_words
     g, u = instance(70, seed)
-    order, _ = order_search(g, seconds=5.0, rmax_cap=28)
+    order, _ = order_search(g, iterations=1000000, rmax_cap=28)
     so = ScanOracle(g, u, order)
     bits = np.random.default_rng(seed).integers(0, 2, (count, 70), dtype=np.uint8)
     words = torch.as_tensor(bits_to_words(bits).view(np.int64)).to(so.device)

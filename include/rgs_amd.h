@@ -236,11 +236,16 @@ int rgs_scan_compile(const uint8_t *g, int n, const int32_t *order, rgs_scan_ste
  * rank_cut (order_optimization/search.cpp:22-29). */
 int rgs_scan_profile(const uint8_t *g, int n, const int32_t *order, int32_t *profile);
 
-/* Annealed order search minimising sum_j 2^{r_j} with max r <= rmax_cap
- * (swap / insertion / reversal moves, geometric cooling over `seconds`).
- * Replaces order_optimization/search.cpp:240-274 and search_width.py:27-45. */
-int rgs_scan_order_search(const uint8_t *g, int n, const int32_t *start, double seconds, uint64_t seed,
-                          int rmax_cap, int32_t *best_order, uint64_t *best_cost);
+/* Annealed order search minimising sum_j 2^{r_j} with max r <= rmax_cap.
+ * `threads` independent chains (seeds derived from `seed`) each run
+ * `iterations` moves (neighbour swap, short-window insertion or reversal,
+ * long swap; epochs that restart from the chain's best order and cool
+ * geometrically); the best order over the chains is returned (ties: the
+ * lowest chain).  Deterministic for given arguments: no wall-clock budget.
+ * Replaces the annealer of order_optimization/search.cpp:16-63 (which runs
+ * for a wall-clock budget) and search_width.py:5-15. */
+int rgs_scan_order_search(const uint8_t *g, int n, const int32_t *start, uint64_t iterations, uint64_t seed,
+                          int rmax_cap, int threads, int32_t *best_order, uint64_t *best_cost);
 
 /* Pack compiled steps into the device plan (uint64 words): per step a header
  * (v | r_in << 8 | r_out << 16 | nker << 24 | split << 32), beta, p0 over the

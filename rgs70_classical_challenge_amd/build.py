@@ -38,7 +38,7 @@ def build(verbose=True):
         obj = os.path.join(objdir, os.path.basename(src) + '.o')
         cmd = [hipcc, f'--offload-arch={ARCH}', '-c', src, '-o', obj] + COMMON
         if src.endswith('.cpp'):
-            cmd = [hipcc, '-x', 'c++', '-c', src, '-o', obj, '-O3', '-std=c++17', '-fPIC',
+            cmd = [hipcc, '-x', 'c++', '-c', src, '-o', obj, '-O3', '-std=c++17', '-fPIC', '-pthread',
                    f'-I{INCLUDE}', f'-I{CSRC}']
         procs.append((src, subprocess.Popen(cmd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT)))
         objs.append(obj)
@@ -54,7 +54,7 @@ def build(verbose=True):
     if failed:
         raise RuntimeError('hipcc failed')
     tmp = LIB + '.tmp'
-    link = [hipcc, f'--offload-arch={ARCH}', '-shared', '-o', tmp] + objs + ['-fPIC']
+    link = [hipcc, f'--offload-arch={ARCH}', '-shared', '-o', tmp] + objs + ['-fPIC', '-pthread']
     subprocess.run(link, check=True)
     os.replace(tmp, LIB)
     if verbose:

@@ -18,6 +18,7 @@
 #include <cstring>
 #include <random>
 #include <string>
+#include <thread>
 #include <vector>
 
 #include "rgs_amd.h"
@@ -202,60 +203,131 @@ extern "C" int rgs_scan_profile(const uint8_t* g, int n, const int32_t* order, i
   return kOk;
 }
 
-extern "C" int rgs_scan_order_search(const uint8_t* g, int n, const int32_t* start, double seconds, uint64_t seed,
-                                     int rmax_cap, int32_t* best_order, uint64_t* best_cost) {
+// One annealing chain over orders.  A move permutes the positions i..j only,
+// so the prefix sets before i and from j on are unchanged and only the ranks
+// of the cuts i..j-1 are recomputed.  Moves: swap of neighbours, insertion
+// and reversal within a short window, and a swap of two arbitrary positions.
+// The run is split into epochs; each restarts from the chain's best order and
+// cools geometrically, with the temperature relative to the current cost (so
+// the same schedule serves any graph), and later epochs start colder.  The
+// chain is a pure function of (graph, start, iterations, seed, cap).
+namespace rgs {
+
+struct Chain {
+  std::vector<int> order, rank;
+  std::vector<u128> prefix;
+  uint64_t cost = 0;
+  int peak = 0;
+};
+
+static int cut_rank_small_side(const std::vector<u128>& adj, u128 left, int n, int left_count) {
+  u128 full = n == 128 ? ~(u128)0 : (((u128)1 << n) - 1);
+  // the cut rank is symmetric: eliminate over the smaller side
+  return 2 * left_count > n ? cut_rank(adj, full & ~left, n) : cut_rank(adj, left, n);
+}
+
+static void chain_init(const std::vector<u128>& adj, const std::vector<int>& order, int n, Chain& c) {
+  c.order = order;
+  c.prefix.assign(n, 0);
+  c.rank.assign(n, 0);
+  u128 p = 0;
+  for (int k = 0; k < n; ++k) {
+    p |= (u128)1 << order[k];
+    c.prefix[k] = p;
+    c.rank[k] = k == n - 1 ? 0 : cut_rank_small_side(adj, p, n, k + 1);
+  }
+  c.cost = cost_of(c.rank);
+  c.peak = *std::max_element(c.rank.begin(), c.rank.end());
+}
+
+static void run_chain(const std::vector<u128>& adj, const std::vector<int>& start, int n, uint64_t iterations,
+                      uint64_t seed, int rmax_cap, Chain& best) {
+  Chain cur;
+  chain_init(adj, start, n, cur);
+  best = cur;
+  bool best_ok = cur.peak <= rmax_cap;
+  std::mt19937_64 rng(seed);
+  std::uniform_real_distribution<double> unif(0.0, 1.0);
+  const int epochs = 6;
+  const uint64_t per_epoch = std::max<uint64_t>(1, iterations / epochs);
+  std::vector<int> cand_order(n);
+  std::vector<int> new_rank(n);
+  std::vector<u128> new_prefix(n);
+  for (uint64_t it = 0; it < iterations; ++it) {
+    const uint64_t e = it / per_epoch, within = it % per_epoch;
+    if (within == 0 && e > 0 && best_ok) cur = best;
+    const double phase = (double)within / (double)per_epoch;
+    const double t0 = e == 0 ? 0.02 : 0.02 * std::pow(0.4, (double)e);
+    const double temp = t0 * std::pow(1e-3, phase);
+    int i, j;
+    const int kind = (int)(rng() % 8);
+    i = (int)(rng() % n);
+    if (kind < 3) j = i + 1;                                        // neighbour swap
+    else if (kind < 7) j = i + 1 + (int)(rng() % 9);                // short window
+    else j = (int)(rng() % n);                                      // anywhere
+    if (j >= n) j = n - 1;
+    if (i == j) continue;
+    if (i > j) std::swap(i, j);
+    std::copy(cur.order.begin(), cur.order.end(), cand_order.begin());
+    if (kind < 3 || kind == 7) std::swap(cand_order[i], cand_order[j]);
+    else if (kind < 5) {
+      if (rng() & 1) std::rotate(cand_order.begin() + i, cand_order.begin() + i + 1, cand_order.begin() + j + 1);
+      else std::rotate(cand_order.begin() + i, cand_order.begin() + j, cand_order.begin() + j + 1);
+    } else std::reverse(cand_order.begin() + i, cand_order.begin() + j + 1);
+    // ranks of the cuts i..j-1 (the cut after position j holds the same set)
+    u128 p = i ? cur.prefix[i - 1] : 0;
+    int64_t diff = 0;
+    for (int k = i; k < j; ++k) {
+      p |= (u128)1 << cand_order[k];
+      new_prefix[k] = p;
+      new_rank[k] = cut_rank_small_side(adj, p, n, k + 1);
+      diff += (int64_t)(1ull << new_rank[k]) - (int64_t)(1ull << cur.rank[k]);
+    }
+    int peak = 0;
+    for (int k = 0; k < n; ++k) peak = std::max(peak, (k >= i && k < j) ? new_rank[k] : cur.rank[k]);
+    if (peak > rmax_cap && cur.peak <= rmax_cap) continue;
+    bool take = diff <= 0 || unif(rng) < std::exp(-(double)diff / (temp * (double)cur.cost));
+    if (!take) continue;
+    for (int k = i; k <= j; ++k) cur.order[k] = cand_order[k];
+    for (int k = i; k < j; ++k) {
+      cur.prefix[k] = new_prefix[k];
+      cur.rank[k] = new_rank[k];
+    }
+    cur.cost = (uint64_t)((int64_t)cur.cost + diff);
+    cur.peak = peak;
+    if (peak <= rmax_cap && (!best_ok || cur.cost < best.cost)) {
+      best = cur;
+      best_ok = true;
+    }
+  }
+  if (!best_ok) best.cost = ~0ull;
+}
+
+}  // namespace rgs
+
+extern "C" int rgs_scan_order_search(const uint8_t* g, int n, const int32_t* start, uint64_t iterations,
+                                     uint64_t seed, int rmax_cap, int threads, int32_t* best_order,
+                                     uint64_t* best_cost) {
   if (!g || !start || !best_order || n < 2 || n > 128) return fail(kInvalid, "order_search: need 2 <= n <= 128");
+  if (threads < 1 || threads > 256) return fail(kInvalid, "order_search: threads must be 1..256");
   std::string err;
   std::vector<int> pos;
   if (!check_graph(g, n, err) || !check_order(start, n, pos, err)) return fail(kInvalid, err);
   const std::vector<u128> adj = adjacency(g, n);
-  std::vector<int> cur(start, start + n), best, prof, cprof;
-  profile_of(adj, cur, n, prof);
-  uint64_t cur_cost = cost_of(prof);
-  int cur_peak = *std::max_element(prof.begin(), prof.end());
-  best = cur;
-  uint64_t best_c = cur_peak <= rmax_cap ? cur_cost : ~0ull;
-  std::mt19937_64 rng(seed);
-  std::uniform_real_distribution<double> unif(0.0, 1.0);
-  auto t0 = std::chrono::steady_clock::now();
-  auto elapsed = [&]() { return std::chrono::duration<double>(std::chrono::steady_clock::now() - t0).count(); };
-  // Annealing: moves are swap, insertion and segment reversal; the
-  // temperature decays geometrically over the budget, relative to the cost.
-  uint64_t it = 0;
-  while (true) {
-    if ((it & 63) == 0 && elapsed() >= seconds) break;
-    ++it;
-    double frac = std::min(1.0, elapsed() / std::max(seconds, 1e-9));
-    double temp = 0.05 * std::pow(1e-3, frac);
-    std::vector<int> cand = cur;
-    int i = (int)(rng() % n), j = (int)(rng() % n);
-    if (i == j) continue;
-    if (i > j) std::swap(i, j);
-    int move = (int)(rng() % 3);
-    if (move == 0) std::swap(cand[i], cand[j]);
-    else if (move == 1) std::rotate(cand.begin() + i, cand.begin() + j, cand.begin() + j + 1);
-    else std::reverse(cand.begin() + i, cand.begin() + j + 1);
-    cprof = prof;
-    profile_of(adj, cand, n, cprof, i);
-    int peak = *std::max_element(cprof.begin(), cprof.end());
-    uint64_t c = cost_of(cprof);
-    double diff = (double)c - (double)cur_cost;
-    bool take = diff <= 0 || unif(rng) < std::exp(-diff / (temp * (double)cur_cost));
-    if (peak > rmax_cap && cur_peak <= rmax_cap) take = false;
-    if (take) {
-      cur.swap(cand);
-      prof.swap(cprof);
-      cur_cost = c;
-      cur_peak = peak;
-      if (peak <= rmax_cap && c < best_c) {
-        best_c = c;
-        best = cur;
-      }
-    }
-  }
-  if (best_c == ~0ull) return fail(kInvalid, "no order within the rank cap found");
-  for (int k = 0; k < n; ++k) best_order[k] = best[k];
-  if (best_cost) *best_cost = best_c;
+  const std::vector<int> st(start, start + n);
+  std::vector<Chain> best(threads);
+  std::vector<std::thread> pool;
+  for (int t = 0; t < threads; ++t)
+    pool.emplace_back([&, t]() {
+      run_chain(adj, st, n, iterations, seed + 0x9E3779B97F4A7C15ull * (uint64_t)t, rmax_cap, best[t]);
+    });
+  for (auto& th : pool) th.join();
+  int win = 0;
+  for (int t = 1; t < threads; ++t)
+    if (best[t].cost < best[win].cost) win = t;          // ties: the lowest chain
+  if (best[win].cost == ~0ull) return fail(kInvalid, "no order within the rank cap found");
+  for (int k = 0; k < n; ++k) best_order[k] = best[win].order[k];
+  if (best_cost) *best_cost = best[win].cost;
   return kOk;
 }
 

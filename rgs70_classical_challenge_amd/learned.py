@@ -24,7 +24,7 @@ def index_to_bits(idx, n):
 
 def run(n=34, seed=20261205, train_samples=200_000, steps=20_000, batch=512, hidden=512, cap=8.0,
         caps_learned=(1.0, 1.2, 1.4, 1.5, 1.6, 1.7, 1.8, 1.9, 2.0, 2.2, 2.5, 4.0, 8.0), accept_target=300_000,
-        proposal_batch=65536, lr=1e-3, order_seconds=5.0, analysis_resamples=500, log=print):
+        proposal_batch=65536, lr=1e-3, order_iterations=1000000, analysis_resamples=500, log=print):
     import torch
     from .engine import GraphStateSampler
     from .records import capped_statistics
@@ -68,7 +68,7 @@ def run(n=34, seed=20261205, train_samples=200_000, steps=20_000, batch=512, hid
     out['heldout_ceiling_nats'] = float(np.mean(np.log(held_p * 2.0 ** n)))
 
     # 3. rejection, judged by the exact scan and audited in complex128
-    order, _ = order_search(g, seconds=order_seconds, rmax_cap=12)
+    order, _ = order_search(g, iterations=order_iterations, rmax_cap=12)
     scorer = ScanOracle(g, u, order)
     auditor = ScanOracle(g, u, order, fp64=True)
     out['scan_rmax'] = scorer.rmax

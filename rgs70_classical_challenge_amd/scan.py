@@ -25,7 +25,8 @@ _V = ctypes.c_void_p
 _lib.register('rgs_scan_compile', ctypes.c_int, [_V, ctypes.c_int, _V, _V, _V])
 _lib.register('rgs_scan_profile', ctypes.c_int, [_V, ctypes.c_int, _V, _V])
 _lib.register('rgs_scan_order_search', ctypes.c_int,
-              [_V, ctypes.c_int, _V, ctypes.c_double, ctypes.c_uint64, ctypes.c_int, _V, ctypes.POINTER(ctypes.c_uint64)])
+              [_V, ctypes.c_int, _V, ctypes.c_uint64, ctypes.c_uint64, ctypes.c_int, ctypes.c_int, _V,
+               ctypes.POINTER(ctypes.c_uint64)])
 _lib.register('rgs_scan_pack', ctypes.c_int, [_V, ctypes.c_int, _V, ctypes.POINTER(ctypes.c_uint64)])
 _lib.register('rgs_scan_batch', ctypes.c_int,
               [_V, ctypes.c_int, _V, _V, ctypes.c_int, ctypes.c_int, _V, _V, _V])
@@ -52,15 +53,17 @@ def profile(g, order):
     return out.tolist()
 
 
-def order_search(g, start=None, seconds=5.0, seed=20261030, rmax_cap=62):
-    """Annealed order minimising sum 2^r (max r <= rmax_cap).  Returns (order, cost)."""
+def order_search(g, start=None, iterations=200000, seed=20261030, rmax_cap=62, threads=1):
+    """Annealed order minimising sum 2^r (max r <= rmax_cap): `threads`
+    chains of `iterations` moves each, deterministic for given arguments.
+    Returns (order, cost)."""
     g = _graph(g)
     n = len(g)
     s = np.ascontiguousarray(np.arange(n, dtype=np.int32) if start is None else np.asarray(start, np.int32))
     best = np.zeros(n, np.int32)
     cost = ctypes.c_uint64()
-    call('rgs_scan_order_search', ptr(g), n, ptr(s), float(seconds), int(seed), int(rmax_cap), ptr(best),
-         ctypes.byref(cost))
+    call('rgs_scan_order_search', ptr(g), n, ptr(s), int(iterations), int(seed), int(rmax_cap), int(threads),
+         ptr(best), ctypes.byref(cost))
     return best.tolist(), int(cost.value)
 
 
@@ -250,7 +253,7 @@ def main(argv):
     precision = int(argv[1]) if len(argv) > 1 else 64
     seed = int(argv[2]) if len(argv) > 2 else 70
     g, u = instance(70, seed)
-    order, _ = order_search(g, seconds=5.0, rmax_cap=28)
+    order, _ = order_search(g, iterations=1000000, rmax_cap=28)
     so = ScanOracle(g, u, order)
     bits = np.random.default_rng(seed).integers(0, 2, (count, 70), dtype=np.uint8)
     words = torch.as_tensor(bits_to_words(bits).view(np.int64)).to(so.device)

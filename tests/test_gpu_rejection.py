@@ -33,7 +33,7 @@ def test_uniform_and_learned_rejection_match_closed_forms():
     g, u = instance(n, 4242)
     psi = gs.full_state(g, u)
     p = np.abs(psi) ** 2
-    order, _ = order_search(g, seconds=0.5)
+    order, _ = order_search(g, iterations=100000)
     scorer, auditor = ScanOracle(g, u, order), ScanOracle(g, u, order, fp64=True)
     # uniform proposals
     cap = 4.0

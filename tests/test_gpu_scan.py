@@ -63,7 +63,7 @@ def test_config3_instance_against_oracle_sweep():
     complex64 within 1e-5 relative for >= 99% of uniform proposals."""
     from rgs70_classical_challenge_amd.scan import ScanOracle, order_search
     g, u = instance(70, 70)
-    order, _ = order_search(g, seconds=5.0, rmax_cap=12)
+    order, _ = order_search(g, iterations=1000000, rmax_cap=12)
     so64 = ScanOracle(g, u, order)
     so128 = ScanOracle(g, u, order, fp64=True)
     assert so64.rmax <= 12
@@ -89,7 +89,7 @@ def test_rank6_kernel_matches_general_kernel(count):
     from rgs70_classical_challenge_amd.scan import ScanOracle, order_search, bits_to_words
     from rgs70_classical_challenge_amd._lib import call, ptr
     g, u = instance(34, 20261205)
-    order, _ = order_search(g, seconds=2.0, rmax_cap=6)
+    order, _ = order_search(g, iterations=400000, rmax_cap=6)
     rng = np.random.default_rng(count)
     bits = rng.integers(0, 2, (count, 34), dtype=np.uint8)
     for fp64 in (False, True):

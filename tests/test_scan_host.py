@@ -40,7 +40,7 @@ def test_profile_matches_dense_rank(golden_dir):
 
 def test_pack_layout():
     g, _ = instance(34, 3)
-    order, _ = scan.order_search(g, seconds=0.5)
+    order, _ = scan.order_search(g, iterations=100000)
     plan = scan.ScanPlan(g, order)
     blob = plan.packed()
     assert blob[0] == 34 and blob[1] == plan.rmax
@@ -100,7 +100,7 @@ def test_order_search_improves_and_respects_cap():
     g, _ = instance(70, 1234)
     start = list(range(70))
     base = sum(1 << r for r in scan.profile(g, start))
-    order, cost = scan.order_search(g, start, seconds=2.0, rmax_cap=20)
+    order, cost = scan.order_search(g, start, iterations=400000, rmax_cap=20)
     prof = scan.profile(g, order)
     assert sorted(order) == start
     assert cost == sum(1 << r for r in prof) and max(prof) <= 20
@@ -175,7 +175,7 @@ def test_packed_plan_halves_and_sum():
             g = a | a.T
             z = rng.normal(size=(n, 2, 2)) + 1j * rng.normal(size=(n, 2, 2))
             u = np.stack([np.linalg.qr(m)[0] for m in z])
-        order = list(range(n)) if p_edge else scan.order_search(g, seconds=0.3)[0]
+        order = list(range(n)) if p_edge else scan.order_search(g, iterations=60000)[0]
         plan = scan.ScanPlan(g, order)
         blob = plan.packed()
         steps, _, _ = osc.compile_plan(g, order)
