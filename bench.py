@@ -25,8 +25,10 @@ Reported beside the headline value:
   scan          batched rank-scan p(x) of 65,536 device-resident uniform 70-bit
                 proposals of a 70-vertex 3-regular instance per launch (FP64
                 sums), proposals/s with complex64 tables (the scorer) and
-                complex128 tables (the audit), and the complex64-vs-complex128
-                error.
+                complex128 tables (the audit), the complex64-vs-complex128
+                error, and the reported p(x): complex64 tables with the
+                proposals of 2^n p < 0.1 rescored in complex128, its rate and
+                its error against complex128.
   scan_rgs70    the rank-30 scan of the reference's own 70-qubit instance
                 (HBM tables): seconds per amplitude, complex64 and complex128.
   production    the reference's production conditional sampler on its own
@@ -38,9 +40,14 @@ Reported beside the headline value:
                 exact samples, both proposal streams are judged by the exact
                 scan and audited in complex128; accepted samples/s of each at
                 FP64-audited linear XEB equal within one standard error.
+  batch         config 4: 256 uneven instances (n in 16..30), 1,000 samples
+                each, as one graph launch against one launch sequence per
+                instance, and the launch-bound n <= 22 part alone.
   cpu_baseline  the oracle (NumPy restatement of the reference's CPU
-                full-state path) on one host core: n=26 state + mass pyramid +
-                10,000 draws, extrapolated to n=34 by 2^8 * 34/26.
+                full-state path) on the host at n=26 - state, mass pyramid,
+                10,000 draws - on every core this process may use (state
+                passes split over threads) and on one core, extrapolated to
+                n=34 (state n 2^n, pyramid 2^n, draws n).
 
 `--calibrate-copy` instead runs one plain device copy of 2^32 bytes (torch
 copy_), the reference kernel for scaling rocprofv3 FETCH_SIZE / WRITE_SIZE.
@@ -213,11 +220,26 @@ def run_scan(torch, count=65536, seed=70, reps=5):
         out['ms_per_launch' + key] = round(ms, 3)
         out['proposals_per_s' + key] = round(count / ms * 1e3, 1)
     out['table_gbps'] = round(so.plan.table_bytes(64) * count / out['ms_per_launch'] / 1e6, 1)
-    p64 = so.probabilities(bits)
-    p128 = so.probabilities(bits, precision=128)
-    rel = np.abs(p64 - p128) / p128
+    # the reported p(x): complex64 tables, proposals with 2^n p < 0.1 rescored
+    # with complex128 tables (a second launch over the flagged ones)
+    so.probabilities_checked(words)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        p, flag = so.probabilities_checked(words)
+    torch.cuda.synchronize()
+    ms = (time.perf_counter() - t0) / reps * 1e3
+    out['ms_per_launch_checked'] = round(ms, 3)
+    out['proposals_per_s_checked'] = round(count / ms * 1e3, 1)
+    out['rescored_fraction'] = round(float(flag.float().mean().item()), 5)
+    p128 = so.probabilities_device(words, 128)
+    p64 = so.probabilities_device(words, 64)
+    rel = (torch.abs(p64 - p128) / p128).cpu().numpy()
+    relc = (torch.abs(p - p128) / p128).cpu().numpy()
     out['c64_vs_c128_rel_err'] = {'median': float(np.median(rel)), 'p99': float(np.quantile(rel, 0.99)),
                                   'max': float(rel.max())}
+    out['reported_vs_c128_rel_err'] = {'median': float(np.median(relc)), 'p99': float(np.quantile(relc, 0.99)),
+                                       'max': float(relc.max())}
     return out
 
 
@@ -269,6 +291,23 @@ def run_learned(torch, n=34):
     return out
 
 
+def run_batch(torch, count=256):
+    """Config 4: 256 instances with n uniform in 16..30 (3-regular for even n,
+    Erdos-Renyi G(n, 3/n) for odd n), 1,000 samples each, tile plans from the
+    autotuner's table: the whole batch as one graph launch against one launch
+    sequence per instance, and the same for its launch-bound n <= 22 part."""
+    from rgs70_classical_challenge_amd.batch import BatchSampler, uneven_instances
+    specs = uneven_instances(count, nmin=16, nmax=30)
+    res = BatchSampler(specs, samples=1000).benchmark()
+    res['workload'] = f'{count} instances, n uniform in 16..30, 1000 samples each'
+    small = [sp for sp in specs if sp[0] <= 22]
+    sres = BatchSampler(small, samples=1000).benchmark()
+    res['small_instances'] = {'n_max': 22, 'instances': len(small),
+                              **{k: sres[k] for k in ('eager_ms', 'graph_ms', 'single_graph_ms', 'graph_speedup',
+                                                      'single_graph_speedup')}}
+    return res
+
+
 def run_production(torch, count=6):
     """The reference's production workload on one device: the 70-qubit
     instance's 2^35-amplitude conditional state per sample, replaying its
@@ -287,21 +326,48 @@ def run_production(torch, count=6):
 
 
 def cpu_baseline(seed):
-    """Oracle (NumPy) on one host core: bounded sample at n=26, extrapolated to n=34."""
+    """The oracle (NumPy restatement of the reference's CPU full-state path,
+    graph_sampler.py:71-93) on the host: the n=26 state (graph phase, then one
+    2x2 gate pass per qubit), its FP64 pyramid and 10,000 draws, on one core
+    and with the state passes split over every core this process may use.
+    Extrapolated to n=34: the state and pyramid work scale as n 2^n and 2^n,
+    the draws (one descent each) as n."""
     from oracle import draw as od
     from oracle import graph_state as gs
     from rgs70_classical_challenge_amd.instances import instance
-    n0, samples = 26, 10000
+    n0, samples, n = 26, 10000, 34
+    cores = len(os.sched_getaffinity(0))
     g, u = instance(n0, seed)
+    uni = np.random.default_rng(1).random(samples)
     t0 = time.perf_counter()
     v = gs.full_state(g, u)
+    t_state = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    vt = gs.full_state_threads(g, u, cores)
+    t_state_mt = time.perf_counter() - t0
+    assert np.array_equal(v, vt)
+    del vt
+    t0 = time.perf_counter()
     levels = od.pyramid(od.probabilities(v))
-    od.tree_draws(levels, np.random.default_rng(1).random(samples))
-    dt = time.perf_counter() - t0
-    scale = 2.0 ** (34 - n0) * 34 / n0
-    return {'value': round(samples / (dt * scale), 4), 'unit': 'samples/s', 'cores': 1, 'kind': 'port',
-            'sample': f'oracle full state + pyramid + {samples} draws at n={n0} took {dt:.1f} s on 1 core; '
-                      f'extrapolated x2^{34 - n0}*34/{n0} to n=34 (state work is n*2^n)'}
+    t_pyr = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    od.tree_draws(levels, uni)
+    t_draw = time.perf_counter() - t0
+    # one write pass (16 B per complex128 amplitude) and n0 read+write gate passes
+    pass_bytes = (16 + 32 * n0) * (1 << n0)
+    grow = 2.0 ** (n - n0)
+
+    def per_s(ts):
+        return samples / (ts * grow * n / n0 + t_pyr * grow + t_draw * n / n0)
+
+    one = {'value': round(per_s(t_state), 4), 'cores': 1, 'state_seconds': round(t_state, 3),
+           'state_gbps_complex128': round(pass_bytes / t_state / 1e9, 2)}
+    return {'value': round(per_s(t_state_mt), 4), 'unit': 'samples/s', 'cores': cores, 'kind': 'port',
+            'sample': f'oracle at n={n0}: state {t_state_mt:.2f} s on {cores} threads ({t_state:.2f} s on 1), '
+                      f'pyramid {t_pyr:.2f} s, {samples} draws {t_draw:.2f} s; extrapolated to n={n} '
+                      f'(state x2^{n - n0}*{n}/{n0}, pyramid x2^{n - n0}, draws x{n}/{n0})',
+            'state_gbps_complex128': round(pass_bytes / t_state_mt / 1e9, 2), 'one_core': one,
+            'pyramid_seconds': round(t_pyr, 3), 'draw_seconds': round(t_draw, 3)}
 
 
 def calibrate_copy(torch):
@@ -400,6 +466,12 @@ def main():
         except Exception as e:
             out['scan_rgs70'] = {'error': repr(e)}
         log('[bench] rank-30 scan done')
+        torch.cuda.empty_cache()
+        try:
+            out['batch'] = run_batch(torch)
+        except Exception as e:
+            out['batch'] = {'error': repr(e)}
+        log('[bench] uneven batch done')
         torch.cuda.empty_cache()
         try:
             out['production'] = run_production(torch)

@@ -262,7 +262,9 @@ int rgs_scan_pack(const rgs_scan_step *steps, int n, uint64_t *blob, uint64_t *w
 /* p(x) of `count` proposals in one persistent launch.  plan_dev: packed plan;
  * rmax: its largest rank, max(r_in, r_out) over the steps (the LDS tables and
  * the kernel, one wavefront per four proposals at rmax <= 6, are chosen by
- * it; a larger value is allowed, a smaller one is not);
+ * it; a larger value is allowed; when the plan's own rank, word 1 of the
+ * blob, exceeds what the chosen kernel holds, the kernel reads no table and
+ * marks every output: value NaN, exponent INT32_MIN);
  * u_dev: complex128 [n][2][2] measurement unitaries (f_v = u[v][x_v][:]);
  * bits_dev: uint64 [count][2] bitstrings (qubit q = bit q).  precision 64
  * stores the tables as complex64 (rmax <= 13), 128 as complex128

@@ -87,10 +87,28 @@ def pyramid(p):
     return levels
 
 
+def descend(levels, t, top, idx=0):
+    """Descend from node ``idx`` of level ``top`` to level 0 with target t.
+
+    At a node with children (L, R): go right iff t >= L and R > 0, then t -= L.
+    Returns (index at level 0, remaining t, smallest |t - L| met on the path).
+    """
+    margin = np.inf
+    for lev in range(top - 1, -1, -1):
+        left = levels[lev][2 * idx]
+        right = levels[lev][2 * idx + 1]
+        margin = min(margin, abs(t - left))
+        if t >= left and right > 0:
+            t -= left
+            idx = 2 * idx + 1
+        else:
+            idx = 2 * idx
+    return idx, t, margin
+
+
 def tree_draw(levels, u):
     """Descend the pyramid with target t = u * root.
 
-    At a node with children (L, R): go right iff t >= L and R > 0, then t -= L.
     Returns (index, margin) where the margin is the smallest |t - L| met on
     the path, in units of total mass (absolute).  If two mass pyramids differ
     in their leaves by delta in L1 norm, every decision agrees whenever the
@@ -98,19 +116,27 @@ def tree_draw(levels, u):
     """
     top = len(levels) - 1
     root = levels[top][0]
-    t = u * root
-    idx = 0
-    margin = np.inf
-    for lev in range(top - 1, -1, -1):
-        left = levels[lev][2 * idx]
-        right = levels[lev][2 * idx + 1]
-        margin = min(margin, abs(t - left) / root)
-        if t >= left and right > 0:
-            t -= left
-            idx = 2 * idx + 1
-        else:
-            idx = 2 * idx
-    return idx, margin
+    idx, _, margin = descend(levels, u * root, top)
+    return idx, margin / root
+
+
+def split_draws(upper, fetch_block, block_bits, uniforms):
+    """The same descent when only the pyramid's block level and above
+    (``upper``, upper[0] = masses of the aligned 2^block_bits blocks) are at
+    hand: descend them to a block, then descend the pairwise pyramid of that
+    block's leaf values, ``fetch_block(blocks)`` -> float64 [len(blocks)][2^block_bits].
+    Equal to ``tree_draws`` on the full pyramid whenever upper[0] holds the
+    pairwise sums of the leaves."""
+    top = len(upper) - 1
+    root = upper[top][0]
+    firsts = [descend(upper, float(u) * root, top) for u in uniforms]
+    blocks = np.array([b for b, _, _ in firsts], np.int64)
+    leaves = fetch_block(blocks)
+    out = np.empty(len(uniforms), np.int64)
+    for k, (b, t, _) in enumerate(firsts):
+        i, _, _ = descend(pyramid(leaves[k]), t, block_bits)
+        out[k] = (b << block_bits) | i
+    return out
 
 
 def tree_draws(levels, uniforms):

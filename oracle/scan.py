@@ -184,5 +184,52 @@ def amplitude(steps, u, bits):
     return val * math.exp(ls) * 2.0 ** (-n / 2)
 
 
+def sweep_batch(steps, f0, f1):
+    """``sweep`` for many proposals at once: f0[b][v], f1[b][v] are the two
+    weights of vertex v for proposal b (complex arrays [B][n]).  The same
+    table walk and the same per-proposal max-modulus renormalisation as
+    ``sweep``, vectorised over proposals.  Returns (values [B], logscale [B])."""
+    f0 = np.asarray(f0, np.complex128)
+    f1 = np.asarray(f1, np.complex128)
+    nb = f0.shape[0]
+    E = np.ones((nb, 1), np.complex128)
+    logscale = np.zeros(nb)
+    for st in steps:
+        T = byte_tables(st.lcols, st.r_out)
+        cp = np.arange(1 << st.r_out, dtype=np.uint64)
+        p0 = np.zeros_like(cp)
+        for b in range(T.shape[0]):
+            p0 ^= T[b][((cp >> np.uint64(8 * b)) & np.uint64(255)).astype(np.int64)]
+        mask = np.uint64((1 << st.r_in) - 1)
+        beta = np.uint64(st.beta)
+        acc = np.zeros((nb, 1 << st.r_out), np.complex128)
+        w0 = f0[:, st.v][:, None]
+        w1 = f1[:, st.v][:, None]
+        for kc in kernel_combos(st.kernel):
+            p = p0 ^ np.uint64(kc)
+            c = (p & mask).astype(np.int64)
+            y = ((p >> np.uint64(st.r_in)) & np.uint64(1)).astype(bool)
+            par = (np.bitwise_count(p & mask & beta) & np.uint8(1)).astype(bool)
+            sgn = np.where(y & par, -1.0, 1.0)
+            acc += E[:, c] * np.where(y[None, :], w1, w0) * sgn[None, :]
+        E = acc
+        mx = np.max(np.abs(E), axis=1)
+        pos = mx > 0
+        E[pos] /= mx[pos, None]
+        logscale[pos] += np.log(mx[pos])
+    return E[:, 0].copy(), logscale
+
+
+def amplitudes(steps, u, bits):
+    """<x| (x)u |G> for each row of ``bits`` ([B][n], qubit 0 first)."""
+    bits = np.asarray(bits, np.int64)
+    n = bits.shape[1]
+    q = np.arange(n)
+    f0 = np.asarray(u)[q[None, :], bits, 0]
+    f1 = np.asarray(u)[q[None, :], bits, 1]
+    val, ls = sweep_batch(steps, f0, f1)
+    return val * np.exp(ls) * 2.0 ** (-n / 2)
+
+
 def plan_cost(profile):
     return sum(1 << r for r in profile)

@@ -116,6 +116,7 @@ def tune_engine(eng, label, reps=3, extra_stage=True):
             best = (dict(cand), ms)
 
     for cand in candidates(n):
+        print(f'[autotune] {label} timing {cand}', file=sys.stderr, flush=True)
         run(cand)
     if extra_stage and best is not None and n > best[0]['low_bits']:
         # a narrow phase group (four bits need no LDS exchange), the rest even
@@ -141,9 +142,19 @@ def tune_engine(eng, label, reps=3, extra_stage=True):
     return best, report
 
 
+# Largest plain-pipeline size the tuner runs.  A plain n = 35 tuning run on
+# one MI355X went silent before its first candidate finished and was stopped;
+# its cause was not established, so that size stays refused.  The conditional
+# pipeline's m = 35 entry ('c35') is tuned by tune_conditional.
+PLAIN_MAX_BITS = 34
+
+
 def tune(n, reps=3, seed=7):
     """Tune the plain pipeline (graph-phase init, Walsh, gates, masses) of a
-    seeded n-qubit instance; returns ((best tuning, ms), report)."""
+    seeded n-qubit instance; returns ((best tuning, ms), report).  Sizes above
+    PLAIN_MAX_BITS are refused."""
+    if n > PLAIN_MAX_BITS:
+        raise ValueError(f'the plain-pipeline tuner runs n <= {PLAIN_MAX_BITS} (n = {n} requested)')
     import torch
     from .engine import GraphStateSampler
     from .instances import instance

@@ -10,6 +10,8 @@
 #include "rgs_amd.h"
 
 #include <algorithm>
+#include <mutex>
+#include <string>
 
 namespace rgs {
 
@@ -153,11 +155,23 @@ __device__ __forceinline__ uint32_t scan_outputs(const S* E, const ScanStepArgs&
 // the outputs overwrite the table in place.  W waves per proposal keep the
 // LDS per proposal while giving each CU W times the wavefronts to hide LDS
 // and FP64 latency, and need W times fewer output registers per lane.
+// Outputs of a launch whose plan exceeds the kernel's rank: NaN values and
+// the smallest exponent, written by the first thread of each workgroup over
+// a grid stride (the host reads NaN as "plan rank above the rmax passed").
+__device__ inline void reject_outputs(int count, double2* out_val, int32_t* out_exp) {
+  if (threadIdx.x != 0) return;
+  for (int i = blockIdx.x; i < count; i += gridDim.x) {
+    out_val[i] = make_double2(__builtin_nan(""), __builtin_nan(""));
+    out_exp[i] = INT32_MIN;
+  }
+}
+
 template <typename S, int K, int W>
 __global__ void __launch_bounds__(64 * W) k_scan_batch(const uint64_t* __restrict__ plan,
                                                       const double2* __restrict__ u,
                                                       const uint64_t* __restrict__ bits, int count,
-                                                      double2* __restrict__ out_val, int32_t* __restrict__ out_exp) {
+                                                      double2* __restrict__ out_val, int32_t* __restrict__ out_exp,
+                                                      int rcap) {
   static_assert((K & (K - 1)) == 0 && (W & (W - 1)) == 0, "K and W are powers of two");
   constexpr int KB = K > 1 ? __builtin_ctz(K) : 0;   // Gray steps within the wave's blocks
   extern __shared__ __align__(16) unsigned char smem[];
@@ -166,6 +180,10 @@ __global__ void __launch_bounds__(64 * W) k_scan_batch(const uint64_t* __restric
   const int lane = threadIdx.x & 63;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);   // uniform: plan reads stay scalar
   const int n = (int)plan[0];
+  if ((int)plan[1] > rcap) {   // the plan's tables would not fit: mark every output, touch no table
+    reject_outputs(count, out_val, out_exp);
+    return;
+  }
   for (int pidx = blockIdx.x; pidx < count; pidx += gridDim.x) {
     const uint64_t x0 = bits[2 * pidx], x1 = bits[2 * pidx + 1];
     if (threadIdx.x == 0) E[0] = make_s<S>(1.0, 0.0);
@@ -251,7 +269,8 @@ static hipError_t launch_scan(const uint64_t* plan, const double2* u, const uint
     if (e != hipSuccess) return e;
     attr = true;
   }
-  hipLaunchKernelGGL((k_scan_batch<S, K, W>), dim3(grid), dim3(64 * W), shmem, st, plan, u, bits, count, val, ex);
+  hipLaunchKernelGGL((k_scan_batch<S, K, W>), dim3(grid), dim3(64 * W), shmem, st, plan, u, bits, count, val, ex,
+                     rmax);
   return hipGetLastError();
 }
 
@@ -343,6 +362,10 @@ __global__ void __launch_bounds__(64 * PW) k_scan_small(const uint64_t* __restri
   uint64_t* xw = reinterpret_cast<uint64_t*>(wb + (size_t)P * 64 * sizeof(S));  // [P][2]
   int* es = reinterpret_cast<int*>(wb + (size_t)P * 64 * sizeof(S) + (size_t)P * 16);   // [P]
   const int n = (int)plan[0];
+  if ((int)plan[1] > 6) {   // this kernel's tables hold rank <= 6
+    reject_outputs(count, out_val, out_exp);
+    return;
+  }
   // Step words: the plan layout at r_out <= 6 is header, beta, 64 lane words,
   // one block word, then 2^nker combinations.
   for (int i = threadIdx.x; i < n * kSmallStepWords; i += 64 * PW) {

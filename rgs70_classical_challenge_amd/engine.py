@@ -377,6 +377,13 @@ class GraphStateSampler:
         self.slab.sync()
 
 
+def _edge_margin(cum, t, k):
+    """Distance of t to the nearer edge of bin k of an inclusive cumulative sum."""
+    lo = cum[k - 1] if k > 0 else 0.0
+    hi = cum[k] if k < len(cum) else cum[-1]
+    return float(min(abs(t - lo), abs(hi - t)))
+
+
 def _check_unitary(m):
     m = np.asarray(m, np.complex128)
     if np.max(np.abs(m.conj().T @ m - np.eye(2))) > 1e-12:
@@ -494,13 +501,21 @@ class ConditionalSampler:
         bit; exactly 3 uniforms when deferred (distributed_sampler.py:153-157,
         deferred_global_sampler.py:72-78)."""
         mass, norm, tb, nmass = self.masses()
-        block = int(np.searchsorted(np.cumsum(mass), rng.random() * norm, side='right'))
+        cm = np.cumsum(mass)
+        t1 = rng.random() * norm
+        block = int(np.searchsorted(cm, t1, side='right'))
         half, chunk = block // nmass, block % nmass
         start = half * self.n + (chunk << tb)
         tile = self._slice(start, min(start + (1 << tb), (half + 1) * self.n))
         p = tile.real.astype(np.float64) ** 2 + tile.imag.astype(np.float64) ** 2
-        within = int(np.searchsorted(np.cumsum(p), rng.random() * p.sum(), side='right'))
+        cp = np.cumsum(p)
+        t2 = rng.random() * p.sum()
+        within = int(np.searchsorted(cp, t2, side='right'))
         index = start + within
+        # distance of each target to the nearer edge of its bin, relative to
+        # the level's total (how far the FP32 state may move before the draw
+        # would pick another bin)
+        self.last_margins = [_edge_margin(cm, t1, block) / norm, _edge_margin(cp, t2, within) / p.sum()]
         if not self.defer_top:
             return index, float(p[within]), norm
         local = index % self.n
@@ -509,7 +524,9 @@ class ConditionalSampler:
         total = float(pp.sum())
         if not np.isfinite(total) or total <= 0:
             raise ArithmeticError('Invalid transformed local mass')
-        bit = int(rng.random() * total >= pp[0])
+        t3 = rng.random() * total
+        bit = int(t3 >= pp[0])
+        self.last_margins.append(abs(t3 - pp[0]) / total)
         return bit * self.n + local, float(pp[bit]), norm
 
     def probability_at(self, index):

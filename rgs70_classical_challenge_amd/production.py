@@ -61,6 +61,7 @@ def replay(g, u, A, B, records, tuning=None, log=None):
         elapsed = time.perf_counter() - t
         p_at = eng.probability_at(int(rec['A_index'])) / norm * 2.0 ** (-m)
         row = dict(sample=rec['sample'], seconds=elapsed, stages=stages, uniforms_used=rng.used,
+                   margins=list(eng.last_margins),
                    index=index, index_matches=index == int(rec['A_index']),
                    bitstring_matches=bitstring_of(A, B, index, rec['B_bits'], len(g)) == rec['bitstring_q0_left'],
                    norm=norm, norm_recorded=rec['conditional_norm'],

@@ -116,7 +116,9 @@ def produce(engine, n, group, out, count=None, meta=None, log=None):
                        elapsed_seconds=time.perf_counter() - start, stages=stages)
             f.write(json.dumps(rec) + '\n')
             f.flush()
-            records.append(rec)
+            # the returned copy also carries the draw's bin-edge margins (not
+            # part of the wire format)
+            records.append(dict(rec, draw_margins=list(getattr(engine, 'last_margins', []))))
             if log:
                 log(f'[records] sample {i}: {rec["elapsed_seconds"]:.3f} s')
         summary = dict(samples=count, group_id=group['group_id'], generation_start_unix=wall0,

@@ -56,16 +56,19 @@ class CappedRejection:
         self.scorer, self.auditor, self.n, self.cap = scorer, auditor, n, cap
         self.proposal = proposal          # a MADE (learned) or None (uniform)
 
-    def run(self, rng, accept_target, batch=8192, max_batches=1000, schedule='batched'):
+    def run(self, rng, accept_target, batch=8192, max_batches=1000, schedule='batched', record_all=False):
         """Draw until `accept_target` outputs are accepted.  Returns a record
         with accepted bitstrings, p (complex64 scan), p_fp64 (complex128),
-        decisions, margins and timing."""
+        decisions, margins and timing; with ``record_all`` also every
+        proposal's bits, uniform, p, p_fp64, acceptance and decision (the
+        reference records all of them, rejection_sampler.py:20-23)."""
         import torch
         n = self.n
         acc_bits, acc_p, acc_p64 = [], [], []
         flips = beyond = 0
         flip_margin = 0.0
         margins = []
+        every = []
         proposals = 0
         gpu_s = 0.0
         t0 = time.perf_counter()
@@ -90,6 +93,8 @@ class CappedRejection:
             if flip.any():
                 flip_margin = max(flip_margin, float(rel[flip].max()))
             margins.append(np.abs(u - acc64))
+            if record_all:
+                every.append((bits, u, p, p64, acc, keep, acc64, keep64))
             proposals += batch
             acc_bits.append(bits[keep])
             acc_p.append(p[keep])
@@ -102,7 +107,11 @@ class CappedRejection:
         p64 = np.concatenate(acc_p64)[:accept_target]
         z = 2.0 ** n * p64
         m = np.concatenate(margins)
-        return dict(accepted=len(bits), proposals=proposals, acceptance=len(bits) / proposals,
+        extra = {}
+        if record_all:
+            names = ('bits', 'u', 'p', 'p_fp64', 'acceptance', 'accepted', 'acceptance_fp64', 'accepted_fp64')
+            extra['all'] = {k: np.concatenate([e[i] for e in every]) for i, k in enumerate(names)}
+        return dict(**extra,accepted=len(bits), proposals=proposals, acceptance=len(bits) / proposals,
                     sample_seconds=gpu_s, wall_seconds=wall, accepted_per_s=len(bits) / gpu_s,
                     xeb=float(z.mean() - 1), xeb_se=float(z.std(ddof=1) / np.sqrt(len(z))),
                     decision_flips_fp64=flips, min_margin_fp64=float(m.min()),

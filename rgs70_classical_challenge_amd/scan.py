@@ -224,6 +224,25 @@ class ScanOracle:
         val, ex = self.scores_device(words_dev, precision)
         return probabilities_from_scores(val, ex, self.n)
 
+    def probabilities_checked(self, words_dev, z_min=0.1):
+        """p(x) to 1e-5 relative for device-resident proposals: every proposal
+        is scored with complex64 tables (FP64 sums), and those whose
+        2^n p(x) falls below ``z_min`` are rescored with complex128 tables in
+        one more launch.  Complex64 table rounding is absolute in the
+        amplitude (about 2^-24 of the step's largest entry), so the relative
+        error of p grows as 1 / sqrt(2^n p): the small-p tail is where the
+        complex64 result can miss 1e-5.  Returns device tensors (p float64,
+        rescored bool)."""
+        torch = self.torch
+        if self.rmax > self.batch_rmax[128]:
+            raise ValueError('the complex128 rescoring needs the batch kernels (rank within their limit)')
+        p = self.probabilities_device(words_dev, 64)
+        flag = p < z_min * 2.0 ** -self.n
+        idx = torch.nonzero(flag).squeeze(1)
+        if idx.numel():
+            p[idx] = self.probabilities_device(words_dev.index_select(0, idx).contiguous(), 128)
+        return p, flag
+
     def amplitudes(self, bits, precision=None):
         val, ex, _ = self.raw(bits, precision)
         v = val.cpu().numpy()

@@ -253,3 +253,26 @@ def test_tuning_table_key_and_entries():
                            phase_rounds=e.get('phase_round_bits', -1), top=e.get('top_bits', 0),
                            order=e.get('group_order', 0))
         assert text
+
+
+def test_plain_tuner_refuses_sizes_above_its_limit():
+    from rgs70_classical_challenge_amd import autotune
+    with pytest.raises(ValueError, match='n <= 34'):
+        autotune.tune(35)
+
+
+def test_bench_plan_is_the_plan_the_gpu_tests_check():
+    """The autotuned n = 34 entry the bench runs equals the tuning the GPU
+    tests run at n = 20, 27 and 34 (so its two k_tile specialisations are the
+    ones checked against the oracle)."""
+    import json
+    import os
+    from test_gpu_headline import HEADLINE
+    table = json.load(open(os.path.join(os.path.dirname(_lib.LIB_PATH), 'tuned.json')))
+    for dev, entries in table.items():
+        e = entries['34']
+        keys = ('low_bits', 'tile_bits', 'lane_bits', 'round_bits', 'group_order')
+        assert {k: e.get(k, 0) for k in keys} == {k: HEADLINE.get(k, 0) for k in keys}, dev
+        assert e.get('top_bits', 0) == 0 and e.get('phase_round_bits', -1) in (-1, 0) and not e.get('dphase_global')
+    n, text = describe(34, walsh=True, low=13, tile=14, lane=6, rounds=0, order=1)
+    assert {tok.split('/', 1)[1] for tok in text.split()} == {'tb13/c0/rb0', 'tb13/c6/rb0'}, text

@@ -37,9 +37,17 @@ def test_rgs70_production_replay(golden_dir):
     # hold complex64 amplitudes (the recorded norms are 1 + 2.3e-7)
     assert all(abs(r['norm'] - 1) < 1e-6 for r in rows), [r['norm'] for r in rows]
     assert summary['max_norm_diff'] < 2e-6
-    # the draw is an inverse CDF over 2^35 bins with FP32 amplitudes: a record may
-    # land within rounding of a bin edge, so allow one index to differ
-    assert summary['index_matches'] >= len(rows) - 1
+    # The draw is an inverse CDF over the FP32 state.  A recorded uniform may
+    # pick another bin only if its target lies within the state's rounding of
+    # a bin edge: the cumulative sums move by at most eps times the level's
+    # total, eps the complex64 p error measured on this instance (the p errors
+    # at the recorded strings, floor 1e-6), and the target u * total by as
+    # much.  Every record whose margins exceed that bound must match.
+    eps = max(summary['max_rel_err_vs_fp64'], 1e-6)
+    for r in rows:
+        print(f"sample {r['sample']}: match {r['index_matches']}, margins {r['margins']}", flush=True)
+        if min(r['margins']) > 2 * eps:
+            assert r['index_matches'], r
     gc.collect()
     torch.cuda.empty_cache()
 
@@ -75,7 +83,11 @@ def test_rgs70_production_records_from_the_seed(golden_dir, tmp_path):
         assert R.same_state(r['rngstate_after'], g['rngstate_after'])
         assert abs(r['conditional_norm'] - 1) < 1e-6
     matches = [r['A_index'] == g['A_index'] for r, g in zip(recs, golden)]
-    assert sum(matches) >= len(golden) - 1, matches
+    # as in the replay above: a record may differ only where a draw margin is
+    # within the complex64 rounding of the state
+    for r, m in zip(recs, matches):
+        if min(r['draw_margins']) > 2e-6:
+            assert m, r
     for r, g, m in zip(recs, golden, matches):
         if m:
             assert r['bitstring_q0_left'] == g['bitstring_q0_left']

@@ -100,3 +100,54 @@ def test_capped_decide_rounds_as_tensor_expressions(learned):
     assert torch.equal(keep.bool(), expect)
     assert torch.equal(expect, half & (acc > 0))
     assert int(accepted.item()) == int(expect.sum().item())
+
+
+def test_capped_rejection_replays_reference_stream(golden_dir):
+    """The reference's cap-1 run on its 70-qubit instance (seed 314159,
+    rejection_sampler.py:18-24) replayed through CappedRejection with the
+    reference's call schedule: complex64-table scorer and complex128-table
+    auditor (rank-30 HBM scan).  Against the 64 recorded proposals
+    (cap1_archive, cap1_full_audit_artifact): the same bitstrings and
+    uniforms, p_fp64 to 1e-9, the complex64 p and the acceptance within 1e-5
+    relative, and the same decision wherever the FP64 margin exceeds that
+    tolerance (audit_all_proposals_fp64.py:43-50)."""
+    import json
+    _torch()
+    from rgs70_classical_challenge_amd.rejection import CappedRejection, FLIP_TOLERANCE
+    from rgs70_classical_challenge_amd.scan import ScanOracle
+    d = np.load(f'{golden_dir}/rgs_graph.npz')
+    g, u = d['graph'], d['measurement_unitaries']
+    fixture = json.load(open(f'{golden_dir}/rgs70_cap1_proposals.json'))
+    meta, rows = fixture['metadata'], fixture['proposals']
+    order = np.load(f'{golden_dir}/scan_order_optimized.npy').tolist()
+    scorer, auditor = ScanOracle(g, u, order), ScanOracle(g, u, order, fp64=True)
+    cr = CappedRejection(scorer, auditor, meta['n'], meta['cap'])
+    r = cr.run(np.random.default_rng(meta['seed']), 10 ** 9, batch=len(rows), max_batches=1, schedule='reference',
+               record_all=True)
+    a = r['all']
+    assert len(a['u']) == len(rows)
+    for k, row in enumerate(rows):
+        assert ''.join(map(str, a['bits'][k])) == row['bitstring_q0_left'], k
+        assert a['u'][k] == row['acceptance_uniform'], k
+    p_fp64 = np.array([row['p_fp64'] for row in rows])
+    p_rec = np.array([row['probability_raw'] for row in rows])
+    acc_rec = np.array([row['acceptance_probability'] for row in rows])
+    keep_rec = np.array([row['accepted'] for row in rows])
+    margin = np.array([row['margin_fp64'] for row in rows])
+    rel128 = np.abs(a['p_fp64'] - p_fp64) / p_fp64
+    rel64 = np.abs(a['p'] - p_fp64) / p_fp64
+    # acceptance against the FP64 acceptance of the recorded complex128 p (the
+    # recorded acceptance itself came from the reference's complex64 p)
+    acc_fp64 = np.minimum(2.0 ** meta['n'] * p_fp64 / meta['cap'], 1.0)
+    relacc = np.abs(a['acceptance'] - acc_fp64) / acc_fp64
+    print(f'complex128 p max rel {rel128.max():.2e}; complex64 p max rel {rel64.max():.2e} '
+          f'(the reference\'s complex64: {np.max(np.abs(p_rec - p_fp64) / p_fp64):.2e}); '
+          f'acceptance max rel {relacc.max():.2e}; accepted {int(a["accepted"].sum())} of {len(rows)}', flush=True)
+    assert rel128.max() < 1e-9
+    assert rel64.max() < 1e-5
+    assert relacc.max() < 1e-5
+    assert np.max(np.abs(acc_rec - acc_fp64) / acc_fp64) < 1e-4
+    sure = margin > FLIP_TOLERANCE * acc_fp64
+    assert np.array_equal(a['accepted'][sure], keep_rec[sure])
+    assert np.array_equal(a['accepted_fp64'][sure], keep_rec[sure])
+    assert r['decision_flips_beyond_tolerance'] == 0

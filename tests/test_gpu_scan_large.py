@@ -80,4 +80,4 @@ def test_rgs70_recorded_amplitudes(golden_dir):
     p64 = ScanOracle(g, u, order).probabilities(bits)
     rel64 = np.abs(p64 - want_p) / want_p
     print('complex64 tables: median / max relative p error', np.median(rel64), rel64.max(), flush=True)
-    assert rel64.max() < 1e-4
+    assert rel64.max() < 1e-5

@@ -75,6 +75,9 @@ def check_pyramid_and_draws(sampler, ref, uniforms):
     assert np.array_equal(idx[ok], ref_idx[ok])
 
 
+# The autotuned n = 34 entry (tile_bits 14 narrows to 13 at that size).
+HEADLINE = dict(low_bits=13, tile_bits=14, lane_bits=6, round_bits=0, group_order=1)
+
 TUNINGS = [None, dict(low_bits=8, tile_bits=8, lane_bits=3), dict(low_bits=10, tile_bits=9, lane_bits=2),
            dict(low_bits=12, tile_bits=12, lane_bits=5),
            # LDS rounds in the strided passes (engaged where the group is wide enough)
@@ -85,7 +88,11 @@ TUNINGS = [None, dict(low_bits=8, tile_bits=8, lane_bits=3), dict(low_bits=10, t
            # a four-bit phase group (no LDS exchange in that pass)
            dict(low_bits=8, tile_bits=12, lane_bits=4, top_bits=4),
            # groups walked from the high bits down (the phase in the lowest group)
-           dict(low_bits=8, tile_bits=10, lane_bits=3, group_order=1)]
+           dict(low_bits=8, tile_bits=10, lane_bits=3, group_order=1),
+           # the n = 34 plan's shape: 13-bit contiguous tiles, 13-bit strided
+           # tiles with 64-amplitude rows and no LDS rounds, groups walked from
+           # the high bits down (k_tile<13, 0, 0> and k_tile<13, 6, 0>)
+           HEADLINE]
 
 
 @pytest.mark.parametrize('pipeline', ['slab', 'literal', 'folded'])
@@ -199,8 +206,12 @@ def test_conditional_sampler_matches_oracle(golden_dir, name, tuning, weight_bit
         i1, p1, n1 = eng.draw(np.random.default_rng(seed))
         i2, p2, n2 = od.reference_draw(raw, np.random.default_rng(seed), terminal=u[A[-1]])
         assert i1 == i2
-        assert p1 == pytest.approx(p2, rel=1e-4)
-        assert eng.probability_at(i1) == pytest.approx(abs(final[i1]) ** 2, rel=1e-4)
+        # 1e-5 relative wherever p >= 0.1 / 2^m (complex64 rounding is
+        # absolute in the amplitude, so below that the bar is absolute)
+        floor = 0.1 * 2.0 ** -len(A)
+        assert abs(p1 - p2) <= 1e-5 * max(p2, floor), (p1, p2)
+        q = abs(final[i1]) ** 2
+        assert abs(eng.probability_at(i1) - q) <= 1e-5 * max(q, floor), (eng.probability_at(i1), q)
 
 
 def test_conditional_sampler_errors(golden_dir):

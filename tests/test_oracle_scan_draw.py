@@ -178,3 +178,31 @@ def test_instances_are_seeded_and_valid():
         if n % 2 == 0:
             assert np.all(g1.sum(1) == 3)
         assert np.allclose(u1 @ np.conj(np.transpose(u1, (0, 2, 1))), np.eye(2), atol=1e-14)
+
+
+def test_split_draws_equal_full_descent():
+    """Descending the block level and then each block's own pairwise pyramid
+    picks the same index as descending the full pyramid."""
+    rng = np.random.default_rng(44)
+    p = rng.exponential(size=1 << 12) * 2.0 ** -12
+    p[rng.integers(0, 1 << 12, 50)] = 0.0
+    levels = od.pyramid(p)
+    uni = rng.random(400)
+    full, _ = od.tree_draws(levels, uni)
+    for bb in (1, 4, 8):
+        got = od.split_draws(levels[bb:], lambda blocks: p.reshape(-1, 1 << bb)[blocks], bb, uni)
+        assert np.array_equal(got, full)
+
+
+def test_batched_sweep_equals_single_sweep():
+    from rgs70_classical_challenge_amd.instances import instance
+    g, u = instance(14, 3)
+    order = np.random.default_rng(1).permutation(14).tolist()
+    steps, _, _ = osc.compile_plan(g, order)
+    bits = np.random.default_rng(2).integers(0, 2, (40, 14))
+    got = osc.amplitudes(steps, u, bits)
+    one = np.array([osc.amplitude(steps, u, r.tolist()) for r in bits])
+    full = gs.full_state(g, u)
+    idx = (bits << np.arange(14)).sum(1)
+    assert np.max(np.abs(got - one)) < 1e-15 * np.max(np.abs(one))
+    assert np.max(np.abs(got - full[idx])) < 1e-14 * np.max(np.abs(full))

@@ -201,3 +201,11 @@ def test_graph_phase_tables():
     b, par = gs.lookup_map_phase(lut, ids)
     assert np.array_equal(b, ids)
     assert np.array_equal(par, gs.quadratic_parity(ids, g))
+
+
+def test_threaded_state_equals_full_state():
+    """The CPU baseline's threaded state build is the same arithmetic split
+    over slices: bit-identical to full_state."""
+    for n, threads in ((10, 3), (14, 8)):
+        g, u = instance(n, 40 + n)
+        assert np.array_equal(gs.full_state_threads(g, u, threads), gs.full_state(g, u))
