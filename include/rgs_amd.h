@@ -315,6 +315,19 @@ int rgs_scan_large(const rgs_scan_step *steps, int n, const uint64_t *plan_dev, 
                    const double *u_host, const uint64_t *bits_host, int precision, void *buf0, void *buf1,
                    uint64_t buf_entries, uint64_t *mx_dev, double *out_val, int32_t *out_exp, void *stream);
 
+/* rgs_scan_large without caller buffers, for callers that cannot allocate
+ * device memory (code under HIP graph capture, serving loops, harnesses that
+ * forbid hipMalloc).  No counterpart in the reference.  The two tables and
+ * the step maxima live in a fixed array of this library, one copy per device
+ * (32 MiB + 4 KiB, held until the process ends), so plans are limited to
+ * rank 21 with complex64 tables and rank 20 with complex128 tables (larger
+ * ranks return RGS_EINVAL before any launch).  The call allocates nothing.
+ * Every call on a device shares that device's array: calls that overlap on
+ * one device (other streams, other threads) race.  Outputs as rgs_scan_large. */
+int rgs_scan_large_static(const rgs_scan_step *steps, int n, const uint64_t *plan_dev, const uint64_t *plan_offsets,
+                          const double *u_host, const uint64_t *bits_host, int precision, double *out_val,
+                          int32_t *out_exp, void *stream);
+
 /* ---- learned proposal (MADE) -------------------------------------------
  * The reference has no counterpart for any entry point below: the learned
  * proposal is new work.  All matrices are row-major; bf16 values are raw

@@ -775,3 +775,68 @@ extern "C" int rgs_scan_large(const rgs_scan_step* steps, int n, const uint64_t*
     return scan_large<double2>(steps, n, plan_offsets, plan_dev, u, bits_host, buf0, buf1, mx, val, out_exp, st);
   return fail(kInvalid, "precision must be 64 (complex64 tables) or 128 (complex128)");
 }
+// ------------------------------------------------- allocation-free variant
+// rgs_scan_large_static runs the same kernels with its two tables and the
+// step maxima in a fixed device array of this library (one copy per device,
+// held until the process ends), for callers that cannot allocate.  Layout:
+// the n <= 128 step maxima, then two tables of kStaticTableBytes each.
+namespace rgs {
+
+constexpr size_t kStaticMaxBytes = 128 * sizeof(unsigned long long);
+constexpr size_t kStaticTableBytes = (size_t)1 << 24;   // 2^20 complex128 or 2^21 complex64 entries
+constexpr size_t kStaticBytes = 4096 + 2 * kStaticTableBytes;
+static_assert(kStaticMaxBytes <= 4096, "maxima fit the first 4 KiB");
+
+static __device__ __align__(256) unsigned char scratch[kStaticBytes];
+
+constexpr int kMaxDevices = 64;
+static std::mutex scratch_lock;
+static unsigned char* scratch_ptr[kMaxDevices];   // device address of `scratch` on each device
+
+// The device address of `scratch` on the current device, looked up once per
+// device (the array's host-side name is not a device address).
+static int scratch_of_current(unsigned char** out) {
+  int dev = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e != hipSuccess) return fail(kHip, std::string("scan_large_static: ") + hipGetErrorString(e));
+  if (dev < 0 || dev >= kMaxDevices) return fail(kInvalid, "scan_large_static: device index above 63");
+  std::lock_guard<std::mutex> g(scratch_lock);
+  if (!scratch_ptr[dev]) {
+    void* p = nullptr;
+    e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+    if (e != hipSuccess || !p) return fail(kHip, std::string("scan_large_static: hipGetSymbolAddress: ") +
+                                                     hipGetErrorString(e));
+    scratch_ptr[dev] = static_cast<unsigned char*>(p);
+  }
+  *out = scratch_ptr[dev];
+  return kOk;
+}
+
+}  // namespace rgs
+
+extern "C" int rgs_scan_large_static(const rgs_scan_step* steps, int n, const uint64_t* plan_dev,
+                                     const uint64_t* plan_offsets, const double* u_host, const uint64_t* bits_host,
+                                     int precision, double* out_val, int32_t* out_exp, void* stream) {
+  if (!steps || !plan_dev || !plan_offsets || !u_host || !bits_host || !out_val || !out_exp || n < 1 || n > 128)
+    return fail(kInvalid, "scan_large_static: bad arguments");
+  if (precision != 64 && precision != 128)
+    return fail(kInvalid, "precision must be 64 (complex64 tables) or 128 (complex128)");
+  int rmax = 0;
+  for (int j = 0; j < n; ++j) rmax = std::max(rmax, std::max(steps[j].r_in, steps[j].r_out));
+  const int limit = precision == 64 ? 21 : 20;
+  if (rmax > limit)
+    return fail(kInvalid, "scan_large_static: rank " + std::to_string(rmax) + " above the fixed tables' limit " +
+                              std::to_string(limit) + " (use rgs_scan_large with caller buffers)");
+  unsigned char* base = nullptr;
+  int rc = scratch_of_current(&base);
+  if (rc) return rc;
+  unsigned long long* mx = reinterpret_cast<unsigned long long*>(base);
+  void* buf0 = base + 4096;
+  void* buf1 = base + 4096 + kStaticTableBytes;
+  hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+  const double2* u = reinterpret_cast<const double2*>(u_host);
+  double2* val = reinterpret_cast<double2*>(out_val);
+  if (precision == 64)
+    return scan_large<float2>(steps, n, plan_offsets, plan_dev, u, bits_host, buf0, buf1, mx, val, out_exp, st);
+  return scan_large<double2>(steps, n, plan_offsets, plan_dev, u, bits_host, buf0, buf1, mx, val, out_exp, st);
+}

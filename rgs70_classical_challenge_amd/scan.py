@@ -35,6 +35,8 @@ _lib.register('rgs_capped_decide', ctypes.c_int,
 _lib.register('rgs_scan_pack_tables', ctypes.c_int, [_V, ctypes.c_int, _V, ctypes.POINTER(ctypes.c_uint64)])
 _lib.register('rgs_scan_large', ctypes.c_int,
               [_V, ctypes.c_int, _V, _V, _V, _V, ctypes.c_int, _V, _V, ctypes.c_uint64, _V, _V, _V, _V])
+_lib.register('rgs_scan_large_static', ctypes.c_int,
+              [_V, ctypes.c_int, _V, _V, _V, _V, ctypes.c_int, _V, _V, _V])
 
 # Largest rank the one-launch batch kernels hold in LDS, per table precision.
 BATCH_RMAX = {64: 13, 128: 12}
