@@ -207,7 +207,9 @@ extern "C" int rgs_scan_profile(const uint8_t* g, int n, const int32_t* order, i
 // so the prefix sets before i and from j on are unchanged and only the ranks
 // of the cuts i..j-1 are recomputed.  Moves: swap of neighbours, insertion
 // and reversal within a short window, and a swap of two arbitrary positions.
-// The run is split into epochs; each restarts from the chain's best order and
+// The chain minimises sum 2^r with the cuts above the rank cap weighted 64x,
+// so a start above the cap is driven under it; once under, it stays.  The
+// run is split into epochs; each restarts from the chain's best order and
 // cools geometrically, with the temperature relative to the current cost (so
 // the same schedule serves any graph), and later epochs start colder.  The
 // chain is a pure function of (graph, start, iterations, seed, cap).
@@ -216,9 +218,12 @@ namespace rgs {
 struct Chain {
   std::vector<int> order, rank;
   std::vector<u128> prefix;
-  uint64_t cost = 0;
+  uint64_t cost = 0;   // sum of 2^r
+  uint64_t eff = 0;    // the same with cuts above the cap weighted 64x: what the chain minimises
   int peak = 0;
 };
+
+static inline uint64_t eff_term(int r, int cap) { return (uint64_t)(r > cap ? 64 : 1) << r; }
 
 static int cut_rank_small_side(const std::vector<u128>& adj, u128 left, int n, int left_count) {
   u128 full = n == 128 ? ~(u128)0 : (((u128)1 << n) - 1);
@@ -226,7 +231,7 @@ static int cut_rank_small_side(const std::vector<u128>& adj, u128 left, int n, i
   return 2 * left_count > n ? cut_rank(adj, full & ~left, n) : cut_rank(adj, left, n);
 }
 
-static void chain_init(const std::vector<u128>& adj, const std::vector<int>& order, int n, Chain& c) {
+static void chain_init(const std::vector<u128>& adj, const std::vector<int>& order, int n, int cap, Chain& c) {
   c.order = order;
   c.prefix.assign(n, 0);
   c.rank.assign(n, 0);
@@ -237,13 +242,15 @@ static void chain_init(const std::vector<u128>& adj, const std::vector<int>& ord
     c.rank[k] = k == n - 1 ? 0 : cut_rank_small_side(adj, p, n, k + 1);
   }
   c.cost = cost_of(c.rank);
+  c.eff = 0;
+  for (int r : c.rank) c.eff += eff_term(r, cap);
   c.peak = *std::max_element(c.rank.begin(), c.rank.end());
 }
 
 static void run_chain(const std::vector<u128>& adj, const std::vector<int>& start, int n, uint64_t iterations,
                       uint64_t seed, int rmax_cap, Chain& best) {
   Chain cur;
-  chain_init(adj, start, n, cur);
+  chain_init(adj, start, n, rmax_cap, cur);
   best = cur;
   bool best_ok = cur.peak <= rmax_cap;
   std::mt19937_64 rng(seed);
@@ -276,17 +283,18 @@ static void run_chain(const std::vector<u128>& adj, const std::vector<int>& star
     } else std::reverse(cand_order.begin() + i, cand_order.begin() + j + 1);
     // ranks of the cuts i..j-1 (the cut after position j holds the same set)
     u128 p = i ? cur.prefix[i - 1] : 0;
-    int64_t diff = 0;
+    int64_t diff = 0, ediff = 0;
     for (int k = i; k < j; ++k) {
       p |= (u128)1 << cand_order[k];
       new_prefix[k] = p;
       new_rank[k] = cut_rank_small_side(adj, p, n, k + 1);
       diff += (int64_t)(1ull << new_rank[k]) - (int64_t)(1ull << cur.rank[k]);
+      ediff += (int64_t)eff_term(new_rank[k], rmax_cap) - (int64_t)eff_term(cur.rank[k], rmax_cap);
     }
     int peak = 0;
     for (int k = 0; k < n; ++k) peak = std::max(peak, (k >= i && k < j) ? new_rank[k] : cur.rank[k]);
     if (peak > rmax_cap && cur.peak <= rmax_cap) continue;
-    bool take = diff <= 0 || unif(rng) < std::exp(-(double)diff / (temp * (double)cur.cost));
+    bool take = ediff <= 0 || unif(rng) < std::exp(-(double)ediff / (temp * (double)cur.eff));
     if (!take) continue;
     for (int k = i; k <= j; ++k) cur.order[k] = cand_order[k];
     for (int k = i; k < j; ++k) {
@@ -294,6 +302,7 @@ static void run_chain(const std::vector<u128>& adj, const std::vector<int>& star
       cur.rank[k] = new_rank[k];
     }
     cur.cost = (uint64_t)((int64_t)cur.cost + diff);
+    cur.eff = (uint64_t)((int64_t)cur.eff + ediff);
     cur.peak = peak;
     if (peak <= rmax_cap && (!best_ok || cur.cost < best.cost)) {
       best = cur;

@@ -529,6 +529,31 @@ class ConditionalSampler:
         self.last_margins.append(abs(t3 - pp[0]) / total)
         return bit * self.n + local, float(pp[bit]), norm
 
+    def sample(self, uniforms, bit_uniforms=None):
+        """Batched draws: the device descent of the mass pyramid picks each
+        index of the pre-terminal state (distribution of the local index
+        exactly as the reference's tile-then-index draw), then, when the top
+        gate is deferred, the two amplitudes at that local index go through
+        the terminal gate in complex128 and one more uniform picks the top bit
+        (DeferredGlobalMixin.draw, deferred_global_sampler.py:72-78).
+        Returns (indices, unnormalised p)."""
+        idx, p = self.slab.draw_many(uniforms)
+        if not self.defer_top:
+            return idx, p
+        if bit_uniforms is None or len(bit_uniforms) != len(idx):
+            raise ValueError('one bit uniform per draw is needed with the deferred top gate')
+        torch = self.slab.torch
+        local = torch.as_tensor(idx % self.n).to(self.slab.device)
+        st = self.slab.state
+        lo = st.index_select(0, local).cpu().numpy().astype(np.complex128)
+        hi = st.index_select(0, local + self.n).cpu().numpy().astype(np.complex128)
+        U = self.deferred_global_unitary
+        a = U[0, 0] * lo + U[0, 1] * hi
+        b = U[1, 0] * lo + U[1, 1] * hi
+        pa, pb = np.abs(a) ** 2, np.abs(b) ** 2
+        bit = (np.asarray(bit_uniforms) * (pa + pb) >= pa).astype(np.int64)
+        return bit * self.n + (idx % self.n), np.where(bit == 1, pb, pa)
+
     def probability_at(self, index):
         if self.defer_top:
             bit = int(index // self.n)

@@ -227,3 +227,68 @@ def test_conditional_sampler_errors(golden_dir):
     bad[A[-1], 0, 0] += 1e-3
     with pytest.raises(ValueError):
         E.ConditionalSampler(g, bad, A, B, [0])
+
+
+def _grouped_chi2_p(idx, p, groups=64):
+    """Chi-square p-value of draws over equal-mass groups of a distribution
+    (the reference's cpu_draw_test/run_all.py:14-18 design)."""
+    from scipy import stats
+    order = np.argsort(p, kind='stable')
+    cm = np.cumsum(p[order])
+    grp = np.empty(len(p), np.int64)
+    grp[order] = np.minimum((cm * groups).astype(np.int64), groups - 1)
+    e = np.bincount(grp, weights=p, minlength=groups) * len(idx)
+    c = np.bincount(grp[idx], minlength=groups)
+    return float(stats.chi2.sf(((c - e) ** 2 / e).sum(), groups - 1))
+
+
+def test_draw_statistics_plain_and_conditional(golden_dir):
+    """Statistical test of the draws against exact Born distributions, as the
+    reference's cpu_draw_test does for its production draw: per trial a
+    chi-square over 64 equal-mass groups and the empirical XEB minus its exact
+    value, pooled.  The plain sampler (n = 20, config 1) draws 10^7 samples;
+    the conditional sampler (induced 24- and 40-qubit fixtures, deferred top
+    gate) draws 10^6 per trial for 4 trials of fresh B outcomes each."""
+    from scipy import stats
+    E = _engine()
+    rng = np.random.default_rng(20261102)
+    pvals, diffs, ses = [], [], []
+
+    def score(idx, p, m):
+        z = (1 << m) * p[idx]
+        pvals.append(_grouped_chi2_p(idx, p))
+        diffs.append(z.mean() - 1 - ((1 << m) * (p ** 2).sum() - 1))
+        ses.append(z.std(ddof=1) / np.sqrt(len(z)))
+
+    g, u = instance(20, 2026)
+    s = E.GraphStateSampler(g, u)
+    s.compute()
+    p = np.abs(gs.full_state(g, u)) ** 2
+    p /= p.sum()
+    for k in range(10):
+        idx, _ = s.sample(rng.random(10 ** 6))
+        score(idx, p, 20)
+    del s
+    torch.cuda.empty_cache()
+    for name in ('induced_24q.npz', 'induced_40q.npz'):
+        d = np.load(f'{golden_dir}/{name}')
+        g, u, A, B = d['graph'], d['measurement_unitaries'], d['A'].tolist(), d['B'].tolist()
+        eng = E.ConditionalSampler(g, u, A, B, [0])
+        for trial in range(4):
+            ob = rng.integers(0, 2, len(B)).tolist()
+            eng.compute(ob)
+            _, final = gs.deferred(g, u, A, B, ob)
+            p = np.abs(final) ** 2
+            p /= p.sum()
+            idx, _ = eng.sample(rng.random(10 ** 6), rng.random(10 ** 6))
+            score(idx, p, len(A))
+        del eng
+        torch.cuda.empty_cache()
+    pvals, diffs, ses = np.array(pvals), np.array(diffs), np.array(ses)
+    bias, bias_se = diffs.mean(), np.sqrt((ses ** 2).sum()) / len(ses)
+    ks = stats.kstest(pvals, 'uniform').pvalue
+    fisher = stats.chi2.sf(-2 * np.log(pvals).sum(), 2 * len(pvals))
+    print(f'chi-square p-values {np.round(pvals, 3).tolist()}; KS uniformity p {ks:.3f}; Fisher combined p '
+          f'{fisher:.3f}; pooled XEB bias {bias:.2e} +- {bias_se:.2e}', flush=True)
+    assert pvals.min() > 1e-4 and fisher > 1e-4
+    assert abs(bias) < 5 * bias_se
