@@ -1,6 +1,6 @@
 """Extract small golden fixtures from the reference's data files.
 
-Reads only data (npz / npy / json / jsonl) under /root/reference with safe
+Reads only data (npz / npy / json / jsonl / the circuit's .qasm text) under /root/reference with safe
 loaders (numpy.load with allow_pickle=False, json); no reference code is
 imported or run.  Writes into tests/golden/.  Re-run from the repository root:
 
@@ -96,6 +96,17 @@ def main():
         'p_fp64': [float(r['p_fp64']) for r in acc], 'xeb_fp64': [float(r['xeb_fp64']) for r in acc],
         'cap': cap8['cap'], 'proposals': cap8['proposals'], 'accepted': cap8['accepted'],
         'score': cap8['score']}))
+    # the 70-qubit circuit file (input of the QASM front end, gzip-compressed)
+    # and the instance block of the reference's structural audit of it
+    import gzip
+    qasm = REF / 'work' / 'qat' / 'data' / 'classically-verifiable-problems' / 'circuit-models' / \
+        'random_graph_sampling' / 'nq70_depth70_checks27_basis_fstate.qasm'
+    with gzip.GzipFile(OUT / 'rgs70_circuit.qasm.gz', 'wb', mtime=0) as f:
+        f.write(qasm.read_bytes())
+    inst = json.loads((WORK / 'audit.json').read_text())['instance']
+    (OUT / 'rgs70_audit_instance.json').write_text(json.dumps(
+        {k: inst[k] for k in ('n', 'graph_edges', 'partition_A', 'partition_B', 'cross_rank',
+                              'has_exact_uniform_B_marginal', 'local_unitary_residual')}, indent=1))
     print('wrote fixtures to', OUT, 'summary keys', sorted(summ))
 
 
