@@ -77,10 +77,13 @@ def group_generator(group):
     return rng
 
 
-def produce(engine, n, group, out, count=None, meta=None, log=None):
+def produce(engine, n, group, out, count=None, meta=None, log=None, setup_seconds=0.0, process_start=None):
     """Draw ``count`` (default: the group's sample_count) samples with a
     conditional engine (ConditionalSampler) on the group's RNG stream and write
-    the JSONL records to ``out``, which must not exist.  Returns the records."""
+    the JSONL records to ``out``, which must not exist.  ``setup_seconds`` (the
+    engine's construction) and the process clock start (perf_counter) go
+    into the header and footer.  Returns the records."""
+    process_start = time.perf_counter() if process_start is None else process_start
     A, B = engine.A, engine.B
     count = group['sample_count'] if count is None else count
     rng = group_generator(group)
@@ -89,9 +92,10 @@ def produce(engine, n, group, out, count=None, meta=None, log=None):
     if out.exists():
         raise FileExistsError('Production output already exists; do not overwrite or silently resume')
     metadata = dict(group=group, engine=type(engine).__name__, numpy=np.__version__, conditional_qubits=len(A),
+                    setup_seconds=setup_seconds, process_to_ready_seconds=time.perf_counter() - process_start,
                     B_vertices=list(B), A_vertices=list(A),
-                    rng_schedule='fresh B integers, then 3 uniform draws; no warmup consumes production RNG',
-                    **(meta or {}))
+                    rng_schedule='fresh B integers, then 3 uniform draws; no warmup consumes production RNG')
+    metadata.update(meta or {})
     records = []
     with out.open('x') as f:
         f.write(json.dumps({'metadata': metadata}) + '\n')
@@ -123,6 +127,7 @@ def produce(engine, n, group, out, count=None, meta=None, log=None):
                 log(f'[records] sample {i}: {rec["elapsed_seconds"]:.3f} s')
         summary = dict(samples=count, group_id=group['group_id'], generation_start_unix=wall0,
                        generation_end_unix=time.time(), generation_seconds=time.perf_counter() - t0,
+                       process_elapsed_seconds=time.perf_counter() - process_start, setup_seconds=setup_seconds,
                        all_count_completed=True)
         f.write(json.dumps({'summary': summary}) + '\n')
         f.flush()
