@@ -31,6 +31,12 @@ __device__ __forceinline__ double2 load_signed(const double2& s, uint32_t sgn) {
                       __longlong_as_double(__double_as_longlong(s.y) ^ m));
 }
 
+// x with its sign flipped when neg is 1 (an integer XOR on the sign bit: exact,
+// and scalar when x and neg are uniform).
+__device__ __forceinline__ double flip_sign(double x, uint32_t neg) {
+  return __longlong_as_double(__double_as_longlong(x) ^ ((long long)neg << 63));
+}
+
 template <typename S>
 __device__ __forceinline__ S make_s(double re, double im);
 template <>
@@ -61,7 +67,7 @@ __device__ __forceinline__ uint32_t wave_max_u32(uint32_t v) {
 // (the frexp exponent of sqrt(max |out|^2)), so max |out| 2^-e lies in
 // [1/2, 1); a zero or subnormal maximum leaves the table unscaled.
 __device__ __forceinline__ uint32_t sq_hi(double re, double im) {
-  return (uint32_t)(__double_as_longlong(re * re + im * im) >> 32);
+  return (uint32_t)(__double_as_longlong(fma(re, re, im * im)) >> 32);
 }
 __device__ __forceinline__ int scale_exponent(uint32_t mxh) {
   const int biased = (int)(mxh >> 20);
@@ -93,6 +99,7 @@ struct ScanStepArgs {
 template <typename S, int K, int NKER, bool SPLIT, typename CP>
 __device__ __forceinline__ uint32_t scan_outputs(const S* E, const ScanStepArgs& a, CP combos, uint32_t lo_w,
                                                  const uint32_t* hstep, int nkw, S (&outv)[K]) {
+#pragma clang fp contract(off)
   constexpr int NC = 1 << NKER, HALF = SPLIT ? NC / 2 : NC;
   uint32_t cmb[NC];
 #pragma unroll
@@ -129,14 +136,93 @@ __device__ __forceinline__ uint32_t scan_outputs(const S* E, const ScanStepArgs&
           s1i += e.y;
         }
       }
+      // explicit fused multiply-adds (contraction is off in these helpers):
+      // every kernel that sums through them rounds the same way
       double orr, oi;
       if (SPLIT) {
-        orr = (s0r * a.f0.x - s0i * a.f0.y) + (s1r * a.f1.x - s1i * a.f1.y);
-        oi = (s0r * a.f0.y + s0i * a.f0.x) + (s1r * a.f1.y + s1i * a.f1.x);
+        orr = fma(s1r, a.f1.x, fma(-s1i, a.f1.y, fma(s0r, a.f0.x, -s0i * a.f0.y)));
+        oi = fma(s1r, a.f1.y, fma(s1i, a.f1.x, fma(s0r, a.f0.y, s0i * a.f0.x)));
       } else {
         const double2 f = yh ? fb : fa;
-        orr = s0r * f.x - s0i * f.y;
-        oi = s0r * f.y + s0i * f.x;
+        orr = fma(s0r, f.x, -s0i * f.y);
+        oi = fma(s0r, f.y, s0i * f.x);
+      }
+      mxh = max(mxh, sq_hi(orr, oi));
+      outv[kk] = make_s<S>(orr, oi);
+    }
+  }
+  return mxh;
+}
+
+// Outputs of one step for this lane on a step whose table coordinates make
+// y(p0) and the sign form vanish on the lane bits (the packer's uniform
+// flag): within a 64-output block every output has the same weight and every
+// term t the same sign, decided from the block word b = hi[wave] ^ hs alone.
+// A term's table index is the lane's masked word lm ^ ((b ^ combo_t) & mask);
+// its sign is uniform, so the signed sums become fused multiply-adds by a
+// scalar +-1 and the first term's sign moves into the weight.  Returns the
+// lane's maximum high word of |out|^2.
+#define LDS_AS __attribute__((address_space(3)))
+// lm8: the lane's masked word as the LDS byte address of its table entry
+// (the table starts at the LDS base, which is 0: the kernel has no static
+// LDS), so a term's address is one XOR with a uniform offset.
+template <typename S, int K, int NKER, bool SPLIT, typename CP>
+__device__ __forceinline__ uint32_t scan_outputs_u(const ScanStepArgs& a, CP combos, uint32_t lm8, uint32_t hw,
+                                                   const uint32_t* hstep, int nkw, S (&outv)[K]) {
+#pragma clang fp contract(off)
+  constexpr int NC = 1 << NKER, HALF = SPLIT ? NC / 2 : NC;
+  constexpr int SH = sizeof(S) == 8 ? 3 : 4;   // log2 of an entry's bytes
+  uint32_t cm8[NC], par[NC];
+#pragma unroll
+  for (int t = 0; t < NC; ++t) {
+    const uint32_t cm = (uint32_t)combos[t] & a.mask;
+    cm8[t] = cm << SH;
+    par[t] = (uint32_t)__popc(cm & a.beta) & 1u;
+  }
+  uint32_t mxh = 0, hs = 0;
+#pragma unroll
+  for (int kk = 0; kk < K; ++kk) {
+    if (kk) hs ^= hstep[__builtin_ctz(kk)];
+    if (kk < nkw) {
+      const uint32_t b = hw ^ hs;
+      const uint32_t bm = b & a.mask;
+      const uint32_t yb = SPLIT ? 0u : (b >> a.r_in) & 1u;
+      const uint32_t sb = (uint32_t)__popc(bm & a.beta) & 1u;
+      // sign of term t: split steps sign only the second half (y = 1);
+      // plain steps sign every term when y = 1
+      uint32_t neg[NC];
+#pragma unroll
+      for (int t = 0; t < NC; ++t) neg[t] = SPLIT ? (t < HALF ? 0u : sb ^ par[t]) : (yb & (sb ^ par[t]));
+      const uint32_t row8 = lm8 ^ (bm << SH);
+      double s0r = 0.0, s0i = 0.0, s1r = 0.0, s1i = 0.0;
+#pragma unroll
+      for (int t = 0; t < NC; ++t) {
+        const double2 e = load_c(*reinterpret_cast<const S LDS_AS*>(row8 ^ cm8[t]));
+        const int first = t < HALF ? 0 : HALF;
+        const double rho = (neg[t] ^ neg[first]) ? -1.0 : 1.0;
+        if (t == 0) {
+          s0r = e.x;
+          s0i = e.y;
+        } else if (t < HALF) {
+          s0r = fma(e.x, rho, s0r);
+          s0i = fma(e.y, rho, s0i);
+        } else if (t == HALF) {
+          s1r = e.x;
+          s1i = e.y;
+        } else {
+          s1r = fma(e.x, rho, s1r);
+          s1i = fma(e.y, rho, s1i);
+        }
+      }
+      double orr, oi;
+      if (SPLIT) {
+        const double g1x = flip_sign(a.f1.x, neg[HALF]), g1y = flip_sign(a.f1.y, neg[HALF]);
+        orr = fma(s1r, g1x, fma(-s1i, g1y, fma(s0r, a.f0.x, -s0i * a.f0.y)));
+        oi = fma(s1r, g1y, fma(s1i, g1x, fma(s0r, a.f0.y, s0i * a.f0.x)));
+      } else {
+        const double fx = flip_sign(yb ? a.f1.x : a.f0.x, neg[0]), fy = flip_sign(yb ? a.f1.y : a.f0.y, neg[0]);
+        orr = fma(s0r, fx, -s0i * fy);
+        oi = fma(s0r, fy, s0i * fx);
       }
       mxh = max(mxh, sq_hi(orr, oi));
       outv[kk] = make_s<S>(orr, oi);
@@ -166,6 +252,9 @@ __device__ inline void reject_outputs(int count, double2* out_val, int32_t* out_
   }
 }
 
+constexpr int kStepHdrWords = 8;     // per step: header (2), beta, offset, 4 combinations
+constexpr int kMaxPlanSteps = 128;   // packed plans have at most 128 steps
+
 template <typename S, int K, int W>
 __global__ void __launch_bounds__(64 * W) k_scan_batch(const uint64_t* __restrict__ plan,
                                                       const double2* __restrict__ u,
@@ -174,34 +263,72 @@ __global__ void __launch_bounds__(64 * W) k_scan_batch(const uint64_t* __restric
                                                       int rcap) {
   static_assert((K & (K - 1)) == 0 && (W & (W - 1)) == 0, "K and W are powers of two");
   constexpr int KB = K > 1 ? __builtin_ctz(K) : 0;   // Gray steps within the wave's blocks
+  constexpr int SH = sizeof(S) == 8 ? 3 : 4;   // log2 of an entry's bytes
+  // the table starts at LDS byte 0 (so an entry's byte offset is its index
+  // shifted), the waves' maxima follow it
   extern __shared__ __align__(16) unsigned char smem[];
   S* E = reinterpret_cast<S*>(smem);
-  __shared__ uint32_t wmx[W];
+  uint32_t* wmx = reinterpret_cast<uint32_t*>(smem + ((size_t)1 << rcap) * sizeof(S));
   const int lane = threadIdx.x & 63;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);   // uniform: plan reads stay scalar
   const int n = (int)plan[0];
-  if ((int)plan[1] > rcap) {   // the plan's tables would not fit: mark every output, touch no table
+  // the table's LDS byte address; terms XOR their offsets into it, which
+  // equals adding them because the table is aligned to its own size
+  const uint32_t lds_base = (uint32_t)(uintptr_t)(LDS_AS unsigned char*)smem;
+  if ((int)plan[1] > rcap || (lds_base & ((2u << (rcap + SH)) - 1u))) {
+    // the plan's tables would not fit (or the table is not size-aligned):
+    // mark every output, touch no table
     reject_outputs(count, out_val, out_exp);
     return;
+  }
+  // Each step's header words, staged in LDS once per workgroup (after the
+  // table and the maxima), so a step starts from an LDS read instead of a
+  // chain of dependent global reads (offset, then header, then combinations).
+  uint32_t* shdr = wmx + 16;
+  for (int i = threadIdx.x; i < n; i += 64 * W) {
+    const uint64_t off = plan[2 + i];
+    const uint64_t* st = plan + off;
+    const uint64_t hdr = st[0];
+    const int r_out = (int)((hdr >> 16) & 255u), nker = (int)((hdr >> 24) & 255u);
+    const uint64_t* cb = st + 66 + (1ull << (r_out > 6 ? r_out - 6 : 0));
+    uint32_t* h = shdr + kStepHdrWords * i;
+    h[0] = (uint32_t)hdr;
+    h[1] = (uint32_t)(hdr >> 32);
+    h[2] = (uint32_t)st[1];
+    h[3] = (uint32_t)off;
+    for (int t = 0; t < 4; ++t) h[4 + t] = t < (1 << nker) ? (uint32_t)cb[t] : 0u;
   }
   for (int pidx = blockIdx.x; pidx < count; pidx += gridDim.x) {
     const uint64_t x0 = bits[2 * pidx], x1 = bits[2 * pidx + 1];
     if (threadIdx.x == 0) E[0] = make_s<S>(1.0, 0.0);
     __syncthreads();
     int esum = 0;
+    // The table holds each step's outputs unscaled; the power of two that
+    // brings the step's largest |output| into [1/2, 1) is applied to the
+    // next step's weights instead (exact in FP64, so every sum rounds as if
+    // the table had been scaled).
+    int e_prev = 0;
     for (int j = 0; j < n; ++j) {
-      const uint64_t* st = plan + plan[2 + j];
-      const uint64_t hdr = st[0];
+      const uint32_t* h = shdr + kStepHdrWords * j;
+      const uint32_t hlo = __builtin_amdgcn_readfirstlane(h[0]), hhi = __builtin_amdgcn_readfirstlane(h[1]);
+      const uint64_t hdr = (uint64_t)hlo | ((uint64_t)hhi << 32);
+      const uint64_t* st = plan + __builtin_amdgcn_readfirstlane(h[3]);
+      uint32_t combos[4];
+#pragma unroll
+      for (int t = 0; t < 4; ++t) combos[t] = __builtin_amdgcn_readfirstlane(h[4 + t]);
       const int v = (int)(hdr & 255u), r_out = (int)((hdr >> 16) & 255u);
       const bool split = (hdr >> 32) & 1u;
       ScanStepArgs a;
       a.r_in = (int)((hdr >> 8) & 255u);
       a.nker = (int)((hdr >> 24) & 255u);
-      a.beta = (uint32_t)st[1];
+      a.beta = __builtin_amdgcn_readfirstlane(h[2]);
       const int xv = (int)((v < 64 ? (x0 >> v) : (x1 >> (v - 64))) & 1u);
       a.f0 = u[(v * 2 + xv) * 2 + 0];
       a.f1 = u[(v * 2 + xv) * 2 + 1];
-      const uint64_t* combos = st + 66 + (1ull << (r_out > 6 ? r_out - 6 : 0));
+      a.f0.x = ldexp(a.f0.x, -e_prev);
+      a.f0.y = ldexp(a.f0.y, -e_prev);
+      a.f1.x = ldexp(a.f1.x, -e_prev);
+      a.f1.y = ldexp(a.f1.y, -e_prev);
       a.mask = (1u << a.r_in) - 1u;
       const int nk = r_out > 6 ? 1 << (r_out - 6) : 1;
       const int nkw = nk >= W ? nk / W : (wave < nk ? 1 : 0);
@@ -213,13 +340,30 @@ __global__ void __launch_bounds__(64 * W) k_scan_batch(const uint64_t* __restric
       S outv[K];
       uint32_t mxh;
       // the packed plan has nker <= 2, and split steps nker >= 1
-      if (split) {
-        mxh = a.nker == 1 ? scan_outputs<S, K, 1, true>(E, a, combos, lo_w, hstep, nkw, outv)
-                          : scan_outputs<S, K, 2, true>(E, a, combos, lo_w, hstep, nkw, outv);
+      if ((hdr >> 33) & 1u) {
+        uint32_t lm8 = (((uint32_t)st[2 + lane] & a.mask) << SH) + lds_base;
+        asm volatile("" : "+v"(lm8));   // keep the shifted word: terms then need one XOR each
+        const uint32_t hw = wave < nk ? (uint32_t)st[66 + wave] : 0u;
+        if (split) {
+          mxh = a.nker == 1 ? scan_outputs_u<S, K, 1, true>(a, combos, lm8, hw, hstep, nkw, outv)
+                            : scan_outputs_u<S, K, 2, true>(a, combos, lm8, hw, hstep, nkw, outv);
+        } else {
+          mxh = a.nker == 0   ? scan_outputs_u<S, K, 0, false>(a, combos, lm8, hw, hstep, nkw, outv)
+                : a.nker == 1 ? scan_outputs_u<S, K, 1, false>(a, combos, lm8, hw, hstep, nkw, outv)
+                              : scan_outputs_u<S, K, 2, false>(a, combos, lm8, hw, hstep, nkw, outv);
+        }
       } else {
-        mxh = a.nker == 0   ? scan_outputs<S, K, 0, false>(E, a, combos, lo_w, hstep, nkw, outv)
-              : a.nker == 1 ? scan_outputs<S, K, 1, false>(E, a, combos, lo_w, hstep, nkw, outv)
-                            : scan_outputs<S, K, 2, false>(E, a, combos, lo_w, hstep, nkw, outv);
+        // steps without the uniform flag have r_out < 8: at most two blocks,
+        // so at most one per wave (nkw <= 1)
+        S (&o1)[1] = *reinterpret_cast<S(*)[1]>(&outv[0]);
+        if (split) {
+          mxh = a.nker == 1 ? scan_outputs<S, 1, 1, true>(E, a, combos, lo_w, hstep, nkw, o1)
+                            : scan_outputs<S, 1, 2, true>(E, a, combos, lo_w, hstep, nkw, o1);
+        } else {
+          mxh = a.nker == 0   ? scan_outputs<S, 1, 0, false>(E, a, combos, lo_w, hstep, nkw, o1)
+                : a.nker == 1 ? scan_outputs<S, 1, 1, false>(E, a, combos, lo_w, hstep, nkw, o1)
+                              : scan_outputs<S, 1, 2, false>(E, a, combos, lo_w, hstep, nkw, o1);
+        }
       }
       // lanes past 2^r_out (steps of rank below six) computed in-range values
       // that are neither counted nor stored
@@ -234,19 +378,16 @@ __global__ void __launch_bounds__(64 * W) k_scan_batch(const uint64_t* __restric
       if (lane_ok) {
 #pragma unroll
         for (int kk = 0; kk < K; ++kk) {
-          if (kk < nkw) {
-            S val = outv[kk];
-            val.x = ldexp(val.x, -e);
-            val.y = ldexp(val.y, -e);
-            E[lane + 64 * (wave + W * (kk ^ (kk >> 1)))] = val;
-          }
+          if (kk < nkw) E[lane + 64 * (wave + W * (kk ^ (kk >> 1)))] = outv[kk];
         }
       }
       esum += e;
+      e_prev = e;
       __syncthreads();
     }
     if (threadIdx.x == 0) {
-      out_val[pidx] = load_c(E[0]);
+      double2 v = load_c(E[0]);
+      out_val[pidx] = make_double2(ldexp(v.x, -e_prev), ldexp(v.y, -e_prev));
       out_exp[pidx] = esum;
     }
     __syncthreads();
@@ -256,7 +397,8 @@ __global__ void __launch_bounds__(64 * W) k_scan_batch(const uint64_t* __restric
 template <typename S, int K, int W>
 static hipError_t launch_scan(const uint64_t* plan, const double2* u, const uint64_t* bits, int count,
                                 double2* val, int32_t* ex, int rmax, hipStream_t st) {
-  size_t shmem = ((size_t)1 << rmax) * sizeof(S);
+  // the table, the waves' maxima (64 bytes), then the step headers
+  size_t shmem = ((size_t)1 << rmax) * sizeof(S) + 64 + kStepHdrWords * 4 * kMaxPlanSteps;
   // at least two rounds of the workgroups that fit a CU at once (160 KiB of
   // LDS, 32 waves), and at least 16 per CU (even shares of large batches)
   const int per_cu = std::max(1, std::min((int)((160 * 1024) / shmem), 32 / W));
@@ -274,10 +416,12 @@ static hipError_t launch_scan(const uint64_t* plan, const double2* u, const uint
   return hipGetLastError();
 }
 
-// Wavefronts per proposal for each table precision (the fastest measured at
-// rank 12); ranks with fewer 64-entry output blocks than that use one wave
-// per block.
-constexpr int kWavesC64 = 4, kWavesC128 = 8;
+// Wavefronts per proposal for each table precision (the fastest measured on
+// 65,536 proposals of a 70-vertex plan: four for complex64 tables, four for
+// complex128 tables up to rank 11 and eight at rank 12, whose 64 KiB table
+// leaves room for two workgroups per CU); ranks with fewer 64-entry output
+// blocks than that use one wave per block.
+constexpr int kWavesC64 = 4, kWavesC128 = 4, kWavesC128R12 = 8;
 
 // Tables of rank up to R: 2^(R-6) output blocks of 64 entries, shared by
 // min(W, blocks) waves with K blocks each.
@@ -722,6 +866,7 @@ extern "C" int rgs_scan_batch(const uint64_t* plan_dev, int rmax, const double* 
     if (rmax <= 6) e = launch_small<float2>(plan_dev, u, bits_dev, count, val, out_exp, st);
     else if (rmax <= 8) e = launch_rank<float2, 8, kWavesC64>(plan_dev, u, bits_dev, count, val, out_exp, st);
     else if (rmax <= 10) e = launch_rank<float2, 10, kWavesC64>(plan_dev, u, bits_dev, count, val, out_exp, st);
+    else if (rmax <= 11) e = launch_rank<float2, 11, kWavesC64>(plan_dev, u, bits_dev, count, val, out_exp, st);
     else if (rmax <= 12) e = launch_rank<float2, 12, kWavesC64>(plan_dev, u, bits_dev, count, val, out_exp, st);
     else if (rmax <= 13) e = launch_rank<float2, 13, kWavesC64>(plan_dev, u, bits_dev, count, val, out_exp, st);
     else return fail(kInvalid, "scan_batch: complex64 batch limited to rmax <= 13");
@@ -730,7 +875,7 @@ extern "C" int rgs_scan_batch(const uint64_t* plan_dev, int rmax, const double* 
     else if (rmax <= 8) e = launch_rank<double2, 8, kWavesC128>(plan_dev, u, bits_dev, count, val, out_exp, st);
     else if (rmax <= 10) e = launch_rank<double2, 10, kWavesC128>(plan_dev, u, bits_dev, count, val, out_exp, st);
     else if (rmax <= 11) e = launch_rank<double2, 11, kWavesC128>(plan_dev, u, bits_dev, count, val, out_exp, st);
-    else if (rmax <= 12) e = launch_rank<double2, 12, kWavesC128>(plan_dev, u, bits_dev, count, val, out_exp, st);
+    else if (rmax <= 12) e = launch_rank<double2, 12, kWavesC128R12>(plan_dev, u, bits_dev, count, val, out_exp, st);
     else return fail(kInvalid, "scan_batch: complex128 batch limited to rmax <= 12");
   } else {
     return fail(kInvalid, "precision must be 64 (complex64 tables) or 128 (complex128)");

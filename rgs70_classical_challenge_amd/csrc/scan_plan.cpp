@@ -342,7 +342,8 @@ extern "C" int rgs_scan_order_search(const uint8_t* g, int n, const int32_t* sta
 
 // Packed device plan (uint64 words):
 //   [0] n  [1] rmax  [2 .. 2+n) offset of each step
-//   step: hdr = v | r_in << 8 | r_out << 16 | nker << 24 | split << 32, beta,
+//   step: hdr = v | r_in << 8 | r_out << 16 | nker << 24 | split << 32 |
+//               uniform << 33, beta,
 //         lo[64]  (p0 of the low six bits of c'),
 //         hi[2^max(0, r_out-6)] (p0 of the remaining bits),
 //         combos[2^nker]
@@ -355,10 +356,93 @@ extern "C" int rgs_scan_order_search(const uint8_t* g, int n, const int32_t* sta
 // of its second half), with no per-lane choice of weight.  When none does
 // (split = 0), all terms of an output share y(p0).  The set of terms of each
 // output, and so its sum, is unchanged; only the order of summation is.
+//
+// Table coordinates.  Each environment table may be indexed in any basis of
+// its syndrome space: a change of coordinates T at the output of step j
+// (c'new = T c') is undone at the input of step j+1, whose p-vectors, kernel
+// and beta are rewritten in the new coordinates (p = (c, y) -> (T c, y),
+// beta -> T^-T beta).  Two linear forms decide an output's weight and signs:
+// y(p0(c')) and the sign parity(p0(c') & mask & beta).  The packer picks, for
+// every step with enough outputs, coordinates in which both forms vanish on
+// the six lane bits (a basis of their common kernel first, pivots taken at
+// the highest coordinates so the low bits keep their meaning), so y and the
+// signs depend on the 64-output block only (uniform = 1): the kernel then
+// decides them once per block instead of once per output and term.
+namespace rgs {
+
+static inline int parity64(uint64_t v) { return __builtin_popcountll(v) & 1; }
+
+// Coordinates for one step's outputs: returns the old coordinates of the new
+// basis vectors (columns) and whether the first min(6, r) are in the common
+// kernel of the forms f1, f2 (bit masks over old coordinates).
+static bool choose_coordinates(int r, uint64_t f1, uint64_t f2, uint64_t* inv_cols) {
+  uint64_t v[64];
+  bool used[64] = {false};
+  for (int k = 0; k < r; ++k) v[k] = 1ull << k;
+  for (uint64_t f : {f1, f2}) {
+    int q = -1;
+    for (int k = r - 1; k >= 0; --k)
+      if (!used[k] && parity64(v[k] & f)) {
+        q = k;
+        break;
+      }
+    if (q < 0) continue;
+    for (int k = 0; k < r; ++k)
+      if (k != q && !used[k] && parity64(v[k] & f)) v[k] ^= v[q];
+    used[q] = true;
+  }
+  int m = 0;
+  for (int k = 0; k < r; ++k)
+    if (!used[k]) inv_cols[m++] = v[k];
+  const int free_count = m;
+  for (int k = 0; k < r; ++k)
+    if (used[k]) inv_cols[m++] = v[k];
+  return free_count >= std::min(6, r);
+}
+
+// Inverse of an r x r GF(2) matrix given by columns.
+static void gf2_invert_cols(int r, const uint64_t* cols, uint64_t* inv) {
+  // rows of [A | I] by Gauss-Jordan on columns: solve A x = e_i for each i
+  uint64_t a[64], b[64];
+  for (int i = 0; i < r; ++i) {
+    a[i] = 0;
+    for (int k = 0; k < r; ++k) a[i] |= ((cols[k] >> i) & 1ull) << k;   // row i of A
+    b[i] = 1ull << i;                                                  // row i of I
+  }
+  for (int c = 0; c < r; ++c) {
+    int p = c;
+    while (p < r && !((a[p] >> c) & 1ull)) ++p;
+    std::swap(a[c], a[p]);
+    std::swap(b[c], b[p]);
+    for (int i = 0; i < r; ++i)
+      if (i != c && ((a[i] >> c) & 1ull)) {
+        a[i] ^= a[c];
+        b[i] ^= b[c];
+      }
+  }
+  // rows of A^-1 are b; convert to columns
+  for (int k = 0; k < r; ++k) {
+    inv[k] = 0;
+    for (int i = 0; i < r; ++i) inv[k] |= ((b[i] >> k) & 1ull) << i;
+  }
+}
+
+// v (an old-coordinate vector of r bits) in new coordinates: sum of the
+// columns of T selected by v's bits.
+static inline uint64_t apply_cols(const uint64_t* cols, uint64_t v) {
+  uint64_t out = 0;
+  for (int i = 0; v; ++i, v >>= 1)
+    if (v & 1ull) out ^= cols[i];
+  return out;
+}
+
+}  // namespace rgs
+
 extern "C" int rgs_scan_pack(const rgs_scan_step* steps, int n, uint64_t* blob, uint64_t* words) {
   if (!steps || n < 1 || n > 128 || !words) return fail(kInvalid, "scan_pack: bad arguments");
   uint64_t w = 2 + (uint64_t)n;
   int rmax = 0;
+  uint64_t t_in[64];   // input coordinate change of the current step: columns T e_i
   for (int j = 0; j < n; ++j) {
     const rgs_scan_step& st = steps[j];
     // A step clears one coordinate of r_in basis vectors (rank drops by at
@@ -370,14 +454,39 @@ extern "C" int rgs_scan_pack(const rgs_scan_step* steps, int n, uint64_t* blob, 
     uint64_t hi = 1ull << std::max(0, st.r_out - 6);
     uint64_t off = w;
     w += 2 + 64 + hi + (1ull << st.nker);
+    const uint64_t ybit = 1ull << st.r_in, cmask = ybit - 1ull;
+    // p-space vectors of this step in the input's new coordinates
+    auto to_new_p = [&](uint64_t p) { return apply_cols(t_in, p & cmask) | (p & ybit); };
+    uint64_t t_in_inv[64];
+    if (j == 0) {
+      for (int i = 0; i < 64; ++i) t_in[i] = t_in_inv[i] = 1ull << i;
+    } else {
+      gf2_invert_cols(st.r_in, t_in, t_in_inv);
+    }
+    // beta in new input coordinates: bit k = <beta, T^-1 e_k>
+    uint64_t beta = 0;
+    for (int k = 0; k < st.r_in; ++k) beta |= (uint64_t)parity64(st.beta & t_in_inv[k]) << k;
+    // the two forms on the step's output coordinates
+    uint64_t fy = 0, fs = 0;
+    for (int k = 0; k < st.r_out; ++k) {
+      fy |= ((st.lcols[k] >> st.r_in) & 1ull) << k;
+      fs |= (uint64_t)parity64(st.lcols[k] & cmask & st.beta) << k;
+    }
+    uint64_t out_inv[64];
+    const bool uniform = st.r_out >= 8 && choose_coordinates(st.r_out, fy, fs, out_inv);
+    if (!uniform)
+      for (int k = 0; k < st.r_out; ++k) out_inv[k] = 1ull << k;
+    // lcols in both new coordinates: L^ e_k = P L (T_out^-1 e_k)
+    uint64_t lcols[64];
+    for (int k = 0; k < st.r_out; ++k) lcols[k] = to_new_p(apply_cols(st.lcols, out_inv[k]));
     if (blob) {
       blob[2 + j] = off;
       uint64_t* p = blob + off;
-      p[0] = (uint64_t)st.v | ((uint64_t)st.r_in << 8) | ((uint64_t)st.r_out << 16) | ((uint64_t)st.nker << 24);
-      p[1] = st.beta;
+      p[0] = (uint64_t)st.v | ((uint64_t)st.r_in << 8) | ((uint64_t)st.r_out << 16) | ((uint64_t)st.nker << 24) |
+             ((uint64_t)uniform << 33);
+      p[1] = beta;
       uint64_t kern[64];
-      for (int k = 0; k < st.nker; ++k) kern[k] = st.kern[k];
-      const uint64_t ybit = 1ull << st.r_in;
+      for (int k = 0; k < st.nker; ++k) kern[k] = to_new_p(st.kern[k]);
       int piv = -1;
       for (int k = 0; k < st.nker; ++k)
         if (kern[k] & ybit) {
@@ -390,13 +499,13 @@ extern "C" int rgs_scan_pack(const rgs_scan_step* steps, int n, uint64_t* blob, 
       for (int c = 0; c < 64; ++c) {
         uint64_t x = 0;
         for (int i = 0; i < 6 && i < st.r_out; ++i)
-          if ((c >> i) & 1) x ^= st.lcols[i];
+          if ((c >> i) & 1) x ^= lcols[i];
         p[2 + c] = (x & ybit) ? x ^ kpiv : x;
       }
       for (uint64_t k = 0; k < hi; ++k) {
         uint64_t x = 0;
         for (int i = 6; i < st.r_out; ++i)
-          if ((k >> (i - 6)) & 1) x ^= st.lcols[i];
+          if ((k >> (i - 6)) & 1) x ^= lcols[i];
         p[66 + k] = (x & ybit) ? x ^ kpiv : x;
       }
       uint64_t* cb = p + 66 + hi;
@@ -404,6 +513,8 @@ extern "C" int rgs_scan_pack(const rgs_scan_step* steps, int n, uint64_t* blob, 
       for (int k = 0; k < st.nker; ++k)
         for (uint64_t t = 0; t < (1ull << k); ++t) cb[(1ull << k) + t] = cb[t] ^ kern[k];
     }
+    // the next step's input coordinates are this step's output coordinates
+    gf2_invert_cols(st.r_out, out_inv, t_in);
   }
   if (blob) {
     blob[0] = (uint64_t)n;
