@@ -198,14 +198,13 @@ def run_scan(torch, count=65536, seed=70, reps=5):
     (the scorer) and complex128 tables (the audit); mean over `reps` launches
     timed with events on the scorer's stream."""
     from rgs70_classical_challenge_amd.instances import instance
-    from rgs70_classical_challenge_amd.scan import ScanOracle, order_search, bits_to_words
+    from rgs70_classical_challenge_amd.scan import ScanOracle, bits_to_words
     g, u = instance(70, seed)
-    order, _ = order_search(g, iterations=1000000, rmax_cap=28)
-    so = ScanOracle(g, u, order)
+    so = ScanOracle(g, u, 'auto')            # the autotuner's order (tuned.json)
     bits = np.random.default_rng(seed).integers(0, 2, (count, 70), dtype=np.uint8)
     words = torch.as_tensor(bits_to_words(bits).view(np.int64)).to(so.device)
     out = {'workload': f'70-vertex 3-regular, {count} uniform proposals per launch (device-resident)',
-           'rmax': so.rmax, 'sum_states': so.plan.sum_states}
+           'rmax': so.rmax, 'sum_states': so.plan.sum_states, 'order': 'autotuner (tuned.json)'}
     for prec in (64, 128):
         so.probabilities_device(words, prec)
         torch.cuda.synchronize()

@@ -179,13 +179,101 @@ def tune_conditional(g, u, A, B, reps=2):
     return out
 
 
+def graph_key(g):
+    """Table key of a graph's scan order: 'scan:' and the first 16 hex digits
+    of the SHA-256 of its adjacency bytes (uint8, row-major)."""
+    import hashlib
+    a = np.ascontiguousarray(np.asarray(g, np.uint8))
+    return 'scan:' + hashlib.sha256(a.tobytes()).hexdigest()[:16]
+
+
+def scan_order(g, props=None):
+    """The tuned scan order of a graph on this device, or None."""
+    import torch
+    props = props or torch.cuda.get_device_properties(0)
+    e = lookup_props(graph_key(g), props)
+    return list(e['order']) if e else None
+
+
+def tune_scan_order(g, u, start=None, iterations=1_000_000, threads=8, count=65536, reps=3, seed=20261030,
+                    log=None):
+    """Pick the scan order of a graph: the annealer (deterministic: fixed
+    iterations, chains and seed) runs at rank caps from the unconstrained
+    optimum's rank downwards; each order within the batch kernels' reach is
+    timed on ``count`` uniform proposals (complex64 tables, best of ``reps``
+    launches) and the fastest kept; above that reach (HBM-table scans, one
+    proposal at a time) the order with the smallest sum of 2^r is kept.
+    Returns the table entry."""
+    import torch
+    from .scan import BATCH_RMAX, ScanOracle, bits_to_words, order_search, profile
+    log = log or (lambda s: print(s, file=sys.stderr, flush=True))
+    n = len(g)
+    cands = []
+    order, cost = order_search(g, start, iterations=iterations, rmax_cap=62, threads=threads, seed=seed)
+    top = max(profile(g, order))
+    cands.append((top, order, cost))
+    # every capped search starts from the caller's order: a cap at the
+    # unconstrained optimum's own rank often finds a cheaper order still
+    cap = top
+    while cap >= 1:
+        try:
+            o, c = order_search(g, start, iterations=iterations, rmax_cap=cap, threads=threads, seed=seed)
+        except ValueError:
+            break
+        cands.append((max(profile(g, o)), o, c))
+        cap = min(cap, max(profile(g, o))) - 1
+    rng = np.random.default_rng(seed)
+    words = None
+    best = None
+    for rmax, o, c in cands:
+        entry = {'order': [int(v) for v in o], 'rmax': int(rmax), 'sum_states': int(c), 'iterations': iterations,
+                 'threads': threads, 'seed': seed}
+        if rmax <= BATCH_RMAX[64]:
+            so = ScanOracle(g, u, o)
+            if words is None:
+                bits = rng.integers(0, 2, (count, n), dtype=np.uint8)
+                words = torch.as_tensor(bits_to_words(bits).view(np.int64)).to(so.device)
+            so.probabilities_device(words, 64)
+            torch.cuda.synchronize()
+            ms = float('inf')
+            for _ in range(reps):
+                e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                e0.record(so.stream)
+                so.probabilities_device(words, 64)
+                e1.record(so.stream)
+                torch.cuda.synchronize()
+                ms = min(ms, e0.elapsed_time(e1))
+            entry['ms'] = round(ms, 4)
+            entry['proposals'] = count
+            log(f'[autotune] scan order rank {rmax}, sum 2^r {c}: {ms:.3f} ms per {count} proposals')
+            if best is None or 'ms' not in best or ms < best['ms']:
+                best = entry
+        else:
+            log(f'[autotune] scan order rank {rmax}, sum 2^r {c} (HBM tables: cost only)')
+            if best is None or ('ms' not in best and c < best['sum_states']):
+                best = entry
+    return best
+
+
 def main(argv):
     """argv: lo [hi [output path]] - the plain pipeline for n = lo..hi; or
+    --scan instance.npz [output path] - the scan order of the instance's
+    graph (arrays graph, measurement_unitaries), stored under 'scan:<hash>'; or
     --conditional instance.npz [output path] - the conditional pipeline of the
     instance in the file (arrays graph, measurement_unitaries, A, B), stored
     under the key 'c<m>'.  The table is merged into the output file (default:
     the package's tuned.json)."""
     import torch
+    if argv and argv[0] == '--scan':
+        d = np.load(argv[1])
+        out = argv[2] if len(argv) > 2 else TABLE
+        table = json.load(open(out)) if os.path.exists(out) else load_table()
+        entry = tune_scan_order(d['graph'], d['measurement_unitaries'])
+        table.setdefault(device_key(torch), {})[graph_key(d['graph'])] = entry
+        with open(out, 'w') as f:
+            json.dump(table, f, indent=1, sort_keys=True)
+        print(json.dumps({k: v for k, v in entry.items() if k != 'order'}), flush=True)
+        return
     if argv and argv[0] == '--conditional':
         d = np.load(argv[1])
         out = argv[2] if len(argv) > 2 else TABLE

@@ -68,7 +68,9 @@ def run(n=34, seed=20261205, train_samples=200_000, steps=20_000, batch=512, hid
     out['heldout_ceiling_nats'] = float(np.mean(np.log(held_p * 2.0 ** n)))
 
     # 3. rejection, judged by the exact scan and audited in complex128
-    order, _ = order_search(g, iterations=order_iterations, rmax_cap=12)
+    from .autotune import scan_order
+    order = scan_order(g, torch.cuda.get_device_properties(0)) or \
+        order_search(g, iterations=order_iterations, rmax_cap=12)[0]
     scorer = ScanOracle(g, u, order)
     auditor = ScanOracle(g, u, order, fp64=True)
     out['scan_rmax'] = scorer.rmax

@@ -134,14 +134,22 @@ class ScanOracle:
     """p(x) by the linear rank-width scan on one device (drop-in for ScanOracle)."""
 
     def __init__(self, g, u, order, source=None, device=0, gpu=True, fp64=False, batch_rmax=None):
-        """batch_rmax: largest rank scored by the one-launch batch kernels
-        (default BATCH_RMAX, their LDS limit); above it proposals are scored
-        one at a time with HBM tables."""
+        """order: the vertex order, or 'auto' for the autotuner's order of
+        this graph on this device (tune_scan_order; an annealed order when the
+        table has none).  batch_rmax: largest rank scored by the one-launch
+        batch kernels (default BATCH_RMAX, their LDS limit); above it
+        proposals are scored one at a time with HBM tables."""
         import torch
         if not gpu:
             raise ValueError('this scorer runs on the device only')
         if not torch.cuda.is_available():
             raise RuntimeError('no HIP device visible')
+        if isinstance(order, str):
+            if order != 'auto':
+                raise ValueError("order must be a vertex order or 'auto'")
+            from .autotune import scan_order
+            order = scan_order(g, torch.cuda.get_device_properties(device)) or \
+                order_search(g, iterations=1_000_000, rmax_cap=28, threads=8)[0]
         self.torch = torch
         self.n = len(g)
         self.u = np.ascontiguousarray(np.asarray(u, np.complex128))

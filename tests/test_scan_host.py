@@ -323,3 +323,42 @@ def test_packed_coordinates_make_weights_and_signs_uniform():
     for _ in range(3):
         bits = rng.integers(0, 2, n)
         assert _packed_amplitude(blob, u, bits) == pytest.approx(osc.amplitude(steps, u, bits), abs=1e-13)
+
+
+def test_tuned_scan_orders_are_valid(golden_dir):
+    """Every scan order in the autotuner's table is a permutation whose
+    recorded rank and sum of 2^r are its profile's; the 70-qubit instance's
+    order costs no more than the reference's optimised order (7,919,632,381,
+    order_optimization/search_summary.json)."""
+    import json
+    from rgs70_classical_challenge_amd import autotune
+    table = json.load(open(autotune.TABLE))
+    rgs = np.load(f'{golden_dir}/rgs_graph.npz')['graph']
+    seen = 0
+    for dev, entries in table.items():
+        for key, e in entries.items():
+            if not key.startswith('scan:'):
+                continue
+            seen += 1
+            n = len(e['order'])
+            assert sorted(e['order']) == list(range(n))
+            if key == autotune.graph_key(rgs):
+                prof = scan.profile(rgs, e['order'])
+                assert max(prof) == e['rmax'] == 30
+                assert sum(1 << r for r in prof) == e['sum_states'] <= 7919632381
+    assert seen >= 3
+
+
+@pytest.mark.slow
+def test_order_search_beats_reference_optimizer_on_rgs70(golden_dir):
+    """The annealer, from the reference's starting order with a fixed budget
+    (8 chains x 8,000,000 moves, seed 20261030, rank cap 30), reaches a lower
+    sum of 2^r than the reference's own 155-second search (7,919,632,381),
+    and is reproducible."""
+    d = np.load(f'{golden_dir}/rgs_graph.npz')
+    start = np.load(f'{golden_dir}/scan_order.npy').tolist()
+    order, cost = scan.order_search(d['graph'], start, iterations=8_000_000, rmax_cap=30, threads=8)
+    prof = scan.profile(d['graph'], order)
+    assert cost == sum(1 << r for r in prof) and max(prof) <= 30
+    assert cost <= 7919632381, cost
+    assert cost == 7854882813
