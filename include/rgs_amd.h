@@ -182,6 +182,34 @@ int rgs_slab_pass(const rgs_slab_desc *d, int pass, void *stream);
  * ..."), for logs and the autotuner. */
 int rgs_slab_describe(const rgs_slab_desc *d, int *passes, char *text, int text_len);
 
+/* Many independent states in few launches.  Each item names a state's slab
+ * descriptor (its `mass` level is required: the pyramid buffer must hold
+ * 2^(nbits-mass_bits+1) - 1 doubles, as for rgs_mass_pyramid) and its draws
+ * (dev uniforms, samples, dev outputs as rgs_draw).  Items whose descriptors
+ * give the same pass schedule share launches: one per pass for the whole
+ * group, then one per chunk of pyramid levels and one for every draw of every
+ * item.  The results equal rgs_slab_pipeline + rgs_mass_pyramid + rgs_draw
+ * per item bit for bit (the same kernels).  create plans the batch and
+ * uploads its kernel arguments (one device allocation, synchronous); run only
+ * enqueues kernel launches on `stream` (it can be captured in a graph).
+ * The descriptors are copied at create; the device buffers they and the items
+ * name must stay valid while the batch is used.  create returns RGS_EINVAL
+ * when one launch would need 2^31 work-items or more (a pass of a group, a
+ * pyramid round, or 2^25 draws): split such a batch.  No
+ * counterpart in the reference, which runs one state per process
+ * (work/rgs_attack/born_production.py:44-58). */
+typedef struct rgs_slab_batch_item {
+  const rgs_slab_desc *desc;   /* HOST descriptor */
+  const double *uniforms;      /* dev float64[samples] */
+  int samples;
+  int64_t *out_index;          /* dev int64[samples] */
+  double *out_p;               /* dev float64[samples] */
+} rgs_slab_batch_item;
+int rgs_slab_batch_create(const rgs_slab_batch_item *items, int count, void **batch);
+int rgs_slab_batch_run(void *batch, void *stream);
+int rgs_slab_batch_info(void *batch, int *launches, int *groups);
+int rgs_slab_batch_destroy(void *batch);
+
 /* Build the upper levels of the FP64 mass pyramid in place.  pyr holds level 0
  * (2^base_bits masses) followed by levels of 2^(base_bits-1), ..., 1 entries:
  * level[k][i] = level[k-1][2i] + level[k-1][2i+1].  Replaces the host

@@ -54,6 +54,16 @@ class SlabDesc(ctypes.Structure):
     ]
 
 
+class SlabBatchItem(ctypes.Structure):
+    _fields_ = [
+        ('desc', P(SlabDesc)),
+        ('uniforms', c_void_p),
+        ('samples', c_int),
+        ('out_index', c_void_p),
+        ('out_p', c_void_p),
+    ]
+
+
 # name -> (restype, argtypes)
 SIGNATURES = {
     'rgs_last_error': (c_char_p, []),
@@ -71,6 +81,10 @@ SIGNATURES = {
     'rgs_mass_pyramid': (c_int, [c_void_p, c_int, c_void_p]),
     'rgs_draw': (c_int, [c_void_p, c_int, c_void_p, c_int, c_void_p, c_int, c_void_p, c_void_p, c_void_p]),
     'rgs_gather_prob': (c_int, [c_void_p, c_void_p, c_int, c_void_p, c_void_p]),
+    'rgs_slab_batch_create': (c_int, [c_void_p, c_int, P(c_void_p)]),
+    'rgs_slab_batch_run': (c_int, [c_void_p, c_void_p]),
+    'rgs_slab_batch_info': (c_int, [c_void_p, P(c_int), P(c_int)]),
+    'rgs_slab_batch_destroy': (c_int, [c_void_p]),
 }
 
 _lib = None

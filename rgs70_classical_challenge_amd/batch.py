@@ -8,6 +8,11 @@ whole batch as ONE graph: each wave forks its instances over several streams
 (default priority) and joins them before the next wave starts, so the batch
 is a single ``graphLaunch`` from the host.  ``run_graph`` captures one graph
 per wave; ``run_eager`` is the one-launch-sequence-per-instance baseline.
+``run_fused`` hands each wave to the native multi-state batch
+(rgs_slab_batch_*): instances with the same pass schedule share every pass's
+launch (k_tile_multi), and all pyramids and draws of a wave take a few
+launches (k_pyramid_multi, k_draw_multi), so the launch count no longer grows
+with the number of instances.
 Tile shapes per n come from the autotuner table (``autotune.py``) when
 present.  The reference runs one state per process invocation
 (work/rgs_attack/born_production.py:44-58); no batched counterpart exists.
@@ -214,6 +219,79 @@ class BatchSampler:
         self.torch.cuda.synchronize()
         return {it['i']: it['idx'].cpu().numpy() for items in waves for it in items}
 
+    def build_fused(self):
+        """Every wave as one multi-state batch (rgs_slab_batch_create): the
+        wave's instances with the same pass schedule (same n and tile plan)
+        share each pass's launch, and the pyramids and draws of the whole
+        wave take a few launches more.  Waves share one arena, as in
+        build_single; each wave's kernel arguments are uploaded once here."""
+        import ctypes
+        self.close_fused()
+        sizes = [sum(footprint(self.specs[i][0], self.samples) for i in wave) for wave in self.waves]
+        self.arena = Arena(max(sizes) + (64 << 20))
+        self.fused = []
+        for wave in self.waves:
+            self.arena.reset()
+            items = self._build(wave, arena=self.arena)
+            arr = (_lib.SlabBatchItem * len(items))()
+            for k, it in enumerate(items):
+                arr[k].desc = ctypes.pointer(it['s']._desc)
+                arr[k].uniforms = ptr(it['uni'])
+                arr[k].samples = self.samples
+                arr[k].out_index = ptr(it['idx'])
+                arr[k].out_p = ptr(it['p'])
+            h = ctypes.c_void_p()
+            call('rgs_slab_batch_create', arr, len(items), ctypes.byref(h))
+            self.fused.append((h, items))
+        return self.fused
+
+    def fused_launches(self):
+        """(kernel launches, pass-schedule groups) summed over the waves."""
+        import ctypes
+        launches = groups = 0
+        for h, _ in self.fused:
+            a, b = ctypes.c_int(), ctypes.c_int()
+            call('rgs_slab_batch_info', h, ctypes.byref(a), ctypes.byref(b))
+            launches += a.value
+            groups += b.value
+        return launches, groups
+
+    def launch_fused(self, stream=None):
+        """Enqueue every wave's batch on one stream (the current one by
+        default); waves run in order, so a wave may reuse the arena."""
+        st = (stream or self.torch.cuda.current_stream()).cuda_stream
+        for h, _ in self.fused:
+            call('rgs_slab_batch_run', h, st)
+
+    def run_fused(self, replays=1):
+        """Run the fused batch; returns {instance: drawn indices}."""
+        if not getattr(self, 'fused', None):
+            self.build_fused()
+        for _ in range(replays):
+            self.launch_fused()
+        self.torch.cuda.synchronize()
+        return {it['i']: it['idx'].cpu().numpy() for _, items in self.fused for it in items}
+
+    def capture_fused(self):
+        """The fused batch captured as one graph (its launches only)."""
+        torch = self.torch
+        if not getattr(self, 'fused', None):
+            self.build_fused()
+        self.launch_fused()                 # warm outside capture
+        torch.cuda.synchronize()
+        main = torch.cuda.Stream()
+        graph = torch.cuda.CUDAGraph()
+        with torch.cuda.stream(main):
+            graph.capture_begin()
+            self.launch_fused(main)
+            graph.capture_end()
+        return graph
+
+    def close_fused(self):
+        for h, _ in getattr(self, 'fused', None) or []:
+            call('rgs_slab_batch_destroy', h)
+        self.fused = None
+
     def benchmark(self, reps=3):
         """Device time of the whole batch (state, masses, draws of every
         instance): one launch sequence per instance on one stream, against
@@ -257,6 +335,28 @@ class BatchSampler:
         del graph
         self.arena = None
         torch.cuda.empty_cache()
+        # the fused batch: instances sharing a pass schedule share launches
+        self.build_fused()
+        launches, groups = self.fused_launches()
+        self.launch_fused()
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        for _ in range(reps):
+            self.launch_fused()
+        torch.cuda.synchronize()
+        fused_s = (time.perf_counter() - t) / reps
+        fgraph = self.capture_fused()
+        fgraph.replay()
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        for _ in range(reps):
+            fgraph.replay()
+        torch.cuda.synchronize()
+        fused_graph_s = (time.perf_counter() - t) / reps
+        del fgraph
+        self.close_fused()
+        self.arena = None
+        torch.cuda.empty_cache()
         total = len(self.specs) * self.samples
         amps = sum(1 << n for n, _ in self.specs)
         return {'instances': len(self.specs), 'samples_per_instance': self.samples, 'waves': len(self.waves),
@@ -265,7 +365,11 @@ class BatchSampler:
                 'single_graph_ms': round(single_s * 1e3, 3),
                 'eager_samples_per_s': round(total / eager_s, 1), 'graph_samples_per_s': round(total / graph_s, 1),
                 'single_graph_samples_per_s': round(total / single_s, 1),
-                'graph_speedup': round(eager_s / graph_s, 3), 'single_graph_speedup': round(eager_s / single_s, 3)}
+                'graph_speedup': round(eager_s / graph_s, 3), 'single_graph_speedup': round(eager_s / single_s, 3),
+                'fused_ms': round(fused_s * 1e3, 3), 'fused_graph_ms': round(fused_graph_s * 1e3, 3),
+                'fused_launches': launches, 'fused_groups': groups,
+                'fused_samples_per_s': round(total / fused_s, 1),
+                'fused_speedup': round(eager_s / fused_s, 3), 'fused_graph_speedup': round(eager_s / fused_graph_s, 3)}
 
 
 def main(argv):
@@ -287,7 +391,9 @@ def main(argv):
         sres = BatchSampler(small, samples=1000).benchmark()
         res['small_instances'] = {'n_max': 22, 'instances': len(small),
                                   **{k: sres[k] for k in ('eager_ms', 'graph_ms', 'single_graph_ms', 'graph_speedup',
-                                                          'single_graph_speedup')}}
+                                                          'single_graph_speedup', 'fused_ms', 'fused_graph_ms',
+                                                          'fused_launches', 'fused_groups', 'fused_speedup',
+                                                          'fused_graph_speedup')}}
     print(json.dumps(res), flush=True)
 
 

@@ -1,6 +1,7 @@
 // FP64 mass pyramid, device draws and probability gathers on a complex64 state.
 #include "rgs_common.h"
 #include "rgs_amd.h"
+#include "multi.h"
 
 #include <algorithm>
 
@@ -8,11 +9,10 @@ namespace rgs {
 
 // ------------------------------------------------------------------ pyramid
 // One workgroup reduces 2^k consecutive entries of level L into levels L+1..L+k.
-__global__ void __launch_bounds__(256) k_pyramid(double* pyr, uint64_t level_off, int level_bits, int k,
-                                                 uint64_t out_off_first) {
+__device__ __forceinline__ void pyramid_group(double* pyr, uint64_t level_off, int level_bits, int k,
+                                              uint64_t out_off_first, uint64_t group) {
   __shared__ double s[512];
   const uint32_t tid = threadIdx.x;
-  const uint64_t group = blockIdx.x;
   const uint32_t width = 1u << k;  // entries per group, <= 512
   const double* in = pyr + level_off + (group << k);
   for (uint32_t i = tid; i < width; i += 256) s[i] = in[i];
@@ -34,6 +34,29 @@ __global__ void __launch_bounds__(256) k_pyramid(double* pyr, uint64_t level_off
   }
 }
 
+__global__ void __launch_bounds__(256) k_pyramid(double* pyr, uint64_t level_off, int level_bits, int k,
+                                                 uint64_t out_off_first) {
+  pyramid_group(pyr, level_off, level_bits, k, out_off_first, blockIdx.x);
+}
+
+// The index of the job owning workgroup b: the last i with first[i] <= b.
+__device__ __forceinline__ int job_of(const uint32_t* first, int count, uint32_t b) {
+  int lo = 0, hi = count;
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (first[mid] <= b) lo = mid;
+    else hi = mid;
+  }
+  return lo;
+}
+
+__global__ void __launch_bounds__(256) k_pyramid_multi(const PyrJob* __restrict__ jobs,
+                                                       const uint32_t* __restrict__ first, int count) {
+  const int i = job_of(first, count, blockIdx.x);
+  const PyrJob& j = jobs[i];
+  pyramid_group(j.pyr, j.level_off, j.level_bits, j.k, j.out_off, blockIdx.x - first[i]);
+}
+
 // ------------------------------------------------------------------ draw
 // One wavefront per draw.  Lanes hold 4 consecutive amplitudes of the leaf
 // block (mass_bits <= 8 -> 256 amplitudes).  Level l of the pyramid (0 = base
@@ -42,14 +65,11 @@ __device__ __forceinline__ uint64_t level_off(int B, int l) {
   return (1ull << (B + 1)) - (1ull << (B + 1 - l));
 }
 
-__global__ void __launch_bounds__(64) k_draw(const float2* state, int nbits, const double* pyr, int mass_bits,
-                                             const double* uniforms, int count, int64_t* out_index,
-                                             double* out_p) {
-  const int d = blockIdx.x;
-  if (d >= count) return;
+__device__ __forceinline__ void draw_one(const float2* state, int nbits, const double* pyr, int mass_bits, double u,
+                                         int64_t* out_index, double* out_p) {
   const int lane = threadIdx.x;
   const int B = nbits - mass_bits;
-  double t = uniforms[d] * pyr[level_off(B, B)];
+  double t = u * pyr[level_off(B, B)];
   uint64_t idx = 0;
   for (int l = B - 1; l >= 0; --l) {
     const double* lev = pyr + level_off(B, l);
@@ -101,9 +121,36 @@ __global__ void __launch_bounds__(64) k_draw(const float2* state, int nbits, con
     if (t >= q0 && q1 > 0.0) { sub = 1; ps = q1; } else { sub = 0; ps = q0; }
   }
   if (lane == 0) {
-    out_index[d] = (int64_t)(bbase + (uint64_t)(node * 4 + sub));
-    out_p[d] = ps;
+    *out_index = (int64_t)(bbase + (uint64_t)(node * 4 + sub));
+    *out_p = ps;
   }
+}
+
+__global__ void __launch_bounds__(64) k_draw(const float2* state, int nbits, const double* pyr, int mass_bits,
+                                             const double* uniforms, int count, int64_t* out_index,
+                                             double* out_p) {
+  const int d = blockIdx.x;
+  if (d >= count) return;
+  draw_one(state, nbits, pyr, mass_bits, uniforms[d], out_index + d, out_p + d);
+}
+
+__global__ void __launch_bounds__(64) k_draw_multi(const DrawJob* __restrict__ jobs, const uint32_t* __restrict__ first,
+                                                   int count) {
+  const int i = job_of(first, count, blockIdx.x);
+  const DrawJob& j = jobs[i];
+  const uint32_t d = blockIdx.x - first[i];
+  draw_one(j.state, j.nbits, j.pyr, j.mass_bits, j.uniforms[d], j.out_index + d, j.out_p + d);
+}
+
+hipError_t launch_pyramid_multi(const PyrJob* jobs, const uint32_t* first, int count, uint32_t groups,
+                                hipStream_t st) {
+  hipLaunchKernelGGL(k_pyramid_multi, dim3(groups), dim3(256), 0, st, jobs, first, count);
+  return hipGetLastError();
+}
+
+hipError_t launch_draw_multi(const DrawJob* jobs, const uint32_t* first, int count, uint32_t draws, hipStream_t st) {
+  hipLaunchKernelGGL(k_draw_multi, dim3(draws), dim3(64), 0, st, jobs, first, count);
+  return hipGetLastError();
 }
 
 __global__ void k_gather_prob(const float2* state, const int64_t* index, int count, double* out) {

@@ -27,19 +27,16 @@ namespace rgs {
 // ---------------------------------------------------------------- device math
 __device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
 __device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
+// Products round as written (one fma over one product), never as the
+// compiler's contraction chooses: every kernel that inlines them gives the
+// same bits for the same inputs.
 __device__ __forceinline__ float2 cmul(float2 a, float2 b) {
-  return make_float2(a.x * b.x - a.y * b.y, a.x * b.y + a.y * b.x);
+  return make_float2(__fmaf_rn(a.x, b.x, -__fmul_rn(a.y, b.y)), __fmaf_rn(a.x, b.y, __fmul_rn(a.y, b.x)));
 }
 __device__ __forceinline__ float2 cneg_if(float2 a, uint32_t neg) {
   // Flip both signs when neg is 1 (exact; no multiply).
   uint32_t s = neg << 31;
   return make_float2(__uint_as_float(__float_as_uint(a.x) ^ s), __uint_as_float(__float_as_uint(a.y) ^ s));
-}
-// u0*a + u1*b
-__device__ __forceinline__ float2 cmac2(float2 u0, float2 a, float2 u1, float2 b) {
-  float re = u0.x * a.x - u0.y * a.y + u1.x * b.x - u1.y * b.y;
-  float im = u0.x * a.y + u0.y * a.x + u1.x * b.y + u1.y * b.x;
-  return make_float2(re, im);
 }
 // |a|^2 in float64: both squares are exact for float inputs, one rounding in the add.
 __device__ __forceinline__ double prob64(float2 a) {

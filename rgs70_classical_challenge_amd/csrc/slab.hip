@@ -16,6 +16,7 @@
 // gates], groups down [gates], low[gates, mass].  Without it: the first group
 // [init, phase, gates], the other groups [gates], low[gates, mass].
 #include "slab_kernel.h"
+#include "slab_plan.h"
 
 #include <algorithm>
 #include <cstdio>
@@ -24,20 +25,9 @@
 
 namespace rgs {
 
-struct PassPlan {
-  int flags;
-  int low;     // 1: contiguous tile
-  int qstart;
-  int count;   // target bits
-  int tb;      // tile bits
-  int rbits;   // log2 registers per thread
-  int c;       // lane bits
-  int rb;      // log2 LDS rounds
-};
-
 static int chunks_of(int bits) { return (bits + 7) / 8; }
 
-static size_t lds_bytes(const PassPlan& p, int nbits, const rgs_slab_desc* d) {
+size_t lds_bytes(const PassPlan& p, int nbits, const rgs_slab_desc* d) {
   size_t slots = ((size_t)1 << (p.tb - p.rb)) + ((size_t)1 << (p.tb - p.rb - 4));
   size_t bytes = slots * 8;
   // LDS tables: the phase pass's diagonal-phase table (unless it is read from
@@ -51,7 +41,7 @@ static size_t lds_bytes(const PassPlan& p, int nbits, const rgs_slab_desc* d) {
   return bytes;
 }
 
-static int plan_passes(const rgs_slab_desc* d, std::vector<PassPlan>& out, std::string& err) {
+int plan_passes(const rgs_slab_desc* d, std::vector<PassPlan>& out, std::string& err) {
   const int n = d->nbits;
   if (n < 6 || n > 40) { err = "nbits must be in 6..40"; return kInvalid; }
   int T0 = std::min(d->low_bits, n);
@@ -213,6 +203,96 @@ static void local_form(const PassPlan& p, const uint64_t* rows, LocalForm& F) {
   }
 }
 
+int check_desc(const rgs_slab_desc* d, std::string& err) {
+  const int n = d->nbits;
+  if (d->init_kind < 0 || d->init_kind > 3) { err = "bad init_kind"; return kInvalid; }
+  if ((d->init_kind == RGS_INIT_GRAPH || d->init_kind == RGS_INIT_COV) && !d->init_lut) {
+    err = "init tables missing";
+    return kInvalid;
+  }
+  if (d->init_kind == RGS_INIT_COV && !d->init_wlut) { err = "covariant weights missing"; return kInvalid; }
+  if (d->init_kind == RGS_INIT_GRAPH && !d->init_rows) { err = "graph init needs its form rows"; return kInvalid; }
+  if (d->init_kind == RGS_INIT_COV && (!d->init_rows || !d->init_cols)) {
+    err = "covariant init needs its form rows and map columns";
+    return kInvalid;
+  }
+  if (d->init_wbits != 0 && d->init_wbits != 8 && d->init_wbits != 9) { err = "init_wbits must be 8 or 9"; return kInvalid; }
+  if (d->phase_lut && !d->phase_rows) { err = "phase_lut needs its form rows"; return kInvalid; }
+  if ((d->phase_lut || d->dphase_lut || d->init_kind == RGS_INIT_GRAPH || d->init_kind == RGS_INIT_COV) && n > 63) {
+    err = "lookup tables need nbits <= 63";
+    return kInvalid;
+  }
+  return kOk;
+}
+
+void build_args(const rgs_slab_desc* d, const PassPlan& p, PassArgs& a) {
+  const int n = d->nbits;
+  a = PassArgs{};
+  a.state = reinterpret_cast<float2*>(d->state);
+  a.init_lut = d->init_lut;
+  a.init_w = reinterpret_cast<const float2*>(d->init_wlut);
+  a.phase_lut = d->phase_lut;
+  a.dphase = reinterpret_cast<const float2*>(d->dphase_lut);
+  // rotations of this pass's target bits travel in the kernel arguments
+  const int t0 = p.low ? 0 : p.qstart;
+  for (int i = 0; i < 16; ++i) {
+    const int q = t0 + i;
+    a.rot[i] = (d->rot && i < p.count && q < n) ? make_float2(d->rot[2 * q], d->rot[2 * q + 1])
+                                                : make_float2(1.f, 0.f);
+  }
+  a.mass = d->mass;
+  a.init_scale = d->init_scale;
+  a.nbits = n;
+  a.qstart = p.low ? 0 : p.qstart;
+  a.flags = p.flags;
+  // a contiguous pass that makes the state and only transforms it walks up
+  // from the lowest layout (slab_kernel.h, F_INIT_UP)
+  const bool init_up = p.low && p.flags == F_WHT;
+  if (init_up) a.flags |= F_INIT_UP;
+  a.init_kind = d->init_kind;
+  const bool gen = !(p.flags & F_LOAD);
+  a.init_chunks = (gen && (d->init_kind == RGS_INIT_GRAPH || d->init_kind == RGS_INIT_COV)) ? chunks_of(n) : 0;
+  a.phase_chunks = ((p.flags & F_PHASE) && d->phase_lut) ? chunks_of(n) : 0;
+  a.dphase_chunks = ((p.flags & F_PHASE) && d->dphase_lut) ? chunks_of(n) : 0;
+  a.gate_limit = d->rot ? std::min(d->gate_limit, n) : 0;
+  {
+    const int s1 = std::max(p.tb - 4, p.c);
+    const int sp = (p.flags & F_WHT) ? p.c : s1;  // layout where the phases are applied
+    const int si = init_up ? p.c : s1;            // layout where the state is made
+    int pi[4], pp[4];
+    for (int i = 0; i < 4; ++i) {
+      pi[i] = global_bit(p, si + i);
+      pp[i] = global_bit(p, sp + i);
+    }
+    if (a.init_chunks) form_tables(d->init_rows, pi, a.iq, &a.ifreg);
+    if (a.init_chunks && d->init_kind == RGS_INIT_GRAPH) local_form(p, d->init_rows, a.iform);
+    if (a.init_chunks && d->init_kind == RGS_INIT_COV)
+      for (int i = 0; i < 4; ++i) a.icol[i] = d->init_cols[pi[i]];
+    if (a.phase_chunks) {
+      form_tables(d->phase_rows, pp, a.pq, &a.pfreg);
+      local_form(p, d->phase_rows, a.pform);
+    }
+  }
+  a.mass_bits = d->mass_bits;
+  a.dphase_global = d->dphase_global ? 1 : 0;
+  a.wbits = d->init_wbits ? d->init_wbits : 8;
+  a.wchunks = (n + a.wbits - 1) / a.wbits;
+}
+
+hipError_t launch_pass_multi(const PassArgs* args, const uint32_t* first, int count, uint64_t tiles,
+                             const PassPlan& p, size_t shmem, hipStream_t st) {
+  switch (p.c) {
+    case 0: return launch_c0_multi(p.tb, p.rb, args, first, count, tiles, shmem, st);
+    case 2: return launch_c2_multi(p.tb, p.rb, args, first, count, tiles, shmem, st);
+    case 3: return launch_c3_multi(p.tb, p.rb, args, first, count, tiles, shmem, st);
+    case 4: return launch_c4_multi(p.tb, p.rb, args, first, count, tiles, shmem, st);
+    case 5: return launch_c5_multi(p.tb, p.rb, args, first, count, tiles, shmem, st);
+    case 6: return launch_c6_multi(p.tb, p.rb, args, first, count, tiles, shmem, st);
+    case 7: return launch_c7_multi(p.tb, p.rb, args, first, count, tiles, shmem, st);
+    default: return hipErrorInvalidValue;
+  }
+}
+
 static int run_passes(const rgs_slab_desc* d, int first, int last, void* stream) {
   if (!d || !d->state) return fail(kInvalid, "null descriptor or state");
   std::vector<PassPlan> plan;
@@ -221,73 +301,14 @@ static int run_passes(const rgs_slab_desc* d, int first, int last, void* stream)
   if (rc) return fail(rc, err);
   if (first >= (int)plan.size()) return fail(kInvalid, "pass index beyond the schedule");
   last = std::min(last, (int)plan.size() - 1);
-  const int n = d->nbits;
-  if (d->init_kind < 0 || d->init_kind > 3) return fail(kInvalid, "bad init_kind");
-  if ((d->init_kind == RGS_INIT_GRAPH || d->init_kind == RGS_INIT_COV) && !d->init_lut)
-    return fail(kInvalid, "init tables missing");
-  if (d->init_kind == RGS_INIT_COV && !d->init_wlut) return fail(kInvalid, "covariant weights missing");
-  if (d->init_kind == RGS_INIT_GRAPH && !d->init_rows) return fail(kInvalid, "graph init needs its form rows");
-  if (d->init_kind == RGS_INIT_COV && (!d->init_rows || !d->init_cols))
-    return fail(kInvalid, "covariant init needs its form rows and map columns");
-  if (d->init_wbits != 0 && d->init_wbits != 8 && d->init_wbits != 9)
-    return fail(kInvalid, "init_wbits must be 8 or 9");
-  if (d->phase_lut && !d->phase_rows) return fail(kInvalid, "phase_lut needs its form rows");
-  if ((d->phase_lut || d->dphase_lut || d->init_kind == RGS_INIT_GRAPH || d->init_kind == RGS_INIT_COV) && n > 63)
-    return fail(kInvalid, "lookup tables need nbits <= 63");
+  rc = check_desc(d, err);
+  if (rc) return fail(rc, err);
   hipStream_t st = reinterpret_cast<hipStream_t>(stream);
   for (int pi = first; pi <= last; ++pi) {
     const PassPlan& p = plan[pi];
-    PassArgs a{};
-    a.state = reinterpret_cast<float2*>(d->state);
-    a.init_lut = d->init_lut;
-    a.init_w = reinterpret_cast<const float2*>(d->init_wlut);
-    a.phase_lut = d->phase_lut;
-    a.dphase = reinterpret_cast<const float2*>(d->dphase_lut);
-    // rotations of this pass's target bits travel in the kernel arguments
-    const int t0 = p.low ? 0 : p.qstart;
-    for (int i = 0; i < 16; ++i) {
-      const int q = t0 + i;
-      a.rot[i] = (d->rot && i < p.count && q < n) ? make_float2(d->rot[2 * q], d->rot[2 * q + 1])
-                                                  : make_float2(1.f, 0.f);
-    }
-    a.mass = d->mass;
-    a.init_scale = d->init_scale;
-    a.nbits = n;
-    a.qstart = p.low ? 0 : p.qstart;
-    a.flags = p.flags;
-    // a contiguous pass that makes the state and only transforms it walks up
-    // from the lowest layout (slab_kernel.h, F_INIT_UP)
-    const bool init_up = p.low && p.flags == F_WHT;
-    if (init_up) a.flags |= F_INIT_UP;
-    a.init_kind = d->init_kind;
-    const bool gen = !(p.flags & F_LOAD);
-    a.init_chunks = (gen && (d->init_kind == RGS_INIT_GRAPH || d->init_kind == RGS_INIT_COV)) ? chunks_of(n) : 0;
-    a.phase_chunks = ((p.flags & F_PHASE) && d->phase_lut) ? chunks_of(n) : 0;
-    a.dphase_chunks = ((p.flags & F_PHASE) && d->dphase_lut) ? chunks_of(n) : 0;
-    a.gate_limit = d->rot ? std::min(d->gate_limit, n) : 0;
-    {
-      const int s1 = std::max(p.tb - 4, p.c);
-      const int sp = (p.flags & F_WHT) ? p.c : s1;  // layout where the phases are applied
-      const int si = init_up ? p.c : s1;            // layout where the state is made
-      int pi[4], pp[4];
-      for (int i = 0; i < 4; ++i) {
-        pi[i] = global_bit(p, si + i);
-        pp[i] = global_bit(p, sp + i);
-      }
-      if (a.init_chunks) form_tables(d->init_rows, pi, a.iq, &a.ifreg);
-      if (a.init_chunks && d->init_kind == RGS_INIT_GRAPH) local_form(p, d->init_rows, a.iform);
-      if (a.init_chunks && d->init_kind == RGS_INIT_COV)
-        for (int i = 0; i < 4; ++i) a.icol[i] = d->init_cols[pi[i]];
-      if (a.phase_chunks) {
-        form_tables(d->phase_rows, pp, a.pq, &a.pfreg);
-        local_form(p, d->phase_rows, a.pform);
-      }
-    }
-    a.mass_bits = d->mass_bits;
-    a.dphase_global = d->dphase_global ? 1 : 0;
-    a.wbits = d->init_wbits ? d->init_wbits : 8;
-    a.wchunks = (n + a.wbits - 1) / a.wbits;
-    hipError_t e = launch_pass(a, p, lds_bytes(p, n, d), st);
+    PassArgs a;
+    build_args(d, p, a);
+    hipError_t e = launch_pass(a, p, lds_bytes(p, d->nbits, d), st);
     if (e != hipSuccess) return fail(kHip, std::string("k_pass launch: ") + hipGetErrorString(e));
   }
   return kOk;

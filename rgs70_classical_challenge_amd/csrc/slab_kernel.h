@@ -275,8 +275,10 @@ __device__ __forceinline__ void rot_reg(float2 (&v)[16], float2 cs) {
   for (int j = 0; j < 16; ++j)
     if (!(j & (1 << B))) {
       const float2 x = v[j], y = v[j | (1 << B)];
-      v[j] = make_float2(cs.x * x.x - cs.y * y.x, cs.x * x.y - cs.y * y.y);
-      v[j | (1 << B)] = make_float2(cs.y * x.x + cs.x * y.x, cs.y * x.y + cs.x * y.y);
+      // rounded as written (see cmul), whatever kernel inlines this
+      v[j] = make_float2(__fmaf_rn(cs.x, x.x, -__fmul_rn(cs.y, y.x)), __fmaf_rn(cs.x, x.y, -__fmul_rn(cs.y, y.y)));
+      v[j | (1 << B)] =
+          make_float2(__fmaf_rn(cs.y, x.x, __fmul_rn(cs.x, y.x)), __fmaf_rn(cs.y, x.y, __fmul_rn(cs.x, y.y)));
     }
 }
 
@@ -346,8 +348,9 @@ __device__ __forceinline__ void walk_rot_up(float2 (&v)[16], float2* lds, uint32
   }
 }
 
+// One tile of a pass: tile index blk of the state described by a.
 template <int TB, int C, int RBW>
-__global__ void __launch_bounds__(1 << (TB - 4)) k_tile(PassArgs a) {
+__device__ __forceinline__ void tile_run(const PassArgs& a, uint64_t blk) {
   using G = Geo<TB, C, RBW>;
   constexpr int S1 = G::S1;
   constexpr int SL = G::at(G::count() - 1);  // == C
@@ -378,7 +381,6 @@ __global__ void __launch_bounds__(1 << (TB - 4)) k_tile(PassArgs a) {
 
   const int mid = a.qstart - C;
   const uint32_t cmask = (1u << C) - 1u;
-  const uint64_t blk = (uint64_t)blockIdx.x + ((uint64_t)blockIdx.y << 24);
   const uint64_t base = ((blk & ((1ull << mid) - 1ull)) << C) | ((blk >> mid) << (a.qstart + TB - C));
   auto gidx = [&](uint32_t l) -> uint64_t {
     return base | (uint64_t)(l & cmask) | ((uint64_t)(l >> C) << a.qstart);
@@ -504,6 +506,44 @@ __global__ void __launch_bounds__(1 << (TB - 4)) k_tile(PassArgs a) {
   for (int j = 0; j < 16; ++j) store_stream(a.state + g0 + ((uint64_t)j << gsh), v[j]);
 }
 
+template <int TB, int C, int RBW>
+__global__ void __launch_bounds__(1 << (TB - 4)) k_tile(PassArgs a) {
+  tile_run<TB, C, RBW>(a, (uint64_t)blockIdx.x + ((uint64_t)blockIdx.y << 24));
+}
+
+// The same pass over many states of one plan in one launch: workgroup b runs
+// tile b - first[i] of state i, i the last entry with first[i] <= b
+// (first[count] = total tiles); the states' arguments are read from global
+// memory.
+template <int TB, int C, int RBW>
+__global__ void __launch_bounds__(1 << (TB - 4)) k_tile_multi(const PassArgs* __restrict__ args,
+                                                             const uint32_t* __restrict__ first, int count) {
+  const uint32_t b = blockIdx.x + (blockIdx.y << 24);
+  if (b >= first[count]) return;   // the grid's last row may run past the tiles
+  int lo = 0, hi = count;
+  while (hi - lo > 1) {
+    const int mid = (lo + hi) >> 1;
+    if (first[mid] <= b) lo = mid;
+    else hi = mid;
+  }
+  tile_run<TB, C, RBW>(args[lo], b - first[lo]);
+}
+
+template <int TB, int C, int RBW>
+hipError_t launch_tile_multi(const PassArgs* args, const uint32_t* first, int count, uint64_t tiles, size_t shmem,
+                             hipStream_t st) {
+  static bool attr_set = false;
+  if (!attr_set) {
+    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_tile_multi<TB, C, RBW>),
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)kLdsBudget);
+    if (e != hipSuccess) return e;
+    attr_set = true;
+  }
+  dim3 grid((unsigned)(tiles < (1ull << 24) ? tiles : (1ull << 24)), (unsigned)((tiles + (1ull << 24) - 1) >> 24));
+  hipLaunchKernelGGL((k_tile_multi<TB, C, RBW>), grid, dim3(1 << (TB - 4)), shmem, st, args, first, count);
+  return hipGetLastError();
+}
+
 // One workgroup per tile.
 template <int TB, int C, int RBW>
 hipError_t launch_tile(const PassArgs& a, size_t shmem, hipStream_t st) {
@@ -529,5 +569,17 @@ hipError_t launch_c4(int tb, int rbw, const PassArgs& a, size_t shmem, hipStream
 hipError_t launch_c5(int tb, int rbw, const PassArgs& a, size_t shmem, hipStream_t st);
 hipError_t launch_c6(int tb, int rbw, const PassArgs& a, size_t shmem, hipStream_t st);
 hipError_t launch_c7(int tb, int rbw, const PassArgs& a, size_t shmem, hipStream_t st);
+// The multi-state launches of the same specialisations (tiles: the total).
+#define RGS_MULTI_DECL(NAME) \
+  hipError_t NAME(int tb, int rbw, const PassArgs* args, const uint32_t* first, int count, uint64_t tiles, \
+                  size_t shmem, hipStream_t st);
+RGS_MULTI_DECL(launch_c0_multi)
+RGS_MULTI_DECL(launch_c2_multi)
+RGS_MULTI_DECL(launch_c3_multi)
+RGS_MULTI_DECL(launch_c4_multi)
+RGS_MULTI_DECL(launch_c5_multi)
+RGS_MULTI_DECL(launch_c6_multi)
+RGS_MULTI_DECL(launch_c7_multi)
+#undef RGS_MULTI_DECL
 
 }  // namespace rgs

@@ -273,7 +273,7 @@ class ScanOracle:
 def main(argv):
     """``python -m rgs70_classical_challenge_amd.scan [count [precision [seed]]]``:
     p(x) of ``count`` uniform proposals of the 70-vertex 3-regular instance
-    (seeded QR unitaries, annealed order) in one launch each, best of five;
+    (seeded QR unitaries, the autotuner's order) in one launch each, best of five;
     prints one JSON line with proposals/s."""
     import json
     import torch
@@ -282,8 +282,7 @@ def main(argv):
     precision = int(argv[1]) if len(argv) > 1 else 64
     seed = int(argv[2]) if len(argv) > 2 else 70
     g, u = instance(70, seed)
-    order, _ = order_search(g, iterations=1000000, rmax_cap=28)
-    so = ScanOracle(g, u, order)
+    so = ScanOracle(g, u, 'auto')            # the autotuner's order of this graph
     bits = np.random.default_rng(seed).integers(0, 2, (count, 70), dtype=np.uint8)
     words = torch.as_tensor(bits_to_words(bits).view(np.int64)).to(so.device)
     so.probabilities_device(words, precision)

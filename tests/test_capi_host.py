@@ -245,6 +245,8 @@ def test_tuning_table_key_and_entries():
     table = json.load(open(autotune.TABLE))
     assert 'gfx950_256cu' in table
     for key, e in table['gfx950_256cu'].items():
+        if key.startswith('scan:'):          # scan orders: checked in test_scan_host
+            continue
         cond = key.startswith('c')           # conditional pipeline entries: 'c<m>'
         n = int(key[1:] if cond else key)
         init = _lib.INIT_COV if cond else _lib.INIT_GRAPH
@@ -276,3 +278,38 @@ def test_bench_plan_is_the_plan_the_gpu_tests_check():
         assert e.get('top_bits', 0) == 0 and e.get('phase_round_bits', -1) in (-1, 0) and not e.get('dphase_global')
     n, text = describe(34, walsh=True, low=13, tile=14, lane=6, rounds=0, order=1)
     assert {tok.split('/', 1)[1] for tok in text.split()} == {'tb13/c0/rb0', 'tb13/c6/rb0'}, text
+
+
+def test_slab_batch_rejects_bad_items_before_touching_the_device():
+    """rgs_slab_batch_create validates every item on the host (no allocation,
+    no launch) and names the offending item."""
+    def item(desc, samples=10):
+        it = _lib.SlabBatchItem()
+        it.desc = ctypes.pointer(desc)
+        it.uniforms, it.samples, it.out_index, it.out_p = 1, samples, 1, 1
+        return it
+
+    def desc(mass_bits=8, state=1):
+        d = _lib.SlabDesc()
+        d.state, d.nbits, d.init_kind, d.init_lut, d.init_rows, d.do_walsh = state, 16, _lib.INIT_GRAPH, 1, 1, 1
+        d.mass, d.mass_bits = 1, mass_bits
+        d.low_bits, d.tile_bits, d.lane_bits = 12, 12, 4
+        d.phase_round_bits = -1
+        return d
+
+    h = ctypes.c_void_p()
+    with pytest.raises(ValueError, match='need items'):
+        _lib.call('rgs_slab_batch_create', None, 0, ctypes.byref(h))
+    good, no_state, wide = desc(), desc(state=None), desc(mass_bits=9)
+    for bad, msg in ((no_state, 'state and a mass'), (wide, 'mass_bits')):
+        arr = (_lib.SlabBatchItem * 2)(item(good), item(bad))
+        with pytest.raises(ValueError, match=msg):
+            _lib.call('rgs_slab_batch_create', arr, 2, ctypes.byref(h))
+        assert not h.value
+    arr = (_lib.SlabBatchItem * 1)(item(good, samples=-1))
+    with pytest.raises(ValueError, match='draw buffers'):
+        _lib.call('rgs_slab_batch_create', arr, 1, ctypes.byref(h))
+    # a null handle is an error for run and info, and a no-op for destroy
+    with pytest.raises(ValueError, match='null handle'):
+        _lib.call('rgs_slab_batch_run', None, None)
+    _lib.call('rgs_slab_batch_destroy', None)
