@@ -173,6 +173,45 @@ __device__ __forceinline__ void cov_init(float2 (&v)[16], uint64_t x0, const Pas
   }
 }
 
+// The same init with the chunk count fixed at compile time.  Register j's
+// chunk-c index is the thread's index (N x0)_c XOR the uniform (M j)_c, M the
+// register columns, so each lookup is one XOR of a per-thread byte offset
+// (chunk base folded in: the index bits lie below it) with a scalar one,
+// instead of a 64-bit shift and mask per lookup.  The products are taken in
+// the same order as cov_init, so the results are the same bits.
+template <int WCH>
+__device__ __forceinline__ void cov_init_fixed(float2 (&v)[16], uint64_t x0, const PassArgs& a,
+                                               const uint64_t* lut, const float2* w) {
+  uint64_t b0 = 0, acc = 0;
+  for (int c = 0; c < a.init_chunks; ++c) {
+    const uint32_t e = (c << 8) | ((x0 >> (8 * c)) & 255u);
+    b0 ^= lut[2 * e];
+    acc ^= lut[2 * e + 1];
+  }
+  const uint32_t sg = register_signs((uint32_t)__popcll(acc & (x0 | (1ull << 63))) & 1u, x0, a.iq, a.ifreg);
+  const int wb = a.wbits;
+  const uint32_t wm = (1u << wb) - 1u;
+  uint32_t off[WCH];   // byte offset of the thread's entry in chunk c
+#pragma unroll
+  for (int c = 0; c < WCH; ++c) off[c] = ((uint32_t)(b0 >> (wb * c)) & wm) << 3 | (uint32_t)c << (wb + 3);
+  const unsigned char* wb8 = reinterpret_cast<const unsigned char*>(w);
+#pragma unroll
+  for (int j = 0; j < 16; ++j) {
+    uint64_t dj = 0;   // uniform: M j
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+      if (j & (1 << i)) dj ^= a.icol[i];
+    float2 val = make_float2(0.f, 0.f);
+#pragma unroll
+    for (int c = 0; c < WCH; ++c) {
+      const uint32_t d8 = ((uint32_t)(dj >> (wb * c)) & wm) << 3;
+      const float2 e = *reinterpret_cast<const float2*>(wb8 + (off[c] ^ d8));
+      val = c ? cmul(val, e) : e;
+    }
+    v[j] = cneg_if(val, (sg >> j) & 1u);
+  }
+}
+
 // State traffic is streamed once per pass: non-temporal loads and stores keep
 // it from displacing the lookup tables and the neighbouring tiles' rows in
 // the caches (measured: 4-7% faster strided passes at n = 34).
@@ -405,7 +444,19 @@ __device__ __forceinline__ void tile_run(const PassArgs& a, uint64_t blk) {
 #pragma unroll
     for (int j = 0; j < 16; ++j) v[j] = cneg_if(make_float2(a.init_scale, 0.f), (e >> j) & 1u);
   } else if (a.init_kind == RGS_INIT_COV) {
-    cov_init(v, gi, a, lut_init, lut_w);
+    if constexpr (C == 0) {
+      // the contiguous pass that makes the state: chunk counts of m <= 45
+      switch (a.wchunks) {
+        case 1: cov_init_fixed<1>(v, gi, a, lut_init, lut_w); break;
+        case 2: cov_init_fixed<2>(v, gi, a, lut_init, lut_w); break;
+        case 3: cov_init_fixed<3>(v, gi, a, lut_init, lut_w); break;
+        case 4: cov_init_fixed<4>(v, gi, a, lut_init, lut_w); break;
+        case 5: cov_init_fixed<5>(v, gi, a, lut_init, lut_w); break;
+        default: cov_init(v, gi, a, lut_init, lut_w); break;
+      }
+    } else {
+      cov_init(v, gi, a, lut_init, lut_w);
+    }
   } else {  // RGS_INIT_DELTA
 #pragma unroll
     for (int j = 0; j < 16; ++j)
