@@ -293,17 +293,21 @@ def run_learned(torch, n=34):
 def run_batch(torch, count=256):
     """Config 4: 256 instances with n uniform in 16..30 (3-regular for even n,
     Erdos-Renyi G(n, 3/n) for odd n), 1,000 samples each, tile plans from the
-    autotuner's table: the whole batch as one graph launch against one launch
-    sequence per instance, and the same for its launch-bound n <= 22 part."""
+    autotuner's table: one launch sequence per instance (eager), one graph per
+    memory wave, the whole batch as one graph, and the native multi-state
+    batch (states of one pass plan share each pass's launch), for the whole
+    batch and for its launch-bound n <= 22 part."""
     from rgs70_classical_challenge_amd.batch import BatchSampler, uneven_instances
     specs = uneven_instances(count, nmin=16, nmax=30)
     res = BatchSampler(specs, samples=1000).benchmark()
     res['workload'] = f'{count} instances, n uniform in 16..30, 1000 samples each'
     small = [sp for sp in specs if sp[0] <= 22]
     sres = BatchSampler(small, samples=1000).benchmark()
-    res['small_instances'] = {'n_max': 22, 'instances': len(small),
+    res['small_instances'] = {'n_max': 22, 'instances': len(small), 'amplitudes': sres['amplitudes'],
                               **{k: sres[k] for k in ('eager_ms', 'graph_ms', 'single_graph_ms', 'graph_speedup',
-                                                      'single_graph_speedup')}}
+                                                      'single_graph_speedup', 'fused_ms', 'fused_graph_ms',
+                                                      'fused_launches', 'fused_groups', 'fused_speedup',
+                                                      'fused_graph_speedup')}}
     return res
 
 
