@@ -261,7 +261,7 @@ int rgs_scan_compile(const uint8_t *g, int n, const int32_t *order, rgs_scan_ste
 
 /* Cut-rank profile by direct GF(2) elimination of every prefix cut.
  * Replaces search_width.profile (work/rgs_attack/search_width.py:5-15) and
- * rank_cut (order_optimization/search.cpp:22-29). */
+ * rank_cut (work/rgs_attack/order_optimization/search.cpp:16-23). */
 int rgs_scan_profile(const uint8_t *g, int n, const int32_t *order, int32_t *profile);
 
 /* Annealed order search minimising sum_j 2^{r_j} with max r <= rmax_cap.
@@ -270,8 +270,10 @@ int rgs_scan_profile(const uint8_t *g, int n, const int32_t *order, int32_t *pro
  * long swap; epochs that restart from the chain's best order and cool
  * geometrically); the best order over the chains is returned (ties: the
  * lowest chain).  Deterministic for given arguments: no wall-clock budget.
- * Replaces the annealer of order_optimization/search.cpp:16-63 (which runs
- * for a wall-clock budget) and search_width.py:5-15. */
+ * Replaces the annealer of work/rgs_attack/order_optimization/search.cpp:
+ * 24-31 (incremental cut ranks and cost) and 34-67 (epochs, moves, the
+ * rank-30 cap, Metropolis acceptance; it runs for a wall-clock budget), and
+ * the swap annealer of work/rgs_attack/search_width.py:17-27. */
 int rgs_scan_order_search(const uint8_t *g, int n, const int32_t *start, uint64_t iterations, uint64_t seed,
                           int rmax_cap, int threads, int32_t *best_order, uint64_t *best_cost);
 
