@@ -187,8 +187,9 @@ int rgs_slab_describe(const rgs_slab_desc *d, int *passes, char *text, int text_
  * 2^(nbits-mass_bits+1) - 1 doubles, as for rgs_mass_pyramid) and its draws
  * (dev uniforms, samples, dev outputs as rgs_draw).  Items whose descriptors
  * give the same pass schedule share launches: one per pass for the whole
- * group, then one per chunk of pyramid levels and one for every draw of every
- * item.  The results equal rgs_slab_pipeline + rgs_mass_pyramid + rgs_draw
+ * group (items of 2^11 or more tiles per pass, which fill the device alone,
+ * keep one launch per item and pass), then one per chunk of pyramid levels
+ * and one for every draw of every item.  The results equal rgs_slab_pipeline + rgs_mass_pyramid + rgs_draw
  * per item bit for bit (the same kernels).  create plans the batch and
  * uploads its kernel arguments (one device allocation, synchronous); run only
  * enqueues kernel launches on `stream` (it can be captured in a graph).

@@ -3,7 +3,8 @@
 //
 // States whose descriptors give the same pass schedule (same size and tile
 // plan) form a group; each pass of a group is one k_tile_multi launch over
-// the tiles of all its states.  The pyramids of all states are built level
+// the tiles of all its states (equal counts per state, so a tile's state is
+// its index shifted).  The pyramids of all states are built level
 // chunk by level chunk (one k_pyramid_multi launch per chunk round), and all
 // draws of all states are one k_draw_multi launch.  The per-state kernel
 // arguments and the job tables are built on the host once, at creation, and
@@ -21,12 +22,19 @@
 
 namespace rgs {
 
+// States of at least 2^kSingleShift tiles per pass fill the device on their
+// own: their passes are launched one state at a time with the arguments in
+// the kernel's argument segment (k_tile), which measured faster for large
+// states than reading them from global memory in a shared launch.
+constexpr int kSingleShift = 11;
+
 struct BatchPass {
   PassPlan plan;
   size_t shmem;
-  int count;
-  uint64_t tiles;
-  size_t args_off, first_off;   // byte offsets in the device allocation
+  int count;                    // states in the group
+  int shift;                    // log2 tiles per state
+  size_t args_off;              // byte offset in the device allocation
+  std::vector<PassArgs> host;   // per-state launches (shift >= kSingleShift)
 };
 
 struct BatchRound {
@@ -99,22 +107,21 @@ extern "C" int rgs_slab_batch_create(const rgs_slab_batch_item* items, int count
       bp.plan = p;
       bp.shmem = lds_bytes(p, d0->nbits, d0);
       bp.count = (int)idx.size();
-      const uint64_t per = 1ull << (d0->nbits - p.tb);
+      // every state of the group has the same size (nbits is in the key)
+      bp.shift = d0->nbits - p.tb;
       std::vector<PassArgs> args(idx.size());
-      std::vector<uint32_t> first(idx.size() + 1);
-      for (size_t k = 0; k < idx.size(); ++k) {
-        build_args(items[idx[k]].desc, p, args[k]);
-        first[k] = (uint32_t)(per * k);
-      }
-      bp.tiles = per * idx.size();
+      for (size_t k = 0; k < idx.size(); ++k) build_args(items[idx[k]].desc, p, args[k]);
       // one launch's work-items (tiles x 2^(tb-4) threads) must stay below 2^31
-      if ((bp.tiles << (p.tb - 4)) >= (1ull << 31)) {
+      if ((((uint64_t)idx.size() << bp.shift) << (p.tb - 4)) >= (1ull << 31)) {
         delete b;
         return fail(kInvalid, "slab_batch: a group's pass needs 2^31 or more work-items; split the batch");
       }
-      first[idx.size()] = (uint32_t)bp.tiles;
-      bp.args_off = img.put(args.data(), args.size() * sizeof(PassArgs));
-      bp.first_off = img.put(first.data(), first.size() * sizeof(uint32_t));
+      if (bp.shift >= kSingleShift) {
+        bp.host = args;
+        bp.args_off = 0;
+      } else {
+        bp.args_off = img.put(args.data(), args.size() * sizeof(PassArgs));
+      }
       b->passes.push_back(bp);
     }
   }
@@ -201,8 +208,14 @@ extern "C" int rgs_slab_batch_run(void* batch, void* stream) {
   hipStream_t st = reinterpret_cast<hipStream_t>(stream);
   unsigned char* base = static_cast<unsigned char*>(b->dev);
   for (const BatchPass& p : b->passes) {
-    hipError_t e = launch_pass_multi(reinterpret_cast<const PassArgs*>(base + p.args_off),
-                                     reinterpret_cast<const uint32_t*>(base + p.first_off), p.count, p.tiles, p.plan,
+    if (!p.host.empty()) {
+      for (const PassArgs& a : p.host) {
+        hipError_t e = launch_pass(a, p.plan, p.shmem, st);
+        if (e != hipSuccess) return fail(kHip, std::string("k_tile launch: ") + hipGetErrorString(e));
+      }
+      continue;
+    }
+    hipError_t e = launch_pass_multi(reinterpret_cast<const PassArgs*>(base + p.args_off), p.shift, p.count, p.plan,
                                      p.shmem, st);
     if (e != hipSuccess) return fail(kHip, std::string("k_tile_multi launch: ") + hipGetErrorString(e));
   }
@@ -223,7 +236,11 @@ extern "C" int rgs_slab_batch_run(void* batch, void* stream) {
 extern "C" int rgs_slab_batch_info(void* batch, int* launches, int* groups) {
   if (!batch) return fail(kInvalid, "slab_batch: null handle");
   const SlabBatch* b = static_cast<const SlabBatch*>(batch);
-  if (launches) *launches = (int)(b->passes.size() + b->rounds.size() + (b->draw_count ? 1 : 0));
+  if (launches) {
+    size_t n = b->rounds.size() + (b->draw_count ? 1 : 0);
+    for (const BatchPass& p : b->passes) n += p.host.empty() ? 1 : p.host.size();
+    *launches = (int)n;
+  }
   if (groups) *groups = b->ngroups;
   return kOk;
 }

@@ -154,7 +154,7 @@ int plan_passes(const rgs_slab_desc* d, std::vector<PassPlan>& out, std::string&
 }
 
 
-static hipError_t launch_pass(const PassArgs& a, const PassPlan& p, size_t shmem, hipStream_t st) {
+hipError_t launch_pass(const PassArgs& a, const PassPlan& p, size_t shmem, hipStream_t st) {
   switch (p.c) {
     case 0: return launch_c0(p.tb, p.rb, a, shmem, st);
     case 2: return launch_c2(p.tb, p.rb, a, shmem, st);
@@ -279,16 +279,16 @@ void build_args(const rgs_slab_desc* d, const PassPlan& p, PassArgs& a) {
   a.wchunks = (n + a.wbits - 1) / a.wbits;
 }
 
-hipError_t launch_pass_multi(const PassArgs* args, const uint32_t* first, int count, uint64_t tiles,
-                             const PassPlan& p, size_t shmem, hipStream_t st) {
+hipError_t launch_pass_multi(const PassArgs* args, int shift, int count, const PassPlan& p, size_t shmem,
+                             hipStream_t st) {
   switch (p.c) {
-    case 0: return launch_c0_multi(p.tb, p.rb, args, first, count, tiles, shmem, st);
-    case 2: return launch_c2_multi(p.tb, p.rb, args, first, count, tiles, shmem, st);
-    case 3: return launch_c3_multi(p.tb, p.rb, args, first, count, tiles, shmem, st);
-    case 4: return launch_c4_multi(p.tb, p.rb, args, first, count, tiles, shmem, st);
-    case 5: return launch_c5_multi(p.tb, p.rb, args, first, count, tiles, shmem, st);
-    case 6: return launch_c6_multi(p.tb, p.rb, args, first, count, tiles, shmem, st);
-    case 7: return launch_c7_multi(p.tb, p.rb, args, first, count, tiles, shmem, st);
+    case 0: return launch_c0_multi(p.tb, p.rb, args, shift, count, shmem, st);
+    case 2: return launch_c2_multi(p.tb, p.rb, args, shift, count, shmem, st);
+    case 3: return launch_c3_multi(p.tb, p.rb, args, shift, count, shmem, st);
+    case 4: return launch_c4_multi(p.tb, p.rb, args, shift, count, shmem, st);
+    case 5: return launch_c5_multi(p.tb, p.rb, args, shift, count, shmem, st);
+    case 6: return launch_c6_multi(p.tb, p.rb, args, shift, count, shmem, st);
+    case 7: return launch_c7_multi(p.tb, p.rb, args, shift, count, shmem, st);
     default: return hipErrorInvalidValue;
   }
 }

@@ -562,27 +562,21 @@ __global__ void __launch_bounds__(1 << (TB - 4)) k_tile(PassArgs a) {
   tile_run<TB, C, RBW>(a, (uint64_t)blockIdx.x + ((uint64_t)blockIdx.y << 24));
 }
 
-// The same pass over many states of one plan in one launch: workgroup b runs
-// tile b - first[i] of state i, i the last entry with first[i] <= b
-// (first[count] = total tiles); the states' arguments are read from global
-// memory.
+// The same pass over many states of one plan in one launch.  The states of
+// a plan group have the same size, so each owns 2^shift consecutive tiles:
+// workgroup b runs tile b & (2^shift - 1) of state b >> shift, with that
+// state's arguments read from global memory.
 template <int TB, int C, int RBW>
-__global__ void __launch_bounds__(1 << (TB - 4)) k_tile_multi(const PassArgs* __restrict__ args,
-                                                             const uint32_t* __restrict__ first, int count) {
-  const uint32_t b = blockIdx.x + (blockIdx.y << 24);
-  if (b >= first[count]) return;   // the grid's last row may run past the tiles
-  int lo = 0, hi = count;
-  while (hi - lo > 1) {
-    const int mid = (lo + hi) >> 1;
-    if (first[mid] <= b) lo = mid;
-    else hi = mid;
-  }
-  tile_run<TB, C, RBW>(args[lo], b - first[lo]);
+__global__ void __launch_bounds__(1 << (TB - 4)) k_tile_multi(const PassArgs* __restrict__ args, int shift,
+                                                             int count) {
+  const uint64_t b = (uint64_t)blockIdx.x + ((uint64_t)blockIdx.y << 24);
+  const uint64_t s = b >> shift;
+  if (s >= (uint64_t)count) return;   // the grid's last row may run past the tiles
+  tile_run<TB, C, RBW>(args[s], b & ((1ull << shift) - 1ull));
 }
 
 template <int TB, int C, int RBW>
-hipError_t launch_tile_multi(const PassArgs* args, const uint32_t* first, int count, uint64_t tiles, size_t shmem,
-                             hipStream_t st) {
+hipError_t launch_tile_multi(const PassArgs* args, int shift, int count, size_t shmem, hipStream_t st) {
   static bool attr_set = false;
   if (!attr_set) {
     hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_tile_multi<TB, C, RBW>),
@@ -590,8 +584,9 @@ hipError_t launch_tile_multi(const PassArgs* args, const uint32_t* first, int co
     if (e != hipSuccess) return e;
     attr_set = true;
   }
+  const uint64_t tiles = (uint64_t)count << shift;
   dim3 grid((unsigned)(tiles < (1ull << 24) ? tiles : (1ull << 24)), (unsigned)((tiles + (1ull << 24) - 1) >> 24));
-  hipLaunchKernelGGL((k_tile_multi<TB, C, RBW>), grid, dim3(1 << (TB - 4)), shmem, st, args, first, count);
+  hipLaunchKernelGGL((k_tile_multi<TB, C, RBW>), grid, dim3(1 << (TB - 4)), shmem, st, args, shift, count);
   return hipGetLastError();
 }
 
@@ -620,10 +615,10 @@ hipError_t launch_c4(int tb, int rbw, const PassArgs& a, size_t shmem, hipStream
 hipError_t launch_c5(int tb, int rbw, const PassArgs& a, size_t shmem, hipStream_t st);
 hipError_t launch_c6(int tb, int rbw, const PassArgs& a, size_t shmem, hipStream_t st);
 hipError_t launch_c7(int tb, int rbw, const PassArgs& a, size_t shmem, hipStream_t st);
-// The multi-state launches of the same specialisations (tiles: the total).
+// The multi-state launches of the same specialisations (count states of
+// 2^shift tiles each).
 #define RGS_MULTI_DECL(NAME) \
-  hipError_t NAME(int tb, int rbw, const PassArgs* args, const uint32_t* first, int count, uint64_t tiles, \
-                  size_t shmem, hipStream_t st);
+  hipError_t NAME(int tb, int rbw, const PassArgs* args, int shift, int count, size_t shmem, hipStream_t st);
 RGS_MULTI_DECL(launch_c0_multi)
 RGS_MULTI_DECL(launch_c2_multi)
 RGS_MULTI_DECL(launch_c3_multi)

@@ -26,7 +26,8 @@ int plan_passes(const rgs_slab_desc* d, std::vector<PassPlan>& out, std::string&
 size_t lds_bytes(const PassPlan& p, int nbits, const rgs_slab_desc* d);
 int check_desc(const rgs_slab_desc* d, std::string& err);
 void build_args(const rgs_slab_desc* d, const PassPlan& p, PassArgs& a);
-hipError_t launch_pass_multi(const PassArgs* args, const uint32_t* first, int count, uint64_t tiles,
-                             const PassPlan& p, size_t shmem, hipStream_t st);
+hipError_t launch_pass(const PassArgs& a, const PassPlan& p, size_t shmem, hipStream_t st);
+hipError_t launch_pass_multi(const PassArgs* args, int shift, int count, const PassPlan& p, size_t shmem,
+                             hipStream_t st);
 
 }  // namespace rgs
