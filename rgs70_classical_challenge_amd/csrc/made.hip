@@ -43,6 +43,75 @@ struct GemmArgs {
 
 constexpr int TM = 64, TN = 64, TK = 32;
 
+// One thread's share of an operand tile: rows [r0, r0 + 64) x columns
+// [k0, k0 + TK), zero outside rows x K, 8 elements per thread (TK = 32).  The
+// operand is row-major with k contiguous (X[r ld + k], kcontig) or r
+// contiguous (X[k ld + r]).  With vec (ld a multiple of 8, X 16-byte aligned,
+// and the contiguous extent a multiple of 8) the 8 elements are one 16-byte
+// load; otherwise one element per load.  load_tile fills registers (so the
+// next tile's loads are in flight while the current one is multiplied),
+// store_tile writes them into S[r][k] (k contiguous).
+static_assert(64 * TK == 256 * 8, "8 elements per thread");
+struct TileRegs {
+  uint4 v;
+};
+
+__device__ __forceinline__ void load_tile(TileRegs& t, const uint16_t* X, int ld, bool kcontig, int rows, int K,
+                                          int r0, int k0, bool vec) {
+  const int tid = threadIdx.x;
+  t.v = make_uint4(0u, 0u, 0u, 0u);
+  if (vec && kcontig) {
+    const int rr = tid / (TK / 8), kc = (tid % (TK / 8)) * 8;
+    const int gr = r0 + rr, gk = k0 + kc;
+    if (gr < rows && gk < K) t.v = *reinterpret_cast<const uint4*>(X + (size_t)gr * ld + gk);
+  } else if (vec) {
+    const int kk = tid >> 3, rc = (tid & 7) * 8;
+    const int gk = k0 + kk, gr = r0 + rc;
+    if (gk < K && gr < rows) t.v = *reinterpret_cast<const uint4*>(X + (size_t)gk * ld + gr);
+  } else {
+    uint32_t w[4] = {0u, 0u, 0u, 0u};
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+      const int e = tid + 256 * i;
+      const int rr = e / TK, kk = e % TK;
+      const int gr = r0 + rr, gk = k0 + kk;
+      uint32_t v = 0;
+      if (gr < rows && gk < K) v = kcontig ? X[(size_t)gr * ld + gk] : X[(size_t)gk * ld + gr];
+      w[i >> 1] |= v << (16 * (i & 1));
+    }
+    t.v = make_uint4(w[0], w[1], w[2], w[3]);
+  }
+}
+
+__device__ __forceinline__ void store_tile(uint16_t (*S)[TK + 4], const TileRegs& t, bool kcontig, bool vec) {
+  const int tid = threadIdx.x;
+  const uint32_t w[4] = {t.v.x, t.v.y, t.v.z, t.v.w};
+  if (vec && kcontig) {
+    const int rr = tid / (TK / 8), kc = (tid % (TK / 8)) * 8;
+    // a row of S is 72 bytes: 8-byte aligned, so two 8-byte stores
+    uint2* d = reinterpret_cast<uint2*>(&S[rr][kc]);
+    d[0] = make_uint2(w[0], w[1]);
+    d[1] = make_uint2(w[2], w[3]);
+  } else if (vec) {
+    const int kk = tid >> 3, rc = (tid & 7) * 8;
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+      S[rc + 2 * i][kk] = (uint16_t)(w[i] & 0xffffu);
+      S[rc + 2 * i + 1][kk] = (uint16_t)(w[i] >> 16);
+    }
+  } else {
+#pragma unroll
+    for (int i = 0; i < 8; ++i) {
+      const int e = tid + 256 * i;
+      S[e / TK][e % TK] = (uint16_t)((w[i >> 1] >> (16 * (i & 1))) & 0xffffu);
+    }
+  }
+}
+
+__device__ __forceinline__ bool vec_ok(const uint16_t* X, int ld, int extent) {
+  return (ld % 8) == 0 && (extent % 8) == 0 && ((uintptr_t)X % 16) == 0;
+}
+
 __global__ void __launch_bounds__(256) k_gemm(GemmArgs g) {
   // As[m][k], Bs[n][k]: k contiguous so a lane's four k values are adjacent.
   __shared__ uint16_t As[TM][TK + 4];
@@ -54,24 +123,22 @@ __global__ void __launch_bounds__(256) k_gemm(GemmArgs g) {
   f32x4 acc[2][2];
   for (int i = 0; i < 2; ++i)
     for (int j = 0; j < 2; ++j) acc[i][j] = (f32x4){0.f, 0.f, 0.f, 0.f};
+  // A is M x K (k contiguous unless ta), B is K x N given as N x K (k
+  // contiguous) when tb
+  const bool akc = !g.ta, bkc = g.tb != 0;
+  const bool avec = vec_ok(g.A, g.lda, akc ? g.K : g.M), bvec = vec_ok(g.B, g.ldb, bkc ? g.K : g.N);
 
+  TileRegs ra, rb;
+  load_tile(ra, g.A, g.lda, akc, g.M, g.K, m0, 0, avec);
+  load_tile(rb, g.B, g.ldb, bkc, g.N, g.K, n0, 0, bvec);
   for (int k0 = 0; k0 < g.K; k0 += TK) {
-    // stage A: 64 x 32 elements, 8 per thread
-    for (int e = tid; e < TM * TK; e += 256) {
-      int mm = e / TK, kk = e % TK;
-      int gm = m0 + mm, gk = k0 + kk;
-      uint16_t v = 0;
-      if (gm < g.M && gk < g.K) v = g.ta ? g.A[(size_t)gk * g.lda + gm] : g.A[(size_t)gm * g.lda + gk];
-      As[mm][kk] = v;
-    }
-    for (int e = tid; e < TN * TK; e += 256) {
-      int nn = e / TK, kk = e % TK;
-      int gn = n0 + nn, gk = k0 + kk;
-      uint16_t v = 0;
-      if (gn < g.N && gk < g.K) v = g.tb ? g.B[(size_t)gn * g.ldb + gk] : g.B[(size_t)gk * g.ldb + gn];
-      Bs[nn][kk] = v;
-    }
+    store_tile(As, ra, akc, avec);
+    store_tile(Bs, rb, bkc, bvec);
     __syncthreads();
+    if (k0 + TK < g.K) {   // the next tile's loads overlap this tile's products
+      load_tile(ra, g.A, g.lda, akc, g.M, g.K, m0, k0 + TK, avec);
+      load_tile(rb, g.B, g.ldb, bkc, g.N, g.K, n0, k0 + TK, bvec);
+    }
 #pragma unroll
     for (int ks = 0; ks < TK; ks += 16) {
       const int kq = ks + 4 * (lane >> 4);
@@ -122,12 +189,15 @@ __global__ void __launch_bounds__(256) k_gemm(GemmArgs g) {
 
 // Loss and its gradient for sigmoid outputs: per row, NLL = sum_o softplus(l)
 // - x l (binary cross-entropy in logits), dlogit = (sigmoid(l) - x) * scale.
-__global__ void k_bce(const float* logits, const uint16_t* x, int B, int n, float scale, float* nll,
-                      float* dlog, uint16_t* dlog_b) {
-  int r = blockIdx.x * blockDim.x + threadIdx.x;
-  if (r >= B) return;
+// One wavefront per row: lane o takes outputs o, o + 64, ...; the row's NLL
+// is the wavefront's butterfly sum (a fixed order).
+__global__ void __launch_bounds__(256) k_bce(const float* logits, const uint16_t* x, int B, int n, float scale,
+                                             float* nll, float* dlog, uint16_t* dlog_b) {
+  const int r = blockIdx.x * 4 + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (r >= B) return;   // whole wavefronts: no barrier follows
   float acc = 0.f;
-  for (int o = 0; o < n; ++o) {
+  for (int o = lane; o < n; o += 64) {
     float l = logits[(size_t)r * n + o];
     float xv = bf16_to_f32(x[(size_t)r * n + o]);
     float sp = l > 0.f ? l + log1pf(expf(-l)) : log1pf(expf(l));
@@ -137,7 +207,9 @@ __global__ void k_bce(const float* logits, const uint16_t* x, int B, int n, floa
     if (dlog) dlog[(size_t)r * n + o] = d;
     if (dlog_b) dlog_b[(size_t)r * n + o] = f32_to_bf16(d);
   }
-  if (nll) nll[r] = acc;
+#pragma unroll
+  for (int off = 32; off > 0; off >>= 1) acc += __shfl_xor(acc, off);
+  if (nll && lane == 0) nll[r] = acc;
 }
 
 // out[c] (+)= sum_r in[r][c]  (bias gradients).  A workgroup takes 64
@@ -600,7 +672,7 @@ extern "C" int rgs_gemm_bf16(int ta, int tb, int M, int N, int K, const uint16_t
 extern "C" int rgs_made_bce(const float* logits, const uint16_t* x, int B, int n, float scale, float* nll,
                             float* dlogits, uint16_t* dlogits_bf16, void* stream) {
   if (!logits || !x || B <= 0 || n <= 0) return fail(kInvalid, "bce: bad arguments");
-  hipLaunchKernelGGL(k_bce, dim3((B + 255) / 256), dim3(256), 0, reinterpret_cast<hipStream_t>(stream), logits, x,
+  hipLaunchKernelGGL(k_bce, dim3((B + 3) / 4), dim3(256), 0, reinterpret_cast<hipStream_t>(stream), logits, x,
                      B, n, scale, nll, dlogits, dlogits_bf16);
   RGS_LAUNCH_CHECK("k_bce");
   return kOk;
