@@ -32,7 +32,7 @@ extern "C" {
 #define RGS_ENOMEM 3 /* MemoryError                          */
 #define RGS_EHIP 4   /* RuntimeError (HIP runtime failure)   */
 
-#define RGS_ABI_VERSION 1
+#define RGS_ABI_VERSION 2
 
 /* Thread-local message of the last failure ("" if none). */
 const char *rgs_last_error(void);
@@ -86,6 +86,15 @@ int rgs_cov_weight_lut(const double *u, int nv, const int32_t *B, const uint8_t 
  * function as above). */
 int rgs_cov_weight_lut_bits(const double *u, int nv, const int32_t *B, const uint8_t *outB, int m,
                             double scale, int chunk_bits, float *wlut);
+
+/* Popcount weights for identical B unitaries u0 (complex128 [2][2]): with
+ * o = outB and r = popcount(o), table[k * (m + 1) + h] =
+ * scale * u00^(m-r-k+h) u01^(k-h) u10^(r-h) u11^h for max(0, r+k-m) <= h <=
+ * min(r, k), else 0 (complex64 [(m+1)^2], built in complex128 and rounded
+ * once), so that prod_j u0[o_j][b_j] = table[popcount(b)][popcount(b & o)].
+ * The covariant init then takes one table read per amplitude.  Replaces
+ * InitSpecialization.weight_table (init_specialization.py:98-115). */
+int rgs_cov_weight_table(const double *u0, const uint8_t *outB, int m, double scale, float *table);
 
 /* ------------------------------------------------------------------ *
  * Slab pipeline: init -> [Walsh] -> [diagonal phase] -> [2x2 gates] -> masses
@@ -152,6 +161,23 @@ typedef struct rgs_slab_desc {
                                   phase tables has exactly this many bits
                                   (4..tile_bits-lane_bits; with four it needs
                                   no LDS exchange) and the others are even   */
+  int group_bits[8];           /* all 0: strided groups from tile_bits,
+                                  lane_bits, round_bits and top_bits as
+                                  above; else the widths of the strided
+                                  groups from the low bits up, 0-terminated,
+                                  summing to nbits - low_bits (each >= 4),
+                                  with their own row widths and rounds:      */
+  int group_lanes[8];          /*   lane bits of explicit group i (2..7),
+                                  its tile bits group_lanes[i] +
+                                  group_bits[i] <= 14                        */
+  int group_rounds[8];         /*   LDS round bits of explicit group i (0..2,
+                                  applied where <= its lane bits and <= its
+                                  tile bits - 10)                            */
+  uint64_t init_omask;         /* COV with init_wpop: the B outcome bits o   */
+  int init_wpop;               /* COV: 0: init_wlut holds chunk products
+                                  (init_wbits); 1: it holds the (nbits+1)^2
+                                  popcount table of rgs_cov_weight_table,
+                                  read at [popc(b)][popc(b & init_omask)]    */
 } rgs_slab_desc;
 
 /* Factor 2x2 unitaries u (complex128 [n][2][2]) as u = D1 R D2 with

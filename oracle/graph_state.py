@@ -384,6 +384,24 @@ def weight_tables(u, B, outB, scale, chunk_bits=8):
     return tab
 
 
+def popcount_weight_table(u0, outB, scale):
+    """The covariant init weight for identical B unitaries u0, as a table in
+    two popcounts (init_specialization.py:98-115): with o = outB and
+    r = popcount(o), table[k, h] = scale u00^n00 u01^n01 u10^n10 u11^n11 with
+    n11 = h, n01 = k - h, n10 = r - h, n00 = m - r - k + h where those counts
+    are valid, else 0; prod_j u0[o_j, b_j] = table[popc(b), popc(b & o)].
+    Built in complex128 (the reference rounds it once to complex64).
+    """
+    m = len(outB)
+    r = int(np.sum(outB))
+    p = np.stack([np.power(complex(a), np.arange(m + 1)) for a in np.asarray(u0, np.complex128).flat])
+    tab = np.zeros((m + 1, m + 1), np.complex128)
+    for k in range(m + 1):
+        for h in range(max(0, r + k - m), min(r, k) + 1):
+            tab[k, h] = scale * p[0, m - r - k + h] * p[1, k - h] * p[2, r - h] * p[3, h]
+    return tab
+
+
 def graph_phase_tables(g, chunk_bits=8):
     """Chunk tables for the plain graph phase (-1)^{q(x)}: the covariant
     tables with N = identity, Q = upper triangle of G, ell = 0."""

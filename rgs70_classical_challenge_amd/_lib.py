@@ -51,6 +51,11 @@ class SlabDesc(ctypes.Structure):
         ('init_wbits', c_int),
         ('group_order', c_int),
         ('top_bits', c_int),
+        ('group_bits', c_int * 8),
+        ('group_lanes', c_int * 8),
+        ('group_rounds', c_int * 8),
+        ('init_omask', ctypes.c_uint64),
+        ('init_wpop', c_int),
     ]
 
 
@@ -75,6 +80,7 @@ SIGNATURES = {
     'rgs_gate_zyz': (c_int, [c_void_p, c_int, c_int, c_void_p, c_void_p, c_void_p]),
     'rgs_cov_weight_lut': (c_int, [c_void_p, c_int, c_void_p, c_void_p, c_int, c_double, c_void_p]),
     'rgs_cov_weight_lut_bits': (c_int, [c_void_p, c_int, c_void_p, c_void_p, c_int, c_double, c_int, c_void_p]),
+    'rgs_cov_weight_table': (c_int, [c_void_p, c_void_p, c_int, c_double, c_void_p]),
     'rgs_slab_pipeline': (c_int, [P(SlabDesc), c_void_p]),
     'rgs_slab_pass': (c_int, [P(SlabDesc), c_int, c_void_p]),
     'rgs_slab_describe': (c_int, [P(SlabDesc), P(c_int), c_char_p, c_int]),

@@ -43,6 +43,38 @@ def candidates(n):
     return out
 
 
+def group_candidates(n, low):
+    """Explicit strided groups above `low` contiguous bits (low bits up, as
+    (width, lane bits, LDS round bits)): as few groups as 8-bit widths allow,
+    widths 4..8 summing to n - low, 32-amplitude rows on 8-bit groups and 32-
+    or 64-amplitude rows on narrower ones (13-bit tiles at most, no LDS
+    rounds)."""
+    rem = n - low
+    if rem < 4:
+        return []
+    k = -(-rem // 8)
+    out = []
+
+    def widths(left, parts):
+        if parts == 1:
+            if 4 <= left <= 8:
+                yield (left,)
+            return
+        for w in range(4, 9):
+            for rest in widths(left - w, parts - 1):
+                yield (w,) + rest
+
+    for ws in widths(rem, k):
+        lanes = [[5] if w == 8 else [5, 6] for w in ws]
+        combos = [[]]
+        for opts in lanes:
+            combos = [c + [l] for c in combos for l in opts]
+        for ls in combos:
+            if ls[0] <= low:
+                out.append([[w, l, 0] for w, l in zip(ws, ls)])
+    return out
+
+
 def device_key_of(props):
     """Table key of a device: its architecture and compute-unit count (the
     marketing name depends on the host's id tables and can change between
@@ -77,10 +109,12 @@ def lookup_props(n, props):
     return json.load(open(TABLE)).get(device_key_of(props), {}).get(str(n))
 
 
-def tune_engine(eng, label, reps=3, extra_stage=True):
+def tune_engine(eng, label, reps=3, extra_stage=True, start=None):
     """Time every candidate tuning of an engine's pipeline (its slab's tuning is
-    swapped in place; the engine's own descriptor is left as it was);
-    returns ((best tuning, ms), report)."""
+    swapped in place; the engine's own descriptor is left as it was), then
+    explicit strided groups (group_candidates) around the fastest; with
+    `start` (a tuning) only that tuning and the group stage are timed.
+    Returns ((best tuning, ms), report)."""
     import torch
     from . import _lib
     from .engine import ctypes_ref, effective_low_bits
@@ -115,10 +149,10 @@ def tune_engine(eng, label, reps=3, extra_stage=True):
         if best is None or ms < best[1]:
             best = (dict(cand), ms)
 
-    for cand in candidates(n):
+    for cand in (candidates(n) if start is None else [dict(start)]):
         print(f'[autotune] {label} timing {cand}', file=sys.stderr, flush=True)
         run(cand)
-    if extra_stage and best is not None and n > best[0]['low_bits']:
+    if start is None and extra_stage and best is not None and n > best[0]['low_bits']:
         # a narrow phase group (four bits need no LDS exchange), the rest even
         base = dict(best[0])
         for top in (4, 5, 6):
@@ -138,6 +172,20 @@ def tune_engine(eng, label, reps=3, extra_stage=True):
         if base.get('group_order', 0) == 0 and best[0] == base and last is not None and \
                 last.get('group_order') == 1 and last['ms'] <= 1.005 * base_ms:
             best = (dict(base, group_order=1), last['ms'])
+    if extra_stage and best is not None and n > best[0]['low_bits']:
+        # explicit groups: each its own width and row width, the contiguous
+        # pass one bit wider or narrower
+        # pass one bit wider or narrower; they replace the best plan only when
+        # more than 0.5 % faster (timing noise)
+        kept = best
+        base = {k: v for k, v in best[0].items() if k not in ('groups', 'top_bits')}
+        for low in sorted({base['low_bits'] - 1, base['low_bits'], base['low_bits'] + 1}):
+            if 6 <= low <= 14 and low < n:
+                for groups in group_candidates(n, low):
+                    print(f'[autotune] {label} timing low {low} groups {groups}', file=sys.stderr, flush=True)
+                    run(dict(base, low_bits=low, groups=groups))
+        if best[1] > kept[1] / 1.005:
+            best = kept
     sl.tuning = saved
     return best, report
 
@@ -149,7 +197,7 @@ def tune_engine(eng, label, reps=3, extra_stage=True):
 PLAIN_MAX_BITS = 34
 
 
-def tune(n, reps=3, seed=7):
+def tune(n, reps=3, seed=7, start=None):
     """Tune the plain pipeline (graph-phase init, Walsh, gates, masses) of a
     seeded n-qubit instance; returns ((best tuning, ms), report).  Sizes above
     PLAIN_MAX_BITS are refused."""
@@ -160,20 +208,20 @@ def tune(n, reps=3, seed=7):
     from .instances import instance
     g, u = instance(n, seed)
     s = GraphStateSampler(g, u, pipeline='slab')
-    out = tune_engine(s, f'n={n}', reps)
+    out = tune_engine(s, f'n={n}', reps, start=start)
     del s
     torch.cuda.empty_cache()
     return out
 
 
-def tune_conditional(g, u, A, B, reps=2):
+def tune_conditional(g, u, A, B, reps=2, start=None):
     """Tune the conditional pipeline (covariant init, Walsh, A-side phase,
     gates, masses) of one instance; returns ((best tuning, ms), report)."""
     import torch
     from .engine import ConditionalSampler
     eng = ConditionalSampler(g, u, A, B)
     eng.compute([0] * len(A))
-    out = tune_engine(eng, f'conditional m={len(A)}', reps)
+    out = tune_engine(eng, f'conditional m={len(A)}', reps, start=start)
     del eng
     torch.cuda.empty_cache()
     return out
@@ -256,7 +304,9 @@ def tune_scan_order(g, u, start=None, iterations=1_000_000, threads=8, count=655
 
 
 def main(argv):
-    """argv: lo [hi [output path]] - the plain pipeline for n = lo..hi; or
+    """argv: [--from-table] lo [hi [output path]] - the plain pipeline for
+    n = lo..hi (--from-table: start from each size's table entry and time only
+    the explicit-group stage); or
     --scan instance.npz [output path] - the scan order of the instance's
     graph (arrays graph, measurement_unitaries), stored under 'scan:<hash>'; or
     --conditional instance.npz [output path] - the conditional pipeline of the
@@ -264,6 +314,19 @@ def main(argv):
     under the key 'c<m>'.  The table is merged into the output file (default:
     the package's tuned.json)."""
     import torch
+    from_table = bool(argv) and argv[0] == '--from-table'
+    if from_table:
+        argv = argv[1:]
+    known = load_table(torch)
+
+    def start_of(key):
+        if not from_table:
+            return None
+        e = known.get(key)
+        if e is None:
+            raise ValueError(f'--from-table: no table entry for {key}')
+        return {k: v for k, v in e.items() if k not in ('ms', 'candidates')}
+
     if argv and argv[0] == '--scan':
         d = np.load(argv[1])
         out = argv[2] if len(argv) > 2 else TABLE
@@ -277,18 +340,20 @@ def main(argv):
     if argv and argv[0] == '--conditional':
         d = np.load(argv[1])
         out = argv[2] if len(argv) > 2 else TABLE
-        jobs = [(f'c{len(d["A"])}', lambda: tune_conditional(d['graph'], d['measurement_unitaries'],
-                                                             d['A'].tolist(), d['B'].tolist()))]
+        key = f'c{len(d["A"])}'
+        jobs = [(key, lambda: tune_conditional(d['graph'], d['measurement_unitaries'], d['A'].tolist(),
+                                               d['B'].tolist(), start=start_of(key)))]
     else:
         lo, hi = int(argv[0]), int(argv[1]) if len(argv) > 1 else int(argv[0])
         out = argv[2] if len(argv) > 2 else TABLE
-        jobs = [(str(n), (lambda n=n: tune(n))) for n in range(lo, hi + 1)]
+        jobs = [(str(n), (lambda n=n: tune(n, start=start_of(str(n))))) for n in range(lo, hi + 1)]
     table = json.load(open(out)) if os.path.exists(out) else load_table()
     dev = table.setdefault(device_key(torch), {})
     for key, job in jobs:
         t0 = time.time()
         (best, ms), report = job()
-        dev[key] = dict(best, ms=round(ms, 4), candidates=len(report))
+        prev = dev.get(key, {}).get('candidates', 0) if from_table else 0
+        dev[key] = dict(best, ms=round(ms, 4), candidates=prev + len(report))
         print(json.dumps({'key': key, 'best': best, 'ms': round(ms, 4), 'tuned_in_s': round(time.time() - t0, 2)}),
               flush=True)
         with open(out, 'w') as f:

@@ -319,6 +319,35 @@ extern "C" int rgs_cov_weight_lut_bits(const double* u, int nv, const int32_t* B
   return kOk;
 }
 
+extern "C" int rgs_cov_weight_table(const double* u0, const uint8_t* outB, int m, double scale, float* table) {
+  if (!u0 || !outB || !table || m < 1 || m > 63) return fail(kInvalid, "cov_weight_table: need 1 <= m <= 63");
+  using cd = std::complex<double>;
+  int r = 0;
+  for (int j = 0; j < m; ++j) {
+    if (outB[j] > 1) return fail(kInvalid, "cov_weight_table: outB must hold bits");
+    r += outB[j];
+  }
+  // powers 0..m of each entry, by repeated products in complex128
+  std::vector<cd> pw(4 * (m + 1));
+  for (int e = 0; e < 4; ++e) {
+    const cd a(u0[2 * e], u0[2 * e + 1]);
+    pw[e * (m + 1)] = cd(1.0, 0.0);
+    for (int k = 1; k <= m; ++k) pw[e * (m + 1) + k] = pw[e * (m + 1) + k - 1] * a;
+  }
+  const int st = m + 1;
+  for (int k = 0; k <= m; ++k)
+    for (int h = 0; h <= m; ++h) {
+      cd w(0.0, 0.0);
+      if (h >= std::max(0, r + k - m) && h <= std::min(r, k)) {
+        const int n00 = m - r - k + h, n01 = k - h, n10 = r - h, n11 = h;
+        w = scale * pw[n00] * pw[st + n01] * pw[2 * st + n10] * pw[3 * st + n11];
+      }
+      table[2 * (k * st + h)] = (float)w.real();
+      table[2 * (k * st + h) + 1] = (float)w.imag();
+    }
+  return kOk;
+}
+
 extern "C" int rgs_cov_weight_lut(const double* u, int nv, const int32_t* B, const uint8_t* outB, int m,
                                   double scale, float* wlut) {
   return rgs_cov_weight_lut_bits(u, nv, B, outB, m, scale, 8, wlut);

@@ -36,7 +36,8 @@ size_t lds_bytes(const PassPlan& p, int nbits, const rgs_slab_desc* d) {
   if ((p.flags & F_PHASE) && d->dphase_lut && !d->dphase_global) bytes += (size_t)chunks_of(nbits) * 256 * 8;
   if (!(p.flags & F_LOAD) && d->init_kind == RGS_INIT_COV) {
     const int wb = d->init_wbits ? d->init_wbits : 8;
-    bytes += (size_t)((nbits + wb - 1) / wb) * ((size_t)1 << wb) * 8;
+    bytes += d->init_wpop ? (size_t)(nbits + 1) * (nbits + 1) * 8
+                          : (size_t)((nbits + wb - 1) / wb) * ((size_t)1 << wb) * 8;
   }
   return bytes;
 }
@@ -51,7 +52,29 @@ int plan_passes(const rgs_slab_desc* d, std::vector<PassPlan>& out, std::string&
   if (d->round_bits < 0 || d->round_bits > 2) { err = "round_bits must be in 0..2"; return kInvalid; }
   if (d->phase_round_bits < -1 || d->phase_round_bits > 2) { err = "phase_round_bits must be in -1..2"; return kInvalid; }
   std::vector<PassPlan> groups;
-  if (n > T0) {
+  if (d->group_bits[0] > 0) {
+    // explicit groups: each with its own width, row width and LDS rounds
+    int q = T0;
+    for (int i = 0; i < 8 && d->group_bits[i] > 0; ++i) {
+      PassPlan g{};
+      g.low = 0;
+      g.qstart = q;
+      g.count = d->group_bits[i];
+      g.c = d->group_lanes[i] ? d->group_lanes[i] : L;
+      g.tb = g.c + g.count;
+      g.rbits = 4;
+      if (g.count < g.rbits) { err = "explicit strided group narrower than a register window"; return kInvalid; }
+      if (g.c < 2 || g.c > 7) { err = "explicit group lane bits must be in 2..7"; return kInvalid; }
+      if (g.tb > 14) { err = "explicit group lane bits plus width exceed 14 tile bits"; return kInvalid; }
+      if (g.c > q) { err = "explicit group lane bits overlap its target bits"; return kInvalid; }
+      const int r = d->group_rounds[i];
+      if (r < 0 || r > 2) { err = "explicit group round bits must be in 0..2"; return kInvalid; }
+      g.rb = std::max(0, std::min(std::min(r, g.c), g.tb - 10));
+      groups.push_back(g);
+      q += g.count;
+    }
+    if (q != n) { err = "explicit strided groups must cover bits low_bits..nbits-1 exactly"; return kInvalid; }
+  } else if (n > T0) {
     if (TS < 6 || TS > 14) { err = "tile_bits must be in 6..14"; return kInvalid; }
     if (L < 2 || L > 7) { err = "lane_bits must be in 2..7"; return kInvalid; }
     const int RBs = 4;
@@ -217,6 +240,8 @@ int check_desc(const rgs_slab_desc* d, std::string& err) {
     return kInvalid;
   }
   if (d->init_wbits != 0 && d->init_wbits != 8 && d->init_wbits != 9) { err = "init_wbits must be 8 or 9"; return kInvalid; }
+  if (d->init_wpop != 0 && d->init_wpop != 1) { err = "init_wpop must be 0 or 1"; return kInvalid; }
+  if (d->init_wpop && (d->init_omask >> n)) { err = "init_omask has bits at or above nbits"; return kInvalid; }
   if (d->phase_lut && !d->phase_rows) { err = "phase_lut needs its form rows"; return kInvalid; }
   if ((d->phase_lut || d->dphase_lut || d->init_kind == RGS_INIT_GRAPH || d->init_kind == RGS_INIT_COV) && n > 63) {
     err = "lookup tables need nbits <= 63";
@@ -277,6 +302,9 @@ void build_args(const rgs_slab_desc* d, const PassPlan& p, PassArgs& a) {
   a.dphase_global = d->dphase_global ? 1 : 0;
   a.wbits = d->init_wbits ? d->init_wbits : 8;
   a.wchunks = (n + a.wbits - 1) / a.wbits;
+  a.wpop = d->init_wpop ? n + 1 : 0;
+  a.wentries = a.wpop ? (n + 1) * (n + 1) : a.wchunks << a.wbits;
+  a.womask = d->init_omask;
 }
 
 hipError_t launch_pass_multi(const PassArgs* args, int shift, int count, const PassPlan& p, size_t shmem,

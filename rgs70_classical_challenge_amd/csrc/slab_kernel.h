@@ -81,6 +81,9 @@ struct PassArgs {
   int dphase_global;  // read dphase from global memory instead of staging it
   int wbits;          // COV weight chunk bits (8 or 9)
   int wchunks;        // COV weight chunks
+  int wpop;           // COV popcount table: its row stride m + 1 (0: chunk products)
+  int wentries;       // COV weight entries staged in LDS
+  uint64_t womask;    // COV popcount table: the B outcome bits
 };
 
 __device__ __forceinline__ uint32_t phase_parity(uint64_t x, const uint64_t* lut, int chunks) {
@@ -209,6 +212,32 @@ __device__ __forceinline__ void cov_init_fixed(float2 (&v)[16], uint64_t x0, con
       val = c ? cmul(val, e) : e;
     }
     v[j] = cneg_if(val, (sg >> j) & 1u);
+  }
+}
+
+// The covariant init with identical B unitaries: the weight of b is
+// table[popc(b)][popc(b & o)] (o: the B outcome bits), one table read per
+// amplitude; map, sign and register offsets as in cov_init_fixed.
+__device__ __forceinline__ void cov_init_popc(float2 (&v)[16], uint64_t x0, const PassArgs& a,
+                                              const uint64_t* lut, const float2* w) {
+  uint64_t b0 = 0, acc = 0;
+  for (int c = 0; c < a.init_chunks; ++c) {
+    const uint32_t e = (c << 8) | ((x0 >> (8 * c)) & 255u);
+    b0 ^= lut[2 * e];
+    acc ^= lut[2 * e + 1];
+  }
+  const uint32_t sg = register_signs((uint32_t)__popcll(acc & (x0 | (1ull << 63))) & 1u, x0, a.iq, a.ifreg);
+  const uint64_t o = a.womask;
+  const uint32_t st = (uint32_t)a.wpop;
+#pragma unroll
+  for (int j = 0; j < 16; ++j) {
+    uint64_t dj = 0;   // uniform: M j
+#pragma unroll
+    for (int i = 0; i < 4; ++i)
+      if (j & (1 << i)) dj ^= a.icol[i];
+    const uint64_t b = b0 ^ dj;
+    const uint32_t k = (uint32_t)__popcll(b), h = (uint32_t)__popcll(b & o);
+    v[j] = cneg_if(w[k * st + h], (sg >> j) & 1u);
   }
 }
 
@@ -414,7 +443,7 @@ __device__ __forceinline__ void tile_run(const PassArgs& a, uint64_t blk) {
     if (!a.dphase_global)
       for (int i = tid; i < a.dphase_chunks * 256; i += G::NT) lut_d[i] = a.dphase[i];
     if (cov)
-      for (int i = tid; i < (a.wchunks << a.wbits); i += G::NT) lut_w[i] = a.init_w[i];
+      for (int i = tid; i < a.wentries; i += G::NT) lut_w[i] = a.init_w[i];
     __syncthreads();
   }
 
@@ -444,7 +473,9 @@ __device__ __forceinline__ void tile_run(const PassArgs& a, uint64_t blk) {
 #pragma unroll
     for (int j = 0; j < 16; ++j) v[j] = cneg_if(make_float2(a.init_scale, 0.f), (e >> j) & 1u);
   } else if (a.init_kind == RGS_INIT_COV) {
-    if constexpr (C == 0) {
+    if (a.wpop) {
+      cov_init_popc(v, gi, a, lut_init, lut_w);
+    } else if constexpr (C == 0) {
       // the contiguous pass that makes the state: chunk counts of m <= 45
       switch (a.wchunks) {
         case 1: cov_init_fixed<1>(v, gi, a, lut_init, lut_w); break;

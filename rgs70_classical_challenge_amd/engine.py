@@ -60,7 +60,7 @@ def resolve_tuning(nbits, props, tuning=None, kind=''):
             entry = lookup_props(nbits, props)
         if entry is not None:
             tuning = {k: entry[k] for k in ('low_bits', 'tile_bits', 'lane_bits', 'round_bits', 'phase_round_bits',
-                                            'dphase_global', 'top_bits', 'group_order') if k in entry}
+                                            'dphase_global', 'top_bits', 'group_order', 'groups') if k in entry}
     return dict(default_tuning(nbits), **(tuning or {}))
 
 
@@ -139,6 +139,14 @@ class _Slab:
         d.dphase_global = int(self.tuning.get('dphase_global', 0))
         d.top_bits = int(self.tuning.get('top_bits', 0))
         d.group_order = int(self.tuning.get('group_order', 0))
+        # explicit strided groups, low bits up: (width, lane bits, LDS round bits)
+        groups = self.tuning.get('groups') or ()
+        if len(groups) > 8:
+            raise ValueError('at most 8 explicit strided groups')
+        for i, (bits, lane, rounds) in enumerate(groups):
+            d.group_bits[i] = int(bits)
+            d.group_lanes[i] = int(lane)
+            d.group_rounds[i] = int(rounds)
         # the descriptor holds raw pointers: keep their owners alive with it
         d._owners = (self.state, self.pyramid, init_lut, init_wlut, phase_lut, gates, init_rows, phase_rows,
                      init_cols)
@@ -402,7 +410,7 @@ class ConditionalSampler:
     """
 
     def __init__(self, g, u, A, B, devices=(0,), chunk_mib=1, fusion_bits=5, lane_bits=5, lookup_bits=8,
-                 defer_top=True, mass_bits=8, tuning=None, weight_bits=None):
+                 defer_top=True, mass_bits=8, tuning=None, weight_bits=None, weight_table=None):
         g = np.ascontiguousarray(np.asarray(g, np.uint8))
         u = np.ascontiguousarray(np.asarray(u, np.complex128))
         A, B = [int(x) for x in A], [int(x) for x in B]
@@ -434,8 +442,16 @@ class ConditionalSampler:
         if weight_bits not in (None, 8, 9):
             raise ValueError('weight_bits must be 8 or 9')
         self._weight_bits = weight_bits
-        # room for either chunking (the descriptor names the one in use)
-        self._wlut_host = np.zeros(max(((m + 7) // 8) << 8, ((m + 8) // 9) << 9) * 2, np.float32)
+        # identical B unitaries: the weight of b depends on two popcounts
+        # (InitSpecialization.weight_table), one table read per amplitude
+        same = bool(np.all(u[Ba] == u[Ba[0]]))
+        if weight_table is None:
+            weight_table = same and weight_bits is None
+        if weight_table and not same:
+            raise ValueError('the popcount weight table needs identical B unitaries')
+        self._weight_table = bool(weight_table)
+        # room for either chunking or the popcount table (the descriptor names the one in use)
+        self._wlut_host = np.zeros(max(((m + 7) // 8) << 8, ((m + 8) // 9) << 9, (m + 1) ** 2) * 2, np.float32)
         self._wlut = sl.to_device(self._wlut_host)
         self.deferred_global_unitary = _check_unitary(u[A[-1]]) if defer_top else None
         self.n = 1 << (m - 1)          # shard size of the reference's two-device layout
@@ -451,10 +467,12 @@ class ConditionalSampler:
         if wbits is None:
             low = effective_low_bits(m, self.slab.tuning['low_bits'])
             wbits = 9 if (m + 8) // 9 < (m + 7) // 8 and low <= 12 else 8
-        return self.slab.desc(_lib.INIT_COV, 0.0, init_lut=self._map, init_wlut=self._wlut, walsh=True,
-                              init_rows=self._cov_rows, init_cols=self._cov_cols, phase_lut=self._aphase,
-                              phase_rows=self._arows, gates=self._gates,
-                              gate_limit=m - 1 if self.defer_top else m, init_wbits=wbits)
+        d = self.slab.desc(_lib.INIT_COV, 0.0, init_lut=self._map, init_wlut=self._wlut, walsh=True,
+                           init_rows=self._cov_rows, init_cols=self._cov_cols, phase_lut=self._aphase,
+                           phase_rows=self._arows, gates=self._gates,
+                           gate_limit=m - 1 if self.defer_top else m, init_wbits=wbits)
+        d.init_wpop = int(self._weight_table)
+        return d
 
     def compute(self, outB):
         if len(outB) != self.m:
@@ -464,8 +482,13 @@ class ConditionalSampler:
             raise ValueError('outB must contain bits')
         stage = {}
         t = time.perf_counter()
-        call('rgs_cov_weight_lut_bits', ptr(self._u), len(self.g), ptr(self._Ba), ptr(ob), self.m,
-             2.0 ** (-self.m / 2), self._desc.init_wbits, ptr(self._wlut_host))
+        if self._weight_table:
+            call('rgs_cov_weight_table', ptr(self._u[self._Ba[0]]), ptr(ob), self.m, 2.0 ** (-self.m / 2),
+                 ptr(self._wlut_host))
+            self._desc.init_omask = int(sum(1 << j for j in range(self.m) if ob[j]))
+        else:
+            call('rgs_cov_weight_lut_bits', ptr(self._u), len(self.g), ptr(self._Ba), ptr(ob), self.m,
+                 2.0 ** (-self.m / 2), self._desc.init_wbits, ptr(self._wlut_host))
         self._wlut.copy_(self.slab.torch.as_tensor(self._wlut_host))
         stage['weights'] = time.perf_counter() - t
         t = time.perf_counter()

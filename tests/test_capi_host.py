@@ -33,7 +33,7 @@ def test_library_exports_every_declared_symbol():
     assert len(names) >= 10
     missing = [n for n in sorted(names) if not hasattr(lib, n)]
     assert not missing, missing
-    assert lib.rgs_abi_version() == 1
+    assert lib.rgs_abi_version() == 2
 
 
 def test_binding_covers_declarations():
@@ -65,6 +65,44 @@ def test_graph_phase_lut_matches_oracle():
         lut = graph_phase_lut(g)
         ref = gs.graph_phase_tables(g)[:, :, 1].reshape(-1)
         assert np.array_equal(lut, ref)
+
+
+def test_popcount_weight_table(golden_dir):
+    """The identical-unitary weight table: the oracle's table equals the
+    generic product prod_j u0[o_j, b_j] for every b at m = 10 (brute force)
+    and is zero at impossible counts; the library's complex64 table is the
+    oracle's rounded once, on the 70-qubit instance's unitary at m = 35."""
+    rng = np.random.default_rng(7)
+    m = 10
+    u0 = np.linalg.qr(rng.normal(size=(2, 2)) + 1j * rng.normal(size=(2, 2)))[0]
+    ob = rng.integers(0, 2, m)
+    o = int(sum(1 << j for j in range(m) if ob[j]))
+    tab = gs.popcount_weight_table(u0, ob, 0.5)
+    for b in range(1 << m):
+        want = 0.5 * np.prod([u0[ob[j], (b >> j) & 1] for j in range(m)])
+        got = tab[bin(b).count('1'), bin(b & o).count('1')]
+        assert abs(got - want) <= 1e-14, b
+    r = int(ob.sum())
+    for k in range(m + 1):
+        for h in range(m + 1):
+            if not max(0, r + k - m) <= h <= min(r, k):
+                assert tab[k, h] == 0
+    d = np.load(f'{golden_dir}/rgs_graph.npz')
+    u = d['measurement_unitaries']
+    assert all(np.array_equal(u[q], u[0]) for q in range(len(u)))    # the instance's unitaries are identical
+    m = 35
+    for seed in range(3):
+        ob = np.random.default_rng(seed).integers(0, 2, m).astype(np.uint8)
+        out = np.zeros(2 * (m + 1) ** 2, np.float32)
+        u0 = np.ascontiguousarray(u[0])
+        _lib.call('rgs_cov_weight_table', _lib.ptr(u0), _lib.ptr(ob), m, 2.0 ** (-m / 2), _lib.ptr(out))
+        got = out.view(np.complex64).reshape(m + 1, m + 1)
+        want = gs.popcount_weight_table(u0, ob, 2.0 ** (-m / 2))
+        assert np.array_equal(got == 0, want == 0)
+        nz = want != 0
+        assert np.all(np.abs(got[nz] - want[nz]) <= 6e-8 * np.abs(want[nz]))
+    with pytest.raises(ValueError):
+        _lib.call('rgs_cov_weight_table', _lib.ptr(u0), _lib.ptr(np.full(m, 2, np.uint8)), m, 1.0, _lib.ptr(out))
 
 
 def test_cov_luts_match_oracle(golden_dir):
@@ -124,7 +162,7 @@ def test_cov_luts_reject_singular_cut():
 
 
 def describe(nbits, walsh, gates=True, phase=False, mass=True, init=_lib.INIT_GRAPH, low=14, tile=14, lane=4,
-             rounds=0, phase_rounds=-1, top=0, order=0):
+             rounds=0, phase_rounds=-1, top=0, order=0, groups=()):
     d = _lib.SlabDesc()
     d.state = 1
     d.nbits = nbits
@@ -140,6 +178,8 @@ def describe(nbits, walsh, gates=True, phase=False, mass=True, init=_lib.INIT_GR
     d.phase_round_bits = phase_rounds
     d.top_bits = top
     d.group_order = order
+    for i, (bits, glane, grounds) in enumerate(groups):
+        d.group_bits[i], d.group_lanes[i], d.group_rounds[i] = bits, glane, grounds
     n = ctypes.c_int()
     buf = ctypes.create_string_buffer(4096)
     _lib.call('rgs_slab_describe', ctypes.byref(d), ctypes.byref(n), buf, 4096)
@@ -194,6 +234,32 @@ def test_pass_schedules():
         'strided[gates]:20+7', 'strided[gates]:27+7', 'low[gates][mass]:0+13'], text
     n, text = describe(34, walsh=True, phase=True, low=14, tile=14, lane=6, top=4, order=1)
     assert '[phase]' in text.split()[3] and text.split()[3].startswith('strided[wht][phase][gates]:14+4/tb10'), text
+
+
+def test_explicit_group_schedules():
+    # explicit groups from the low bits up, each with its own lane bits and
+    # rounds; tile bits = lane bits + width; rounds capped at tile bits - 10
+    n, text = describe(35, walsh=True, phase=True, low=12, groups=[(8, 5, 0), (8, 6, 1), (7, 7, 1)])
+    assert text.split() == [
+        'low[init][wht]:0+12/tb12/c0/rb0', 'strided[wht]:12+8/tb13/c5/rb0', 'strided[wht]:20+8/tb14/c6/rb1',
+        'strided[wht][phase][gates]:28+7/tb14/c7/rb1', 'strided[gates]:20+8/tb14/c6/rb1',
+        'strided[gates]:12+8/tb13/c5/rb0', 'low[gates][mass]:0+12/tb12/c0/rb0'], text
+    n, text = describe(34, walsh=True, phase=True, low=13, order=1, groups=[(7, 6, 0), (7, 6, 0), (7, 7, 2)])
+    assert text.split()[1] == 'strided[wht]:27+7/tb14/c7/rb2' and '[phase]' in text.split()[3], text
+    n, text = describe(22, walsh=True, phase=True, low=8, groups=[(7, 7, 2), (7, 6, 1)])
+    assert text.split()[1:3] == ['strided[wht]:8+7/tb14/c7/rb2', 'strided[wht][phase][gates]:15+7/tb13/c6/rb1'], text
+    n, text = describe(20, walsh=False, phase=True, low=8, groups=[(5, 6, 1), (7, 7, 1)])
+    assert text.split()[0] == 'strided[init][phase][gates]:8+5/tb11/c6/rb1', text
+    for bad in ([(7, 6, 0), (6, 7, 1)],          # bits 8..20 only
+                [(7, 8, 0), (7, 6, 0)],          # lane bits above 7
+                [(7, 1, 0), (7, 6, 0)],          # lane bits below 2
+                [(8, 7, 0), (6, 6, 0)],          # 15 tile bits
+                [(3, 6, 0), (11, 3, 0)],         # narrower than a register window
+                [(7, 6, 3), (7, 6, 0)]):         # rounds above 2
+        with pytest.raises(ValueError):
+            describe(22, walsh=True, low=8, groups=bad)
+    with pytest.raises(ValueError):
+        describe(20, walsh=True, low=6, groups=[(7, 7, 0), (7, 6, 0)])   # 7 lane bits, 6 bits below the group
 
 
 def test_gate_factorisation_and_unit_rounding(golden_dir):

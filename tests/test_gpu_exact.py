@@ -66,7 +66,7 @@ def test_graph_phase_stages_bit_exact(n, plan):
     _exact(_run(sl, sl.desc(_lib.INIT_DELTA, 1.0, walsh=True, phase_lut=lut, phase_rows=rows, mass=False)), sign)
 
 
-@pytest.mark.parametrize('wbits', [8, 9])
+@pytest.mark.parametrize('wbits', [8, 9, 'popcount'])
 @pytest.mark.parametrize('plan', [PLANS[0], PLANS[2], PLANS[5]])
 @pytest.mark.parametrize('name', ['induced_24q.npz', 'induced_40q.npz'])
 def test_covariant_init_bit_exact(golden_dir, name, plan, wbits):
@@ -78,7 +78,8 @@ def test_covariant_init_bit_exact(golden_dir, name, plan, wbits):
     m = len(A)
     assert m % 2 == 0                       # 2^{-m/2} is then a power of two
     ones = np.ones((len(g), 2, 2), np.complex128)
-    rng = np.random.default_rng(m + wbits)
+    pop = wbits == 'popcount'
+    rng = np.random.default_rng(m + (0 if pop else wbits))
     outB = rng.integers(0, 2, m).astype(np.uint8)
     sl = _Slab(m, 0, tuning=plan, kind='c')
     map_lut = np.zeros(((m + 7) // 8) * 256 * 2, np.uint64)
@@ -87,19 +88,31 @@ def test_covariant_init_bit_exact(golden_dir, name, plan, wbits):
     gg = np.ascontiguousarray(g, np.uint8)
     call('rgs_cov_luts', ptr(gg), len(g), ptr(Aa), ptr(Ba), m, ptr(map_lut), ptr(aphase))
     rows, cols = cov_form(map_lut, m)
-    wl = np.zeros((((m + wbits - 1) // wbits) << wbits) * 2, np.float32)
-    call('rgs_cov_weight_lut_bits', ptr(ones), len(g), ptr(Ba), ptr(outB), m, 2.0 ** (-m / 2), wbits, ptr(wl))
-    assert np.all(wl[0::2] != 0) and np.all(wl[1::2] == 0)
+    if pop:
+        # the popcount table: with unit weights every valid entry is 2^{-m/2}
+        wl = np.zeros((m + 1) ** 2 * 2, np.float32)
+        call('rgs_cov_weight_table', ptr(np.ascontiguousarray(ones[0])), ptr(outB), m, 2.0 ** (-m / 2), ptr(wl))
+        assert np.all(wl[1::2] == 0)
+    else:
+        wl = np.zeros((((m + wbits - 1) // wbits) << wbits) * 2, np.float32)
+        call('rgs_cov_weight_lut_bits', ptr(ones), len(g), ptr(Ba), ptr(outB), m, 2.0 ** (-m / 2), wbits, ptr(wl))
+        assert np.all(wl[0::2] != 0) and np.all(wl[1::2] == 0)
     dmap, dwl, dph = sl.to_device(map_lut), sl.to_device(wl), sl.to_device(aphase)
     print(f'case {name} m={m} plan={plan} wbits={wbits}', flush=True)
     init = gs.covariant_init(g, ones, A, B, outB.tolist())
-    got = _run(sl, sl.desc(_lib.INIT_COV, 0.0, init_lut=dmap, init_wlut=dwl, init_rows=rows, init_cols=cols,
-                           init_wbits=wbits, mass=False))
+
+    def desc(**kw):
+        d = sl.desc(_lib.INIT_COV, 0.0, init_lut=dmap, init_wlut=dwl, init_rows=rows, init_cols=cols,
+                    init_wbits=8 if pop else wbits, mass=False, **kw)
+        if pop:
+            d.init_wpop = 1
+            d.init_omask = int(sum(1 << j for j in range(m) if outB[j]))
+        return d
+
+    got = _run(sl, desc())
     _exact(got, init)
     want = init.copy()
     gs.walsh(want)
     gs.quadratic_phase(want, g[np.ix_(A, A)])
-    got = _run(sl, sl.desc(_lib.INIT_COV, 0.0, init_lut=dmap, init_wlut=dwl, init_rows=rows, init_cols=cols,
-                           init_wbits=wbits, walsh=True, phase_lut=dph, phase_rows=form_rows(g[np.ix_(A, A)]),
-                           mass=False))
+    got = _run(sl, desc(walsh=True, phase_lut=dph, phase_rows=form_rows(g[np.ix_(A, A)])))
     _exact(got, want)

@@ -214,6 +214,42 @@ def test_conditional_sampler_matches_oracle(golden_dir, name, tuning, weight_bit
         assert abs(eng.probability_at(i1) - q) <= 1e-5 * max(q, floor), (eng.probability_at(i1), q)
 
 
+@pytest.mark.parametrize('tuning', [None, dict(low_bits=8, tile_bits=11, lane_bits=3)])
+def test_conditional_popcount_weights(golden_dir, tuning):
+    """Identical B unitaries: the popcount weight table (one read per
+    amplitude) against the complex128 oracle and against the chunk products,
+    on the induced 40-qubit instance (the same unitary on every site)."""
+    E = _engine()
+    d = np.load(f'{golden_dir}/induced_40q.npz')
+    g, u, A, B = d['graph'], d['measurement_unitaries'], d['A'].tolist(), d['B'].tolist()
+    assert np.all(u == u[0])
+    rng = np.random.default_rng(4041)
+    pop = E.ConditionalSampler(g, u, A, B, [0], tuning=tuning)
+    chunked = E.ConditionalSampler(g, u, A, B, [0], tuning=tuning, weight_table=False)
+    assert pop._weight_table and not chunked._weight_table
+    for trial in range(2):
+        ob = rng.integers(0, 2, len(B)).tolist()
+        print(f'case trial={trial} tuning={tuning} plan={pop.plan_text}', flush=True)
+        raw, final = gs.deferred(g, u, A, B, ob)
+        pop.compute(ob)
+        got = pop.host()
+        assert np.linalg.norm(got - final) < 1e-5
+        chunked.compute(ob)
+        assert np.linalg.norm(got - chunked.host()) < 2e-6
+        mass, norm, tb, nmass = pop.masses()
+        assert abs(norm - 1) < 1e-5
+        seed = int(rng.integers(1 << 30))
+        i1, p1, _ = pop.draw(np.random.default_rng(seed))
+        i2, p2, _ = od.reference_draw(raw, np.random.default_rng(seed), terminal=u[A[-1]])
+        assert i1 == i2
+        floor = 0.1 * 2.0 ** -len(A)
+        assert abs(p1 - p2) <= 1e-5 * max(p2, floor), (p1, p2)
+    mixed = u.copy()
+    mixed[B[0]] = mixed[B[0]] @ np.array([[0, 1], [1, 0]])
+    with pytest.raises(ValueError):
+        E.ConditionalSampler(g, mixed, A, B, [0], weight_table=True)
+
+
 def test_conditional_sampler_errors(golden_dir):
     E = _engine()
     d = np.load(f'{golden_dir}/induced_12q.npz')
