@@ -1,11 +1,13 @@
 // Batched linear rank-width scan: p(x) of many proposals in one persistent
-// launch.  One workgroup of W wavefronts owns one proposal at a time and
-// sweeps all n steps with the environment table resident in LDS (2^rmax
-// entries); outputs of a step are gathered into registers (K per lane), the
-// waves combine their maximum modulus through LDS, and the outputs are written
-// back in place scaled by an exact power of two, so there is no host round
-// trip between steps.  Sums and weights are FP64; the table is stored as
-// complex64 (S = float2) or complex128 (S = double2, the audit precision).
+// launch.  One workgroup of W wavefronts owns P proposals at a time (one or
+// two) and sweeps all n steps with their environment tables resident in LDS
+// (2^rmax entries each); the table addresses and signs of a step depend on
+// the plan only and are shared by the P proposals.  Outputs of a step are
+// gathered into registers (K per lane), the waves combine their maximum
+// modulus through LDS, and the outputs are written back in place, their scale
+// carried as an exact power of two, so there is no host round trip between
+// steps.  Sums and weights are FP64; the table is stored as complex64
+// (S = float2) or complex128 (S = double2, the audit precision).
 #include "rgs_common.h"
 #include "rgs_amd.h"
 
@@ -160,15 +162,18 @@ __device__ __forceinline__ uint32_t scan_outputs(const S* E, const ScanStepArgs&
 // term t the same sign, decided from the block word b = hi[wave] ^ hs alone.
 // A term's table index is the lane's masked word lm ^ ((b ^ combo_t) & mask);
 // its sign is uniform, so the signed sums become fused multiply-adds by a
-// scalar +-1 and the first term's sign moves into the weight.  Returns the
-// lane's maximum high word of |out|^2.
+// scalar +-1 and the first term's sign moves into the weight.  The indices
+// and signs depend on the plan only, so P proposals (tables TBYTES apart,
+// weights f0[q], f1[q]) share them.  With TRACK, sets each proposal's maximum
+// high word of |out|^2 over the lane's outputs.
 #define LDS_AS __attribute__((address_space(3)))
-// lm8: the lane's masked word as the LDS byte address of its table entry
-// (the table starts at the LDS base, which is 0: the kernel has no static
-// LDS), so a term's address is one XOR with a uniform offset.
-template <typename S, int K, int NKER, bool SPLIT, typename CP>
-__device__ __forceinline__ uint32_t scan_outputs_u(const ScanStepArgs& a, CP combos, uint32_t lm8, uint32_t hw,
-                                                   const uint32_t* hstep, int nkw, S (&outv)[K]) {
+// lm8: the lane's masked word as the LDS byte address of its entry in the
+// first table (the tables start at the LDS base, which is 0: the kernel has
+// no static LDS), so a term's address is one XOR with a uniform offset.
+template <typename S, int K, int NKER, bool SPLIT, int P, int TBYTES, bool TRACK, typename CP>
+__device__ __forceinline__ void scan_outputs_u(const ScanStepArgs& a, const double2 (&f0)[P], const double2 (&f1)[P],
+                                               CP combos, uint32_t lm8, uint32_t hw, const uint32_t* hstep, int nkw,
+                                               S (&outv)[P][K], uint32_t (&mxh)[P]) {
 #pragma clang fp contract(off)
   constexpr int NC = 1 << NKER, HALF = SPLIT ? NC / 2 : NC;
   constexpr int SH = sizeof(S) == 8 ? 3 : 4;   // log2 of an entry's bytes
@@ -179,7 +184,9 @@ __device__ __forceinline__ uint32_t scan_outputs_u(const ScanStepArgs& a, CP com
     cm8[t] = cm << SH;
     par[t] = (uint32_t)__popc(cm & a.beta) & 1u;
   }
-  uint32_t mxh = 0, hs = 0;
+#pragma unroll
+  for (int q = 0; q < P; ++q) mxh[q] = 0;
+  uint32_t hs = 0;
 #pragma unroll
   for (int kk = 0; kk < K; ++kk) {
     if (kk) hs ^= hstep[__builtin_ctz(kk)];
@@ -194,41 +201,50 @@ __device__ __forceinline__ uint32_t scan_outputs_u(const ScanStepArgs& a, CP com
 #pragma unroll
       for (int t = 0; t < NC; ++t) neg[t] = SPLIT ? (t < HALF ? 0u : sb ^ par[t]) : (yb & (sb ^ par[t]));
       const uint32_t row8 = lm8 ^ (bm << SH);
-      double s0r = 0.0, s0i = 0.0, s1r = 0.0, s1i = 0.0;
+      double s0r[P], s0i[P], s1r[P], s1i[P];
 #pragma unroll
       for (int t = 0; t < NC; ++t) {
-        const double2 e = load_c(*reinterpret_cast<const S LDS_AS*>(row8 ^ cm8[t]));
+        // one address for every proposal: proposal q's table is TBYTES
+        // further on (a constant offset of the LDS read)
+        const uint32_t at = row8 ^ cm8[t];
         const int first = t < HALF ? 0 : HALF;
         const double rho = (neg[t] ^ neg[first]) ? -1.0 : 1.0;
-        if (t == 0) {
-          s0r = e.x;
-          s0i = e.y;
-        } else if (t < HALF) {
-          s0r = fma(e.x, rho, s0r);
-          s0i = fma(e.y, rho, s0i);
-        } else if (t == HALF) {
-          s1r = e.x;
-          s1i = e.y;
-        } else {
-          s1r = fma(e.x, rho, s1r);
-          s1i = fma(e.y, rho, s1i);
+#pragma unroll
+        for (int q = 0; q < P; ++q) {
+          const double2 e = load_c(*reinterpret_cast<const S LDS_AS*>(at + q * TBYTES));
+          if (t == 0) {
+            s0r[q] = e.x;
+            s0i[q] = e.y;
+          } else if (t < HALF) {
+            s0r[q] = fma(e.x, rho, s0r[q]);
+            s0i[q] = fma(e.y, rho, s0i[q]);
+          } else if (t == HALF) {
+            s1r[q] = e.x;
+            s1i[q] = e.y;
+          } else {
+            s1r[q] = fma(e.x, rho, s1r[q]);
+            s1i[q] = fma(e.y, rho, s1i[q]);
+          }
         }
       }
-      double orr, oi;
-      if (SPLIT) {
-        const double g1x = flip_sign(a.f1.x, neg[HALF]), g1y = flip_sign(a.f1.y, neg[HALF]);
-        orr = fma(s1r, g1x, fma(-s1i, g1y, fma(s0r, a.f0.x, -s0i * a.f0.y)));
-        oi = fma(s1r, g1y, fma(s1i, g1x, fma(s0r, a.f0.y, s0i * a.f0.x)));
-      } else {
-        const double fx = flip_sign(yb ? a.f1.x : a.f0.x, neg[0]), fy = flip_sign(yb ? a.f1.y : a.f0.y, neg[0]);
-        orr = fma(s0r, fx, -s0i * fy);
-        oi = fma(s0r, fy, s0i * fx);
+#pragma unroll
+      for (int q = 0; q < P; ++q) {
+        double orr, oi;
+        if (SPLIT) {
+          const double g1x = flip_sign(f1[q].x, neg[HALF]), g1y = flip_sign(f1[q].y, neg[HALF]);
+          orr = fma(s1r[q], g1x, fma(-s1i[q], g1y, fma(s0r[q], f0[q].x, -s0i[q] * f0[q].y)));
+          oi = fma(s1r[q], g1y, fma(s1i[q], g1x, fma(s0r[q], f0[q].y, s0i[q] * f0[q].x)));
+        } else {
+          const double fx = flip_sign(yb ? f1[q].x : f0[q].x, neg[0]);
+          const double fy = flip_sign(yb ? f1[q].y : f0[q].y, neg[0]);
+          orr = fma(s0r[q], fx, -s0i[q] * fy);
+          oi = fma(s0r[q], fy, s0i[q] * fx);
+        }
+        if (TRACK) mxh[q] = max(mxh[q], sq_hi(orr, oi));
+        outv[q][kk] = make_s<S>(orr, oi);
       }
-      mxh = max(mxh, sq_hi(orr, oi));
-      outv[kk] = make_s<S>(orr, oi);
     }
   }
-  return mxh;
 }
 
 // Wave w of the workgroup computes the output blocks k = w + W g of a step
@@ -253,36 +269,48 @@ __device__ inline void reject_outputs(int count, double2* out_val, int32_t* out_
 }
 
 constexpr int kStepHdrWords = 8;     // per step: header (2), beta, offset, 4 combinations
+
+// The tables are renormalised by a power of two on every kRenormSteps-th
+// step of rank 8 or more (and on every smaller step and the last); between
+// those the outputs keep their scale.  A power of two changes no mantissa
+// (sums are FP64, the table complex64 or complex128, and a step changes
+// magnitudes by a few bits at most, far from the formats' ranges), and the
+// last step's renormalisation fixes the returned (value, exponent) pair, so
+// the results are the same bits as renormalising every step.
+constexpr int kRenormSteps = 8;
 constexpr int kMaxPlanSteps = 128;   // packed plans have at most 128 steps
 
-template <typename S, int K, int W>
+template <typename S, int K, int W, int P, int R>
 __global__ void __launch_bounds__(64 * W) k_scan_batch(const uint64_t* __restrict__ plan,
                                                       const double2* __restrict__ u,
                                                       const uint64_t* __restrict__ bits, int count,
-                                                      double2* __restrict__ out_val, int32_t* __restrict__ out_exp,
-                                                      int rcap) {
+                                                      double2* __restrict__ out_val, int32_t* __restrict__ out_exp) {
   static_assert((K & (K - 1)) == 0 && (W & (W - 1)) == 0, "K and W are powers of two");
+  static_assert(P * W <= 16, "the waves' maxima hold 16 words");
   constexpr int KB = K > 1 ? __builtin_ctz(K) : 0;   // Gray steps within the wave's blocks
   constexpr int SH = sizeof(S) == 8 ? 3 : 4;   // log2 of an entry's bytes
-  // the table starts at LDS byte 0 (so an entry's byte offset is its index
-  // shifted), the waves' maxima follow it
+  constexpr int TN = 1 << R;                   // entries per table
+  constexpr int TBYTES = TN * (int)sizeof(S);
+  // P tables from LDS byte 0 (so an entry's byte offset is its index
+  // shifted, and proposal q's entry is TBYTES further on), the waves' maxima
+  // after them
   extern __shared__ __align__(16) unsigned char smem[];
   S* E = reinterpret_cast<S*>(smem);
-  uint32_t* wmx = reinterpret_cast<uint32_t*>(smem + ((size_t)1 << rcap) * sizeof(S));
+  uint32_t* wmx = reinterpret_cast<uint32_t*>(smem + (size_t)P * TBYTES);
   const int lane = threadIdx.x & 63;
   const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);   // uniform: plan reads stay scalar
   const int n = (int)plan[0];
-  // the table's LDS byte address; terms XOR their offsets into it, which
-  // equals adding them because the table is aligned to its own size
+  // the tables' LDS byte address; terms XOR their offsets into it, which
+  // equals adding them because each table is aligned to its own size
   const uint32_t lds_base = (uint32_t)(uintptr_t)(LDS_AS unsigned char*)smem;
-  if ((int)plan[1] > rcap || (lds_base & ((2u << (rcap + SH)) - 1u))) {
+  if ((int)plan[1] > R || (lds_base & ((2u * TBYTES) - 1u))) {
     // the plan's tables would not fit (or the table is not size-aligned):
     // mark every output, touch no table
     reject_outputs(count, out_val, out_exp);
     return;
   }
   // Each step's header words, staged in LDS once per workgroup (after the
-  // table and the maxima), so a step starts from an LDS read instead of a
+  // tables and the maxima), so a step starts from an LDS read instead of a
   // chain of dependent global reads (offset, then header, then combinations).
   uint32_t* shdr = wmx + 16;
   for (int i = threadIdx.x; i < n; i += 64 * W) {
@@ -298,16 +326,25 @@ __global__ void __launch_bounds__(64 * W) k_scan_batch(const uint64_t* __restric
     h[3] = (uint32_t)off;
     for (int t = 0; t < 4; ++t) h[4 + t] = t < (1 << nker) ? (uint32_t)cb[t] : 0u;
   }
-  for (int pidx = blockIdx.x; pidx < count; pidx += gridDim.x) {
-    const uint64_t x0 = bits[2 * pidx], x1 = bits[2 * pidx + 1];
-    if (threadIdx.x == 0) E[0] = make_s<S>(1.0, 0.0);
+  // P proposals at a time; past the end of the batch a workgroup repeats
+  // its first proposal and discards the copy
+  for (int pb = blockIdx.x * P; pb < count; pb += gridDim.x * P) {
+    uint64_t x0[P], x1[P];
+#pragma unroll
+    for (int q = 0; q < P; ++q) {
+      const int pq = pb + q < count ? pb + q : pb;
+      x0[q] = bits[2 * pq];
+      x1[q] = bits[2 * pq + 1];
+    }
+    if (threadIdx.x < P) E[threadIdx.x * TN] = make_s<S>(1.0, 0.0);
     __syncthreads();
-    int esum = 0;
-    // The table holds each step's outputs unscaled; the power of two that
+    int esum[P], e_prev[P];
+#pragma unroll
+    for (int q = 0; q < P; ++q) esum[q] = e_prev[q] = 0;
+    // The tables hold each step's outputs unscaled; the power of two that
     // brings the step's largest |output| into [1/2, 1) is applied to the
     // next step's weights instead (exact in FP64, so every sum rounds as if
     // the table had been scaled).
-    int e_prev = 0;
     for (int j = 0; j < n; ++j) {
       const uint32_t* h = shdr + kStepHdrWords * j;
       const uint32_t hlo = __builtin_amdgcn_readfirstlane(h[0]), hhi = __builtin_amdgcn_readfirstlane(h[1]);
@@ -322,97 +359,144 @@ __global__ void __launch_bounds__(64 * W) k_scan_batch(const uint64_t* __restric
       a.r_in = (int)((hdr >> 8) & 255u);
       a.nker = (int)((hdr >> 24) & 255u);
       a.beta = __builtin_amdgcn_readfirstlane(h[2]);
-      const int xv = (int)((v < 64 ? (x0 >> v) : (x1 >> (v - 64))) & 1u);
-      a.f0 = u[(v * 2 + xv) * 2 + 0];
-      a.f1 = u[(v * 2 + xv) * 2 + 1];
-      a.f0.x = ldexp(a.f0.x, -e_prev);
-      a.f0.y = ldexp(a.f0.y, -e_prev);
-      a.f1.x = ldexp(a.f1.x, -e_prev);
-      a.f1.y = ldexp(a.f1.y, -e_prev);
       a.mask = (1u << a.r_in) - 1u;
+      double2 f0[P], f1[P];
+#pragma unroll
+      for (int q = 0; q < P; ++q) {
+        const int xv = (int)((v < 64 ? (x0[q] >> v) : (x1[q] >> (v - 64))) & 1u);
+        f0[q] = u[(v * 2 + xv) * 2 + 0];
+        f1[q] = u[(v * 2 + xv) * 2 + 1];
+        f0[q].x = ldexp(f0[q].x, -e_prev[q]);
+        f0[q].y = ldexp(f0[q].y, -e_prev[q]);
+        f1[q].x = ldexp(f1[q].x, -e_prev[q]);
+        f1[q].y = ldexp(f1[q].y, -e_prev[q]);
+      }
       const int nk = r_out > 6 ? 1 << (r_out - 6) : 1;
       const int nkw = nk >= W ? nk / W : (wave < nk ? 1 : 0);
       const bool lane_ok = r_out >= 6 || lane < (1 << r_out);
-      const uint32_t lo_w = (uint32_t)st[2 + lane] ^ (wave < nk ? (uint32_t)st[66 + wave] : 0u);
-      uint32_t hstep[KB > 0 ? KB : 1];
-#pragma unroll
-      for (int b = 0; b < KB; ++b) hstep[b] = (W << b) < nk ? (uint32_t)st[66 + (W << b)] : 0u;
-      S outv[K];
-      uint32_t mxh;
+      // renormalise on every kRenormSteps-th step, on small steps and on the
+      // last (see kRenormSteps)
+      const bool renorm = r_out < 8 || j % kRenormSteps == kRenormSteps - 1 || j == n - 1;
+      S outv[P][K];
+      uint32_t mxh[P];
       // the packed plan has nker <= 2, and split steps nker >= 1
       if ((hdr >> 33) & 1u) {
+        uint32_t hstep[KB > 0 ? KB : 1];
+#pragma unroll
+        for (int b = 0; b < KB; ++b) hstep[b] = (W << b) < nk ? (uint32_t)st[66 + (W << b)] : 0u;
         uint32_t lm8 = (((uint32_t)st[2 + lane] & a.mask) << SH) + lds_base;
         asm volatile("" : "+v"(lm8));   // keep the shifted word: terms then need one XOR each
         const uint32_t hw = wave < nk ? (uint32_t)st[66 + wave] : 0u;
-        if (split) {
-          mxh = a.nker == 1 ? scan_outputs_u<S, K, 1, true>(a, combos, lm8, hw, hstep, nkw, outv)
-                            : scan_outputs_u<S, K, 2, true>(a, combos, lm8, hw, hstep, nkw, outv);
-        } else {
-          mxh = a.nker == 0   ? scan_outputs_u<S, K, 0, false>(a, combos, lm8, hw, hstep, nkw, outv)
-                : a.nker == 1 ? scan_outputs_u<S, K, 1, false>(a, combos, lm8, hw, hstep, nkw, outv)
-                              : scan_outputs_u<S, K, 2, false>(a, combos, lm8, hw, hstep, nkw, outv);
-        }
+        auto run = [&](auto track) {
+          constexpr bool T = decltype(track)::value;
+          if (split) {
+            if (a.nker == 1)
+              scan_outputs_u<S, K, 1, true, P, TBYTES, T>(a, f0, f1, combos, lm8, hw, hstep, nkw, outv, mxh);
+            else
+              scan_outputs_u<S, K, 2, true, P, TBYTES, T>(a, f0, f1, combos, lm8, hw, hstep, nkw, outv, mxh);
+          } else {
+            if (a.nker == 0)
+              scan_outputs_u<S, K, 0, false, P, TBYTES, T>(a, f0, f1, combos, lm8, hw, hstep, nkw, outv, mxh);
+            else if (a.nker == 1)
+              scan_outputs_u<S, K, 1, false, P, TBYTES, T>(a, f0, f1, combos, lm8, hw, hstep, nkw, outv, mxh);
+            else
+              scan_outputs_u<S, K, 2, false, P, TBYTES, T>(a, f0, f1, combos, lm8, hw, hstep, nkw, outv, mxh);
+          }
+        };
+        if (renorm)
+          run(std::true_type());
+        else
+          run(std::false_type());
       } else {
         // steps without the uniform flag have r_out < 8: at most two blocks,
-        // so at most one per wave (nkw <= 1)
-        S (&o1)[1] = *reinterpret_cast<S(*)[1]>(&outv[0]);
-        if (split) {
-          mxh = a.nker == 1 ? scan_outputs<S, 1, 1, true>(E, a, combos, lo_w, hstep, nkw, o1)
-                            : scan_outputs<S, 1, 2, true>(E, a, combos, lo_w, hstep, nkw, o1);
-        } else {
-          mxh = a.nker == 0   ? scan_outputs<S, 1, 0, false>(E, a, combos, lo_w, hstep, nkw, o1)
-                : a.nker == 1 ? scan_outputs<S, 1, 1, false>(E, a, combos, lo_w, hstep, nkw, o1)
-                              : scan_outputs<S, 1, 2, false>(E, a, combos, lo_w, hstep, nkw, o1);
+        // so at most one per wave (nkw <= 1); each proposal in turn
+        const uint32_t lo_w = (uint32_t)st[2 + lane] ^ (wave < nk ? (uint32_t)st[66 + wave] : 0u);
+        const uint32_t hstep1[1] = {0u};
+#pragma unroll
+        for (int q = 0; q < P; ++q) {
+          ScanStepArgs aq = a;
+          aq.f0 = f0[q];
+          aq.f1 = f1[q];
+          S (&o1)[1] = *reinterpret_cast<S(*)[1]>(&outv[q][0]);
+          const S* Eq = E + q * TN;
+          if (split) {
+            mxh[q] = a.nker == 1 ? scan_outputs<S, 1, 1, true>(Eq, aq, combos, lo_w, hstep1, nkw, o1)
+                                 : scan_outputs<S, 1, 2, true>(Eq, aq, combos, lo_w, hstep1, nkw, o1);
+          } else {
+            mxh[q] = a.nker == 0   ? scan_outputs<S, 1, 0, false>(Eq, aq, combos, lo_w, hstep1, nkw, o1)
+                     : a.nker == 1 ? scan_outputs<S, 1, 1, false>(Eq, aq, combos, lo_w, hstep1, nkw, o1)
+                                   : scan_outputs<S, 1, 2, false>(Eq, aq, combos, lo_w, hstep1, nkw, o1);
+          }
         }
       }
       // lanes past 2^r_out (steps of rank below six) computed in-range values
       // that are neither counted nor stored
-      mxh = wave_max_u32(lane_ok ? mxh : 0u);
-      if (W > 1 && lane == 0) wmx[wave] = mxh;
-      __syncthreads();  // every read of E done; the waves' maxima visible
-      if (W > 1) {
+      if (renorm) {
 #pragma unroll
-        for (int w2 = 0; w2 < W; ++w2) mxh = max(mxh, wmx[w2]);
-      }
-      const int e = scale_exponent(mxh);
-      if (lane_ok) {
-#pragma unroll
-        for (int kk = 0; kk < K; ++kk) {
-          if (kk < nkw) E[lane + 64 * (wave + W * (kk ^ (kk >> 1)))] = outv[kk];
+        for (int q = 0; q < P; ++q) {
+          mxh[q] = wave_max_u32(lane_ok ? mxh[q] : 0u);
+          if (W > 1 && lane == 0) wmx[q * W + wave] = mxh[q];
         }
       }
-      esum += e;
-      e_prev = e;
+      __syncthreads();  // every read of the tables done; the waves' maxima visible
+#pragma unroll
+      for (int q = 0; q < P; ++q) {
+        int e = 0;
+        if (renorm) {
+          if (W > 1) {
+#pragma unroll
+            for (int w2 = 0; w2 < W; ++w2) mxh[q] = max(mxh[q], wmx[q * W + w2]);
+          }
+          e = scale_exponent(mxh[q]);
+        }
+        if (lane_ok) {
+#pragma unroll
+          for (int kk = 0; kk < K; ++kk) {
+            if (kk < nkw) E[q * TN + lane + 64 * (wave + W * (kk ^ (kk >> 1)))] = outv[q][kk];
+          }
+        }
+        esum[q] += e;
+        e_prev[q] = e;
+      }
       __syncthreads();
     }
-    if (threadIdx.x == 0) {
-      double2 v = load_c(E[0]);
-      out_val[pidx] = make_double2(ldexp(v.x, -e_prev), ldexp(v.y, -e_prev));
-      out_exp[pidx] = esum;
+    if (threadIdx.x < P && pb + (int)threadIdx.x < count) {
+      const int q = threadIdx.x;
+      int ep = e_prev[0], es = esum[0];
+#pragma unroll
+      for (int q2 = 1; q2 < P; ++q2)
+        if (q == q2) {
+          ep = e_prev[q2];
+          es = esum[q2];
+        }
+      double2 v = load_c(E[q * TN]);
+      out_val[pb + q] = make_double2(ldexp(v.x, -ep), ldexp(v.y, -ep));
+      out_exp[pb + q] = es;
     }
     __syncthreads();
   }
 }
 
-template <typename S, int K, int W>
+template <typename S, int K, int W, int P, int R>
 static hipError_t launch_scan(const uint64_t* plan, const double2* u, const uint64_t* bits, int count,
-                                double2* val, int32_t* ex, int rmax, hipStream_t st) {
-  // the table, the waves' maxima (64 bytes), then the step headers
-  size_t shmem = ((size_t)1 << rmax) * sizeof(S) + 64 + kStepHdrWords * 4 * kMaxPlanSteps;
+                                double2* val, int32_t* ex, hipStream_t st) {
+  // the tables, the waves' maxima (64 bytes), then the step headers
+  size_t shmem = (size_t)P * ((size_t)1 << R) * sizeof(S) + 64 + kStepHdrWords * 4 * kMaxPlanSteps;
   // at least two rounds of the workgroups that fit a CU at once (160 KiB of
   // LDS, 32 waves), and at least 16 per CU (even shares of large batches)
   const int per_cu = std::max(1, std::min((int)((160 * 1024) / shmem), 32 / W));
-  const int grid = std::min(count, 256 * std::max(16, 2 * per_cu));
+  const int groups = (count + P - 1) / P;
+  const int grid = std::min(groups, 256 * std::max(16, 2 * per_cu));
   static bool attr = false;
   if (!attr) {
     // dynamic LDS up to the CU's 160 KiB less the kernel's own static array
-    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_scan_batch<S, K, W>),
+    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_scan_batch<S, K, W, P, R>),
                                        hipFuncAttributeMaxDynamicSharedMemorySize, 160 * 1024 - 256);
     if (e != hipSuccess) return e;
     attr = true;
   }
-  hipLaunchKernelGGL((k_scan_batch<S, K, W>), dim3(grid), dim3(64 * W), shmem, st, plan, u, bits, count, val, ex,
-                     rmax);
+  hipLaunchKernelGGL((k_scan_batch<S, K, W, P, R>), dim3(grid), dim3(64 * W), shmem, st, plan, u, bits, count, val,
+                     ex);
   return hipGetLastError();
 }
 
@@ -423,14 +507,21 @@ static hipError_t launch_scan(const uint64_t* plan, const double2* u, const uint
 // blocks than that use one wave per block.
 constexpr int kWavesC64 = 4, kWavesC128 = 4, kWavesC128R12 = 8;
 
+// Proposals per workgroup: the plan-only index and sign work of an output is
+// shared by this many proposals, their tables side by side in LDS.  Two for
+// complex64 tables up to rank 11 (rank 11: 5.15 -> 4.63 ms per 65,536
+// proposals); one at rank 12 and for complex128 tables, where the second
+// table and its registers cost more occupancy than the sharing saves.
+constexpr int kPropsC64 = 2;
+
 // Tables of rank up to R: 2^(R-6) output blocks of 64 entries, shared by
 // min(W, blocks) waves with K blocks each.
-template <typename S, int R, int W>
+template <typename S, int R, int W, int P = 1>
 static hipError_t launch_rank(const uint64_t* plan, const double2* u, const uint64_t* bits, int count, double2* val,
                               int32_t* ex, hipStream_t st) {
   constexpr int NB = R > 6 ? 1 << (R - 6) : 1;
   constexpr int WE = W < NB ? W : NB;
-  return launch_scan<S, NB / WE, WE>(plan, u, bits, count, val, ex, R, st);
+  return launch_scan<S, NB / WE, WE, P, R>(plan, u, bits, count, val, ex, st);
 }
 
 // ---------------------------------------------------------------- rank <= 6
@@ -864,9 +955,9 @@ extern "C" int rgs_scan_batch(const uint64_t* plan_dev, int rmax, const double* 
   hipError_t e;
   if (precision == 64) {
     if (rmax <= 6) e = launch_small<float2>(plan_dev, u, bits_dev, count, val, out_exp, st);
-    else if (rmax <= 8) e = launch_rank<float2, 8, kWavesC64>(plan_dev, u, bits_dev, count, val, out_exp, st);
-    else if (rmax <= 10) e = launch_rank<float2, 10, kWavesC64>(plan_dev, u, bits_dev, count, val, out_exp, st);
-    else if (rmax <= 11) e = launch_rank<float2, 11, kWavesC64>(plan_dev, u, bits_dev, count, val, out_exp, st);
+    else if (rmax <= 8) e = launch_rank<float2, 8, kWavesC64, kPropsC64>(plan_dev, u, bits_dev, count, val, out_exp, st);
+    else if (rmax <= 10) e = launch_rank<float2, 10, kWavesC64, kPropsC64>(plan_dev, u, bits_dev, count, val, out_exp, st);
+    else if (rmax <= 11) e = launch_rank<float2, 11, kWavesC64, kPropsC64>(plan_dev, u, bits_dev, count, val, out_exp, st);
     else if (rmax <= 12) e = launch_rank<float2, 12, kWavesC64>(plan_dev, u, bits_dev, count, val, out_exp, st);
     else if (rmax <= 13) e = launch_rank<float2, 13, kWavesC64>(plan_dev, u, bits_dev, count, val, out_exp, st);
     else return fail(kInvalid, "scan_batch: complex64 batch limited to rmax <= 13");
