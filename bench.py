@@ -119,7 +119,7 @@ def run_headline(args, torch):
         ev[k][npass][1].record(stream)
         ev[k][npass + 1][0].record(stream)
         _lib.call('rgs_draw', _lib.ptr(sl.state), n, _lib.ptr(sl.pyramid), sl.mass_bits, _lib.ptr(uni[k]),
-                  args.samples, _lib.ptr(idx), _lib.ptr(pdr), sp)
+                  args.samples, s.flip, _lib.ptr(idx), _lib.ptr(pdr), sp)
         ev[k][npass + 1][1].record(stream)
 
     for k in range(args.warmup):
@@ -180,7 +180,7 @@ def run_folded(s, args, torch):
         _lib.call('rgs_slab_pipeline', ctypes_ref(d), sp)
         _lib.call('rgs_mass_pyramid', _lib.ptr(sl.pyramid), sl.base_bits, sp)
         _lib.call('rgs_draw', _lib.ptr(sl.state), s.n, _lib.ptr(sl.pyramid), sl.mass_bits, _lib.ptr(uni),
-                  args.samples, _lib.ptr(idx), _lib.ptr(pdr), sp)
+                  args.samples, s.flip, _lib.ptr(idx), _lib.ptr(pdr), sp)
     step()
     torch.cuda.synchronize()
     t0 = time.perf_counter()

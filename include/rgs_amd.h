@@ -130,10 +130,15 @@ typedef struct rgs_slab_desc {
                                   (as init_rows); required with phase_lut   */
   const float *dphase_lut;     /* dev complex64 [ceil(nbits/8)][256] diagonal
                                   phase applied with phase_lut, or NULL      */
-  const float *rot;            /* HOST float32 [nbits][2] (cos, sin) of the
-                                  real rotation of each qubit, or NULL; read
-                                  when a pass is enqueued (the values travel
-                                  in the kernel arguments)                   */
+  const float *rot;            /* HOST float32 [nbits] tangents t of the
+                                  qubits' rotations M(t) = [[1, -t], [t, 1]]
+                                  (rgs_gate_shear), or NULL; read when a pass
+                                  is enqueued (the values travel in the kernel
+                                  arguments).  The state is multiplied once
+                                  by rgs_gate_scale(rot, gate_limit): in the
+                                  init (GRAPH, DELTA, and COV through its
+                                  staged weights) or, for LOAD, as the first
+                                  pass loads it                              */
   int gate_limit;              /* rotations on qubits >= gate_limit skipped */
   double *mass;                /* dev float64[2^(nbits-mass_bits)] or NULL  */
   int mass_bits;               /* granularity of the mass level (4..10)     */
@@ -173,6 +178,10 @@ typedef struct rgs_slab_desc {
   int group_rounds[8];         /*   LDS round bits of explicit group i (0..2,
                                   applied where <= its lane bits and <= its
                                   tile bits - 10)                            */
+  uint64_t flip;               /* outcome x is held at state index x ^ flip
+                                  (rgs_gate_shear's flips of the gates in
+                                  rot, below gate_limit): the mass level is
+                                  written in outcome order                   */
   uint64_t init_omask;         /* COV with init_wpop: the B outcome bits o   */
   int init_wpop;               /* COV: 0: init_wlut holds chunk products
                                   (init_wbits); 1: it holds the (nbits+1)^2
@@ -180,15 +189,27 @@ typedef struct rgs_slab_desc {
                                   read at [popc(b)][popc(b & init_omask)]    */
 } rgs_slab_desc;
 
-/* Factor 2x2 unitaries u (complex128 [n][2][2]) as u = D1 R D2 with
- * R = [[c, -s], [s, c]] real, D1 = diag(e^{i a0}, e^{i a1}),
- * D2 = diag(1, e^{i b}).  Outputs (each optional): rot float32 [n][2] (c, s);
- * dphase_lut complex64 [ceil(n/8)][256], the product of the D2 phases of the
- * qubits < gate_limit selected by each byte; d1 complex128 [n][2].  The slab
- * pipeline applies D2 with the pointwise phase and R per qubit; D1 changes no
- * probability and is applied to amplitudes read back (graph_sampler.apply_one
- * semantics, work/rgs_attack/graph_sampler.py:71-74). */
-int rgs_gate_zyz(const double *u, int n, int gate_limit, float *rot, float *dphase_lut, double *d1);
+/* Factor 2x2 unitaries u (complex128 [n][2][2]) as
+ *   u = X^f D1 m M(t) D2,   M(t) = [[1, -t], [t, 1]], m = (1 + t^2)^(-1/2),
+ * from u = D1 R D2 with R = [[c, -s], [s, c]] real, D1 diagonal and
+ * D2 = diag(1, e^{i b}): t = s / c (f = 0) when c >= s, else t = -c / s with
+ * the bit flip X (f = 1) and D1 replaced by diag(e^{i a1}, -e^{i a0}), so
+ * |t| <= 1.  The slab pipeline applies D2 with the pointwise phase, M(t) per
+ * qubit (two fused multiply-adds per amplitude and qubit) and the product of
+ * the m's once (rgs_gate_scale); D1 changes no probability and the flips only
+ * relabel the outcome (index x of the state is outcome x ^ flip), both applied
+ * to amplitudes and indices read back (graph_sampler.apply_one semantics,
+ * work/rgs_attack/graph_sampler.py:71-74).  Outputs (each optional):
+ * tan float32 [n] (t, rounded once); dphase_lut complex64 [ceil(n/8)][256],
+ * the product of the D2 phases of the qubits < gate_limit selected by each
+ * byte; d1 complex128 [n][2]; flip: bit q set iff f_q = 1 (q < gate_limit;
+ * the other qubits keep f = 0). */
+int rgs_gate_shear(const double *u, int n, int gate_limit, float *tan, float *dphase_lut, double *d1,
+                   uint64_t *flip);
+
+/* prod_{q < gate_limit} (1 + tan[q]^2)^(-1/2) in FP64: the factor the slab
+ * pipeline applies once for the shear rotations (no reference counterpart). */
+double rgs_gate_scale(const float *tan, int gate_limit);
 
 /* Enqueue the fused pipeline.  Replaces SpecializedDeferredSampler.compute
  * and its kernels initcov_specialized_global, lowfwht, fusedfwht,
@@ -231,6 +252,7 @@ typedef struct rgs_slab_batch_item {
   int samples;
   int64_t *out_index;          /* dev int64[samples] */
   double *out_p;               /* dev float64[samples] */
+  uint64_t flip;               /* the descriptor's flip (rgs_draw)          */
 } rgs_slab_batch_item;
 int rgs_slab_batch_create(const rgs_slab_batch_item *items, int count, void **batch);
 int rgs_slab_batch_run(void *batch, void *stream);
@@ -248,11 +270,14 @@ int rgs_mass_pyramid(double *pyr, int base_bits, void *stream);
  * with t = u * root (go to child 1 iff t >= mass(child 0) and mass(child 1) > 0,
  * subtracting mass(child 0)), then inside the 2^mass_bits leaf block with masses
  * recomputed by the same pairwise tree from the complex64 amplitudes.
- * Outputs the state index and p = re^2 + im^2 (float64) of each draw.
- * Replaces DistributedSampler.draw's host searchsorted / tile copy
+ * The state holds outcome x at index x ^ flip (the rgs_gate_shear flips of
+ * its gates; 0 for none) and the pyramid is in outcome order (as the slab
+ * pipeline writes it), so draws descend in outcome order.  Outputs the
+ * outcome and p = re^2 + im^2 (float64) of each draw.  Replaces
+ * DistributedSampler.draw's host searchsorted / tile copy
  * (work/rgs_attack/distributed_sampler.py:153-157). */
 int rgs_draw(const void *state, int nbits, const double *pyr, int mass_bits,
-             const double *uniforms, int count, int64_t *out_index, double *out_p,
+             const double *uniforms, int count, uint64_t flip, int64_t *out_index, double *out_p,
              void *stream);
 
 /* float64 p = re^2 + im^2 at `count` indices (probability_at,

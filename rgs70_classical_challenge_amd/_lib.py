@@ -54,6 +54,7 @@ class SlabDesc(ctypes.Structure):
         ('group_bits', c_int * 8),
         ('group_lanes', c_int * 8),
         ('group_rounds', c_int * 8),
+        ('flip', ctypes.c_uint64),
         ('init_omask', ctypes.c_uint64),
         ('init_wpop', c_int),
     ]
@@ -66,6 +67,7 @@ class SlabBatchItem(ctypes.Structure):
         ('samples', c_int),
         ('out_index', c_void_p),
         ('out_p', c_void_p),
+        ('flip', ctypes.c_uint64),
     ]
 
 
@@ -77,7 +79,8 @@ SIGNATURES = {
     'rgs_gf2_rank': (c_int, [c_void_p, c_int, c_int, P(c_int)]),
     'rgs_graph_phase_lut': (c_int, [c_void_p, c_int, c_void_p]),
     'rgs_cov_luts': (c_int, [c_void_p, c_int, c_void_p, c_void_p, c_int, c_void_p, c_void_p]),
-    'rgs_gate_zyz': (c_int, [c_void_p, c_int, c_int, c_void_p, c_void_p, c_void_p]),
+    'rgs_gate_shear': (c_int, [c_void_p, c_int, c_int, c_void_p, c_void_p, c_void_p, c_void_p]),
+    'rgs_gate_scale': (c_double, [c_void_p, c_int]),
     'rgs_cov_weight_lut': (c_int, [c_void_p, c_int, c_void_p, c_void_p, c_int, c_double, c_void_p]),
     'rgs_cov_weight_lut_bits': (c_int, [c_void_p, c_int, c_void_p, c_void_p, c_int, c_double, c_int, c_void_p]),
     'rgs_cov_weight_table': (c_int, [c_void_p, c_void_p, c_int, c_double, c_void_p]),
@@ -85,7 +88,8 @@ SIGNATURES = {
     'rgs_slab_pass': (c_int, [P(SlabDesc), c_int, c_void_p]),
     'rgs_slab_describe': (c_int, [P(SlabDesc), P(c_int), c_char_p, c_int]),
     'rgs_mass_pyramid': (c_int, [c_void_p, c_int, c_void_p]),
-    'rgs_draw': (c_int, [c_void_p, c_int, c_void_p, c_int, c_void_p, c_int, c_void_p, c_void_p, c_void_p]),
+    'rgs_draw': (c_int, [c_void_p, c_int, c_void_p, c_int, c_void_p, c_int, ctypes.c_uint64, c_void_p, c_void_p,
+                         c_void_p]),
     'rgs_gather_prob': (c_int, [c_void_p, c_void_p, c_int, c_void_p, c_void_p]),
     'rgs_slab_batch_create': (c_int, [c_void_p, c_int, P(c_void_p)]),
     'rgs_slab_batch_run': (c_int, [c_void_p, c_void_p]),

@@ -118,7 +118,7 @@ class BatchSampler:
         call('rgs_slab_pipeline', ctypes_ref(s._desc), h)
         call('rgs_mass_pyramid', ptr(sl.pyramid), sl.base_bits, h)
         call('rgs_draw', ptr(sl.state), s.n, ptr(sl.pyramid), sl.mass_bits, ptr(item['uni']), self.samples,
-             ptr(item['idx']), ptr(item['p']), h)
+             s.flip, ptr(item['idx']), ptr(item['p']), h)
 
     def run_eager(self):
         """One launch sequence per instance on the current stream, wave by wave."""
@@ -240,6 +240,7 @@ class BatchSampler:
                 arr[k].samples = self.samples
                 arr[k].out_index = ptr(it['idx'])
                 arr[k].out_p = ptr(it['p'])
+                arr[k].flip = it['s'].flip
             h = ctypes.c_void_p()
             call('rgs_slab_batch_create', arr, len(items), ctypes.byref(h))
             self.fused.append((h, items))

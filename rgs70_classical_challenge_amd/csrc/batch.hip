@@ -172,7 +172,7 @@ extern "C" int rgs_slab_batch_create(const rgs_slab_batch_item* items, int count
     const rgs_slab_desc* d = items[i].desc;
     if (items[i].samples == 0) continue;
     djobs.push_back(DrawJob{reinterpret_cast<const float2*>(d->state), d->mass, items[i].uniforms, items[i].out_index,
-                            items[i].out_p, d->nbits, d->mass_bits});
+                            items[i].out_p, d->nbits, d->mass_bits, items[i].flip});
     dfirst.push_back((uint32_t)draws);
     draws += (uint64_t)items[i].samples;
   }

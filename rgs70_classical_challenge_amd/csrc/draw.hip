@@ -58,15 +58,16 @@ __global__ void __launch_bounds__(256) k_pyramid_multi(const PyrJob* __restrict_
 }
 
 // ------------------------------------------------------------------ draw
-// One wavefront per draw.  Lanes hold 4 consecutive amplitudes of the leaf
-// block (mass_bits <= 8 -> 256 amplitudes).  Level l of the pyramid (0 = base
-// with 2^B entries) starts at 2^(B+1) - 2^(B+1-l).
+// One wavefront per draw.  Lanes hold 4 consecutive outcomes of the leaf
+// block (mass_bits <= 8 -> 256 amplitudes); outcome x is read from state
+// index x ^ flip, and the pyramid is in outcome order.  Level l of the
+// pyramid (0 = base with 2^B entries) starts at 2^(B+1) - 2^(B+1-l).
 __device__ __forceinline__ uint64_t level_off(int B, int l) {
   return (1ull << (B + 1)) - (1ull << (B + 1 - l));
 }
 
 __device__ __forceinline__ void draw_one(const float2* state, int nbits, const double* pyr, int mass_bits, double u,
-                                         int64_t* out_index, double* out_p) {
+                                         uint64_t flip, int64_t* out_index, double* out_p) {
   const int lane = threadIdx.x;
   const int B = nbits - mass_bits;
   double t = u * pyr[level_off(B, B)];
@@ -88,7 +89,7 @@ __device__ __forceinline__ void draw_one(const float2* state, int nbits, const d
 #pragma unroll
   for (int j = 0; j < 4; ++j) {
     int l = lane * 4 + j;
-    p[j] = l < bsize ? prob64(state[bbase + l]) : 0.0;
+    p[j] = l < bsize ? prob64(state[(bbase + l) ^ flip]) : 0.0;
   }
   double lv[7];
   lv[0] = __dadd_rn(__dadd_rn(p[0], p[1]), __dadd_rn(p[2], p[3]));
@@ -127,11 +128,11 @@ __device__ __forceinline__ void draw_one(const float2* state, int nbits, const d
 }
 
 __global__ void __launch_bounds__(64) k_draw(const float2* state, int nbits, const double* pyr, int mass_bits,
-                                             const double* uniforms, int count, int64_t* out_index,
+                                             const double* uniforms, int count, uint64_t flip, int64_t* out_index,
                                              double* out_p) {
   const int d = blockIdx.x;
   if (d >= count) return;
-  draw_one(state, nbits, pyr, mass_bits, uniforms[d], out_index + d, out_p + d);
+  draw_one(state, nbits, pyr, mass_bits, uniforms[d], flip, out_index + d, out_p + d);
 }
 
 __global__ void __launch_bounds__(64) k_draw_multi(const DrawJob* __restrict__ jobs, const uint32_t* __restrict__ first,
@@ -139,7 +140,7 @@ __global__ void __launch_bounds__(64) k_draw_multi(const DrawJob* __restrict__ j
   const int i = job_of(first, count, blockIdx.x);
   const DrawJob& j = jobs[i];
   const uint32_t d = blockIdx.x - first[i];
-  draw_one(j.state, j.nbits, j.pyr, j.mass_bits, j.uniforms[d], j.out_index + d, j.out_p + d);
+  draw_one(j.state, j.nbits, j.pyr, j.mass_bits, j.uniforms[d], j.flip, j.out_index + d, j.out_p + d);
 }
 
 hipError_t launch_pyramid_multi(const PyrJob* jobs, const uint32_t* first, int count, uint32_t groups,
@@ -183,14 +184,14 @@ extern "C" int rgs_mass_pyramid(double* pyr, int base_bits, void* stream) {
 }
 
 extern "C" int rgs_draw(const void* state, int nbits, const double* pyr, int mass_bits, const double* uniforms,
-                        int count, int64_t* out_index, double* out_p, void* stream) {
+                        int count, uint64_t flip, int64_t* out_index, double* out_p, void* stream) {
   if (!state || !pyr || !uniforms || !out_index || !out_p) return fail(kInvalid, "null pointer");
   if (mass_bits < 2 || mass_bits > 8 || nbits - mass_bits < 0 || nbits - mass_bits > 47)
     return fail(kInvalid, "draw needs 2 <= mass_bits <= 8");
   if (count <= 0) return kOk;
   hipStream_t st = reinterpret_cast<hipStream_t>(stream);
   hipLaunchKernelGGL(k_draw, dim3(count), dim3(64), 0, st, reinterpret_cast<const float2*>(state), nbits, pyr,
-                     mass_bits, uniforms, count, out_index, out_p);
+                     mass_bits, uniforms, count, flip, out_index, out_p);
   RGS_LAUNCH_CHECK("k_draw");
   return kOk;
 }

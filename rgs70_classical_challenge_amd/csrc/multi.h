@@ -23,6 +23,7 @@ struct DrawJob {         // all draws of one state
   double* out_p;
   int nbits;
   int mass_bits;
+  uint64_t flip;         // outcome x is held at state index x ^ flip
 };
 
 hipError_t launch_pyramid_multi(const PyrJob* jobs, const uint32_t* first, int count, uint32_t groups,

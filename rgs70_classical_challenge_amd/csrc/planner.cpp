@@ -204,29 +204,25 @@ extern "C" int rgs_cov_luts(const uint8_t* g, int nv, const int32_t* A, const in
 // D2 = diag(1, e^{i b}): |u00| = c, |u10| = s, a0 = arg u00, a1 = arg u10,
 // b = arg u11 - a1 (= arg(-u01) - a0 by unitarity).  Degenerate columns
 // (c = 0 or s = 0) put the free phase into D1.
-// Float32 rounding of a unit pair (xd, yd).  Rounding cos and sin
-// independently leaves x^2 + y^2 - 1 = eps with |eps| up to about 6e-8, and a
-// rotation scales every probability by 1 + eps; identical gates on every qubit
-// would compound eps.  Among the floats within one ulp of (xd, yd) this picks
-// the pair whose eps keeps the running sum `*carry` of earlier choices nearest
-// zero (ties: nearest to (xd, yd)), so the product of the norms stays within
-// one rounding of 1 at any qubit count.  carry == NULL: smallest |eps|.
-static void unit_pair(double xd, double yd, float* x, float* y, double* carry) {
+// Float32 rounding of a unit complex number (xd, yd): rounding each part
+// independently leaves x^2 + y^2 - 1 = eps with |eps| up to about 6e-8, which
+// scales every probability the phase multiplies; among the floats within one
+// ulp of (xd, yd) this picks the pair of smallest |eps| (ties: nearest to
+// (xd, yd)).
+static void unit_pair(double xd, double yd, float* x, float* y) {
   const float x0 = (float)xd, y0 = (float)yd;
-  const double base = carry ? *carry : 0.0;
   float bx = x0, by = y0;
-  double best = 1e300, best_eps = 0.0, best_dist = 1e300;
+  double best = 1e300, best_dist = 1e300;
   for (int i = -1; i <= 1; ++i)
     for (int j = -1; j <= 1; ++j) {
       float cx = x0, cy = y0;
       if (i) cx = std::nextafter(x0, i > 0 ? 2.0f : -2.0f);
       if (j) cy = std::nextafter(y0, j > 0 ? 2.0f : -2.0f);
       const double eps = (double)cx * cx + (double)cy * cy - 1.0;
-      const double score = std::abs(base + eps);
+      const double score = std::abs(eps);
       const double dist = std::abs(cx - xd) + std::abs(cy - yd);
       if (score < best - 1e-18 || (std::abs(score - best) <= 1e-18 && dist < best_dist)) {
         best = score;
-        best_eps = eps;
         best_dist = dist;
         bx = cx;
         by = cy;
@@ -234,23 +230,26 @@ static void unit_pair(double xd, double yd, float* x, float* y, double* carry) {
     }
   *x = bx;
   *y = by;
-  if (carry) *carry = base + best_eps;
 }
 
-extern "C" int rgs_gate_zyz(const double* u, int n, int gate_limit, float* rot, float* dphase_lut, double* d1) {
-  if (!u || n < 1 || n > 63) return fail(kInvalid, "gate_zyz: need 1 <= n <= 63");
+
+extern "C" int rgs_gate_shear(const double* u, int n, int gate_limit, float* tan, float* dphase_lut, double* d1,
+                              uint64_t* flip) {
+  if (!u || n < 1 || n > 63) return fail(kInvalid, "gate_shear: need 1 <= n <= 63");
   using cd = std::complex<double>;
   std::vector<double> b(n, 0.0);
-  double carry = 0.0;  // running sum of the rotations' norm errors
+  uint64_t fl = 0;
   for (int q = 0; q < n; ++q) {
     const double* e = u + 8 * q;
     cd u00(e[0], e[1]), u01(e[2], e[3]), u10(e[4], e[5]), u11(e[6], e[7]);
     double c = std::abs(u00), s = std::abs(u10);
     double nrm = std::hypot(c, s);
     if (std::abs(nrm - 1.0) > 1e-9 || std::abs(std::abs(u01) - s) > 1e-9 || std::abs(std::abs(u11) - c) > 1e-9)
-      return fail(kInvalid, "gate_zyz: matrix is not unitary");
+      return fail(kInvalid, "gate_shear: matrix is not unitary");
     c /= nrm;
     s /= nrm;
+    // u = D1 R D2 with R = [[c, -s], [s, c]], D1 = diag(e^{i a0}, e^{i a1}),
+    // D2 = diag(1, e^{i b})
     double a0, a1, bb;
     if (s < 1e-300) {           // diagonal
       a0 = std::arg(u00);
@@ -269,15 +268,23 @@ extern "C" int rgs_gate_zyz(const double* u, int n, int gate_limit, float* rot, 
       a1 = std::arg(u10);
       bb = std::arg(u11) - a1;
     }
-    if (rot) unit_pair(c, s, rot + 2 * q, rot + 2 * q + 1, &carry);
+    // c >= s: R = c M(s / c).  s > c: R = X Z s M(-c / s) with X Z =
+    // [[0, -1], [1, 0]], and D1 X = X diag(e^{i a1}, e^{i a0}), so
+    // u = X diag(e^{i a1}, -e^{i a0}) s M(-c / s) D2.
+    const bool swap = q < gate_limit && s > c;
+    const double t = swap ? -c / s : (c > 0.0 ? s / c : 0.0);
+    if (tan) tan[q] = (float)t;
+    if (swap) fl |= 1ull << q;
     if (d1) {
-      d1[4 * q + 0] = std::cos(a0);
-      d1[4 * q + 1] = std::sin(a0);
-      d1[4 * q + 2] = std::cos(a1);
-      d1[4 * q + 3] = std::sin(a1);
+      const double p0 = swap ? a1 : a0, p1 = swap ? a0 : a1, sg1 = swap ? -1.0 : 1.0;
+      d1[4 * q + 0] = std::cos(p0);
+      d1[4 * q + 1] = std::sin(p0);
+      d1[4 * q + 2] = sg1 * std::cos(p1);
+      d1[4 * q + 3] = sg1 * std::sin(p1);
     }
     b[q] = q < gate_limit ? bb : 0.0;
   }
+  if (flip) *flip = fl;
   if (dphase_lut) {
     const int chunks = (n + 7) / 8;
     for (int ch = 0; ch < chunks; ++ch)
@@ -288,10 +295,16 @@ extern "C" int rgs_gate_zyz(const double* u, int n, int gate_limit, float* rot, 
           if (q < n && ((code >> t) & 1)) ang += b[q];
         }
         unit_pair(std::cos(ang), std::sin(ang), dphase_lut + 2 * (ch * 256 + code),
-                  dphase_lut + 2 * (ch * 256 + code) + 1, nullptr);
+                  dphase_lut + 2 * (ch * 256 + code) + 1);
       }
   }
   return kOk;
+}
+
+extern "C" double rgs_gate_scale(const float* tan, int gate_limit) {
+  double sc = 1.0;
+  for (int q = 0; tan && q < gate_limit; ++q) sc /= std::sqrt(1.0 + (double)tan[q] * (double)tan[q]);
+  return sc;
 }
 
 extern "C" int rgs_cov_weight_lut_bits(const double* u, int nv, const int32_t* B, const uint8_t* outB, int m,

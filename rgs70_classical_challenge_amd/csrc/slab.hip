@@ -161,6 +161,7 @@ int plan_passes(const rgs_slab_desc* d, std::vector<PassPlan>& out, std::string&
   for (auto& p : out)
     if ((p.flags & ~F_LOAD) || !(p.flags & F_LOAD)) kept.push_back(p);
   out.swap(kept);
+  for (size_t i = 0; i < out.size(); ++i) out[i].first = i == 0;
   // the strided pass that applies the phase tables holds them in LDS besides
   // the tile: it may take its own number of exchange rounds
   if (d->phase_round_bits >= 0)
@@ -241,6 +242,7 @@ int check_desc(const rgs_slab_desc* d, std::string& err) {
   }
   if (d->init_wbits != 0 && d->init_wbits != 8 && d->init_wbits != 9) { err = "init_wbits must be 8 or 9"; return kInvalid; }
   if (d->init_wpop != 0 && d->init_wpop != 1) { err = "init_wpop must be 0 or 1"; return kInvalid; }
+  if (d->flip >> n) { err = "flip has bits at or above nbits"; return kInvalid; }
   if (d->init_wpop && (d->init_omask >> n)) { err = "init_omask has bits at or above nbits"; return kInvalid; }
   if (d->phase_lut && !d->phase_rows) { err = "phase_lut needs its form rows"; return kInvalid; }
   if ((d->phase_lut || d->dphase_lut || d->init_kind == RGS_INIT_GRAPH || d->init_kind == RGS_INIT_COV) && n > 63) {
@@ -262,11 +264,15 @@ void build_args(const rgs_slab_desc* d, const PassPlan& p, PassArgs& a) {
   const int t0 = p.low ? 0 : p.qstart;
   for (int i = 0; i < 16; ++i) {
     const int q = t0 + i;
-    a.rot[i] = (d->rot && i < p.count && q < n) ? make_float2(d->rot[2 * q], d->rot[2 * q + 1])
-                                                : make_float2(1.f, 0.f);
+    a.tng[i] = (d->rot && i < p.count && q < n) ? d->rot[q] : 0.f;
   }
+  // the rotations' common scale, applied once: in the values the state is
+  // made from, or as the first pass loads a given state
+  const double gsc = d->rot ? rgs_gate_scale(d->rot, std::min(d->gate_limit, n)) : 1.0;
   a.mass = d->mass;
-  a.init_scale = d->init_scale;
+  a.init_scale = (float)((double)d->init_scale * gsc);
+  a.wscale = (float)gsc;
+  a.lscale = (p.first && d->init_kind == RGS_INIT_LOAD) ? (float)gsc : 1.f;
   a.nbits = n;
   a.qstart = p.low ? 0 : p.qstart;
   a.flags = p.flags;
@@ -299,6 +305,7 @@ void build_args(const rgs_slab_desc* d, const PassPlan& p, PassArgs& a) {
     }
   }
   a.mass_bits = d->mass_bits;
+  a.mflip = d->mass_bits < 64 ? d->flip >> d->mass_bits : 0;
   a.dphase_global = d->dphase_global ? 1 : 0;
   a.wbits = d->init_wbits ? d->init_wbits : 8;
   a.wchunks = (n + a.wbits - 1) / a.wbits;

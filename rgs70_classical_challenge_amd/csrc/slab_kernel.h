@@ -54,7 +54,7 @@ struct PassArgs {
   const uint64_t* phase_lut;  // [pc][256] sign words, or null
   const float2* dphase;       // [dc][256] complex diagonal phase, or null
   double* mass;
-  float2 rot[16];             // (cos, sin) of target bit i of the pass (identity beyond gate_limit)
+  float tng[16];              // tangent t of target bit i's rotation M(t) (0 beyond gate_limit)
   // Separable sign evaluation of the init / phase_lut forms over the four
   // register bits (global bits p_0..p_3) of the layout where each is applied:
   // q[i] = symmetric form row of p_i, freg bit j = f(r_j) ^ f(0) for the
@@ -68,7 +68,9 @@ struct PassArgs {
   LocalForm pform;
   uint32_t ifreg;
   uint32_t pfreg;
-  float init_scale;
+  float init_scale;           // GRAPH / DELTA value, the rotations' scale included
+  float wscale;               // COV: the rotations' scale, applied to the staged weights
+  float lscale;               // LOAD: the rotations' scale, applied as the first pass loads (else 1)
   int nbits;
   int qstart;      // global bit of local bit C (0 for the low tile)
   int flags;
@@ -84,6 +86,7 @@ struct PassArgs {
   int wpop;           // COV popcount table: its row stride m + 1 (0: chunk products)
   int wentries;       // COV weight entries staged in LDS
   uint64_t womask;    // COV popcount table: the B outcome bits
+  uint64_t mflip;     // mass blocks: block b of the state is outcome block b ^ mflip
 };
 
 __device__ __forceinline__ uint32_t phase_parity(uint64_t x, const uint64_t* lut, int chunks) {
@@ -336,17 +339,18 @@ __device__ __forceinline__ void wht_reg(float2 (&v)[16]) {
     }
 }
 
-// Real rotation [[c, -s], [s, c]] on register bit B.
+// Rotation M(t) = [[1, -t], [t, 1]] on register bit B (|t| <= 1; the
+// rotation's scale (1 + t^2)^(-1/2) is applied once for all qubits): two
+// fused multiply-adds per amplitude.
 template <int B>
-__device__ __forceinline__ void rot_reg(float2 (&v)[16], float2 cs) {
+__device__ __forceinline__ void rot_reg(float2 (&v)[16], float t) {
 #pragma unroll
   for (int j = 0; j < 16; ++j)
     if (!(j & (1 << B))) {
       const float2 x = v[j], y = v[j | (1 << B)];
       // rounded as written (see cmul), whatever kernel inlines this
-      v[j] = make_float2(__fmaf_rn(cs.x, x.x, -__fmul_rn(cs.y, y.x)), __fmaf_rn(cs.x, x.y, -__fmul_rn(cs.y, y.y)));
-      v[j | (1 << B)] =
-          make_float2(__fmaf_rn(cs.y, x.x, __fmul_rn(cs.x, y.x)), __fmaf_rn(cs.y, x.y, __fmul_rn(cs.x, y.y)));
+      v[j] = make_float2(__fmaf_rn(-t, y.x, x.x), __fmaf_rn(-t, y.y, x.y));
+      v[j | (1 << B)] = make_float2(__fmaf_rn(t, x.x, y.x), __fmaf_rn(t, x.y, y.y));
     }
 }
 
@@ -364,10 +368,10 @@ template <typename G, int C, int K>
 __device__ __forceinline__ void rot_layout(float2 (&v)[16], const PassArgs& a) {
   constexpr int s = G::at(K), h = G::hi(K);
   // target bit index of register bit B: s + B - C
-  if constexpr (s + 0 < h) if (a.qstart + s + 0 - C < a.gate_limit) rot_reg<0>(v, a.rot[s + 0 - C]);
-  if constexpr (s + 1 < h) if (a.qstart + s + 1 - C < a.gate_limit) rot_reg<1>(v, a.rot[s + 1 - C]);
-  if constexpr (s + 2 < h) if (a.qstart + s + 2 - C < a.gate_limit) rot_reg<2>(v, a.rot[s + 2 - C]);
-  if constexpr (s + 3 < h) if (a.qstart + s + 3 - C < a.gate_limit) rot_reg<3>(v, a.rot[s + 3 - C]);
+  if constexpr (s + 0 < h) if (a.qstart + s + 0 - C < a.gate_limit) rot_reg<0>(v, a.tng[s + 0 - C]);
+  if constexpr (s + 1 < h) if (a.qstart + s + 1 - C < a.gate_limit) rot_reg<1>(v, a.tng[s + 1 - C]);
+  if constexpr (s + 2 < h) if (a.qstart + s + 2 - C < a.gate_limit) rot_reg<2>(v, a.tng[s + 2 - C]);
+  if constexpr (s + 3 < h) if (a.qstart + s + 3 - C < a.gate_limit) rot_reg<3>(v, a.tng[s + 3 - C]);
 }
 
 // Walk down the layouts from K applying the Walsh butterflies (ends at layout count-1).
@@ -443,7 +447,13 @@ __device__ __forceinline__ void tile_run(const PassArgs& a, uint64_t blk) {
     if (!a.dphase_global)
       for (int i = tid; i < a.dphase_chunks * 256; i += G::NT) lut_d[i] = a.dphase[i];
     if (cov)
-      for (int i = tid; i < a.wentries; i += G::NT) lut_w[i] = a.init_w[i];
+      for (int i = tid; i < a.wentries; i += G::NT) {
+        // the rotations' scale enters every weight once: the popcount table's
+        // entries, or the first chunk table of a product of chunk tables
+        const float2 w = a.init_w[i];
+        const float sc = (a.wpop || i < (1 << a.wbits)) ? a.wscale : 1.f;
+        lut_w[i] = make_float2(__fmul_rn(w.x, sc), __fmul_rn(w.y, sc));
+      }
     __syncthreads();
   }
 
@@ -466,6 +476,10 @@ __device__ __forceinline__ void tile_run(const PassArgs& a, uint64_t blk) {
   if (a.flags & F_LOAD) {
 #pragma unroll
     for (int j = 0; j < 16; ++j) v[j] = load_stream(a.state + g0 + ((uint64_t)j << gsh));
+    if (a.lscale != 1.f) {
+#pragma unroll
+      for (int j = 0; j < 16; ++j) v[j] = make_float2(__fmul_rn(v[j].x, a.lscale), __fmul_rn(v[j].y, a.lscale));
+    }
   } else if (a.init_kind == RGS_INIT_GRAPH) {
     const uint32_t li = up ? local_base<SL, C, RBW>(tl, tr) : local_base<S1, C, RBW>(tl, tr);
     const uint32_t fb = phase_parity(base, a.init_lut, a.init_chunks);
@@ -568,7 +582,9 @@ __device__ __forceinline__ void tile_run(const PassArgs& a, uint64_t blk) {
       double m = p[0];
       const int sb = a.mass_bits - 4;
       for (int b = 0; b < sb; ++b) m = __dadd_rn(m, __shfl_xor(m, 1 << b));
-      if ((tid & ((1u << sb) - 1u)) == 0) a.mass[(base >> a.mass_bits) + (tid >> sb)] = m;
+      // a block's pairwise sum is the same in state and outcome order (a
+      // XOR relabelling keeps every pair), so only its slot moves
+      if ((tid & ((1u << sb) - 1u)) == 0) a.mass[((base >> a.mass_bits) + (tid >> sb)) ^ a.mflip] = m;
     }
   }
   if (!at_top) {

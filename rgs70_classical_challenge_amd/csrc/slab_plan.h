@@ -17,6 +17,7 @@ struct PassPlan {
   int rbits;   // log2 registers per thread
   int c;       // lane bits
   int rb;      // log2 LDS rounds
+  int first;   // 1: the schedule's first pass
 };
 
 // The pass schedule of a descriptor (RGS_EINVAL with a message if its tiles

@@ -139,6 +139,7 @@ class _Slab:
         d.dphase_global = int(self.tuning.get('dphase_global', 0))
         d.top_bits = int(self.tuning.get('top_bits', 0))
         d.group_order = int(self.tuning.get('group_order', 0))
+        d.flip = gates.flip if gates is not None else 0
         # explicit strided groups, low bits up: (width, lane bits, LDS round bits)
         groups = self.tuning.get('groups') or ()
         if len(groups) > 8:
@@ -177,12 +178,14 @@ class _Slab:
     def root(self):
         return float(self.pyramid[-1].item())
 
-    def draw_many(self, uniforms):
+    def draw_many(self, uniforms, flip=0):
+        """Device draws (the state holds outcome x at index x ^ flip, the
+        pyramid is in outcome order): (outcomes, p)."""
         torch = self.torch
         u = torch.as_tensor(np.ascontiguousarray(uniforms, np.float64)).to(self.device)
         idx = torch.empty(len(u), dtype=torch.int64, device=self.device)
         p = torch.empty(len(u), dtype=torch.float64, device=self.device)
-        call('rgs_draw', ptr(self.state), self.nbits, ptr(self.pyramid), self.mass_bits, ptr(u), len(u),
+        call('rgs_draw', ptr(self.state), self.nbits, ptr(self.pyramid), self.mass_bits, ptr(u), len(u), int(flip),
              ptr(idx), ptr(p), self.stream.cuda_stream)
         return idx.cpu().numpy(), p.cpu().numpy()
 
@@ -203,29 +206,40 @@ def ctypes_ref(d):
 
 
 class Gates:
-    """Per-qubit unitaries u_q = D1 R D2 (rgs_gate_zyz): device rotations
-    (cos, sin), the device chunk table of the D2 layer (qubits < gate_limit)
-    and the host D1 phases that read-back applies."""
+    """Per-qubit unitaries u_q = X^f D1 m M(t) D2 (rgs_gate_shear): host
+    rotation tangents t (read when passes are enqueued; the pipeline applies
+    the product of the m's once), the device chunk table of the D2 layer
+    (qubits < gate_limit), the host D1 phases that read-back applies, and the
+    flip mask: outcome x is held at state index x ^ flip."""
 
     def __init__(self, slab, mats, gate_limit=None):
+        import ctypes
         u = np.ascontiguousarray(np.asarray(mats, np.complex128))
         n = len(u)
         self.gate_limit = n if gate_limit is None else gate_limit
-        rot = np.zeros(2 * n, np.float32)
+        tan = np.zeros(n, np.float32)
         dph = np.zeros(((n + 7) // 8) * 256 * 2, np.float32)
         d1 = np.zeros(4 * n, np.float64)
-        call('rgs_gate_zyz', ptr(u), n, self.gate_limit, ptr(rot), ptr(dph), ptr(d1))
-        self.rot = rot                      # host: read when passes are enqueued
+        flip = ctypes.c_uint64()
+        call('rgs_gate_shear', ptr(u), n, self.gate_limit, ptr(tan), ptr(dph), ptr(d1), ctypes.byref(flip))
+        self.rot = tan                      # host: read when passes are enqueued
+        self.flip = int(flip.value)
         self.dphase = slab.to_device(dph)
         self.d1 = d1.view(np.complex128).reshape(n, 2)
 
     def apply_d1(self, v):
-        """In place: multiply amplitude x by prod_{q < gate_limit} D1_q[x_q]."""
+        """In place: multiply amplitude x (state order) by prod_{q < gate_limit} D1_q[x_q]."""
         ids = np.arange(len(v), dtype=np.uint64)
         for q in range(self.gate_limit):
             bit = ((ids >> np.uint64(q)) & np.uint64(1)).astype(np.int64)
             v *= self.d1[q][bit]
         return v
+
+    def to_outcomes(self, v):
+        """A state-ordered vector in outcome order: out[x] = v[x ^ flip]."""
+        if not self.flip:
+            return v
+        return v[np.arange(len(v), dtype=np.int64) ^ self.flip]
 
 
 def form_rows(g):
@@ -357,29 +371,36 @@ class GraphStateSampler:
             raise ArithmeticError(f'norm check failed: {r}')
         return r
 
+    @property
+    def flip(self):
+        """Outcome x is held at state index x ^ flip (rgs_gate_shear)."""
+        return self._gates.flip
+
     def sample(self, uniforms):
-        """Device qubit-by-qubit draws, one uniform each: (indices, p)."""
-        return self.slab.draw_many(uniforms)
+        """Device qubit-by-qubit draws, one uniform each: (outcomes, p)."""
+        return self.slab.draw_many(uniforms, self.flip)
 
     def draw(self, rng):
-        """One draw consuming one uniform: (index, p, norm)."""
+        """One draw consuming one uniform: (outcome, p, norm)."""
         norm = self.norm()
-        idx, p = self.slab.draw_many([rng.random()])
+        idx, p = self.slab.draw_many([rng.random()], self.flip)
         return int(idx[0]), float(p[0]), norm
 
     def probability_at(self, index):
         if not 0 <= index < (1 << self.n):
             raise ValueError('index outside the state')
-        return float(self.slab.gather_prob([index])[0])
+        return float(self.slab.gather_prob([index ^ self.flip])[0])
 
     def device_amplitudes(self):
-        """The device complex64 values before the D1 phases, as complex128:
-        the numbers the masses, the pyramid and the draws are computed from."""
-        return self.slab.state.cpu().numpy().astype(np.complex128)
+        """The device complex64 values before the D1 phases, as complex128, in
+        outcome order: the numbers the masses, the pyramid and the draws are
+        computed from."""
+        return self._gates.to_outcomes(self.slab.state.cpu().numpy().astype(np.complex128))
 
     def host(self):
-        """The state as complex128 (device complex64 values times the D1 phases)."""
-        return self._gates.apply_d1(self.device_amplitudes())
+        """The state as complex128 in outcome order (device complex64 values
+        times the D1 phases)."""
+        return self._gates.to_outcomes(self._gates.apply_d1(self.slab.state.cpu().numpy().astype(np.complex128)))
 
     def sync(self):
         self.slab.sync()
@@ -511,12 +532,29 @@ class ConditionalSampler:
     def _slice(self, start, stop):
         return self.slab.state[start:stop].cpu().numpy()
 
+    def _outcome_slice(self, start, stop):
+        """Outcomes start..stop-1 (an aligned power-of-two block) of the device
+        state: the block of state indices x ^ flip, reordered."""
+        flip = self._gates.flip
+        if not flip:
+            return self._slice(start, stop)
+        size = stop - start
+        lo = flip & (size - 1)
+        s0 = start ^ (flip & ~(size - 1))
+        blk = self._slice(s0, s0 + size)
+        return blk[np.arange(size, dtype=np.int64) ^ lo]
+
+    def _pair(self, local):
+        """The terminal gate applied to the two amplitudes at state index local."""
+        raw = np.array([complex(self._slice(local, local + 1)[0]),
+                        complex(self._slice(self.n + local, self.n + local + 1)[0])], np.complex128)
+        return self.deferred_global_unitary @ raw
+
     def transformed_pair(self, local_index):
+        """The two amplitudes of local outcome local_index after the terminal gate."""
         if not 0 <= local_index < self.n:
             raise ValueError('Local index outside shard')
-        raw = np.array([complex(self._slice(local_index, local_index + 1)[0]),
-                        complex(self._slice(self.n + local_index, self.n + local_index + 1)[0])], np.complex128)
-        return self.deferred_global_unitary @ raw
+        return self._pair(local_index ^ self._gates.flip)
 
     def draw(self, rng):
         """Reference draw: tile by searchsorted over FP64 tile masses, index in
@@ -529,7 +567,7 @@ class ConditionalSampler:
         block = int(np.searchsorted(cm, t1, side='right'))
         half, chunk = block // nmass, block % nmass
         start = half * self.n + (chunk << tb)
-        tile = self._slice(start, min(start + (1 << tb), (half + 1) * self.n))
+        tile = self._outcome_slice(start, min(start + (1 << tb), (half + 1) * self.n))
         p = tile.real.astype(np.float64) ** 2 + tile.imag.astype(np.float64) ** 2
         cp = np.cumsum(p)
         t2 = rng.random() * p.sum()
@@ -560,13 +598,14 @@ class ConditionalSampler:
         the terminal gate in complex128 and one more uniform picks the top bit
         (DeferredGlobalMixin.draw, deferred_global_sampler.py:72-78).
         Returns (indices, unnormalised p)."""
-        idx, p = self.slab.draw_many(uniforms)
+        flip = self._gates.flip
+        idx, p = self.slab.draw_many(uniforms, flip)
         if not self.defer_top:
             return idx, p
         if bit_uniforms is None or len(bit_uniforms) != len(idx):
             raise ValueError('one bit uniform per draw is needed with the deferred top gate')
         torch = self.slab.torch
-        local = torch.as_tensor(idx % self.n).to(self.slab.device)
+        local = torch.as_tensor((idx % self.n) ^ flip).to(self.slab.device)     # state indices
         st = self.slab.state
         lo = st.index_select(0, local).cpu().numpy().astype(np.complex128)
         hi = st.index_select(0, local + self.n).cpu().numpy().astype(np.complex128)
@@ -583,9 +622,10 @@ class ConditionalSampler:
             if bit not in (0, 1):
                 raise ValueError('Bad global index')
             return float(abs(self.transformed_pair(index % self.n)[bit]) ** 2)
-        return float(self.slab.gather_prob([index])[0])
+        return float(self.slab.gather_prob([index ^ self._gates.flip])[0])
 
     def host(self):
+        """The conditional state as complex128 in outcome order."""
         raw = self._gates.apply_d1(self.slab.state.cpu().numpy().astype(np.complex128))
         if self.defer_top:
             blk = raw.reshape(2, -1)
@@ -593,7 +633,7 @@ class ConditionalSampler:
             U = self.deferred_global_unitary
             blk[0] = U[0, 0] * a + U[0, 1] * b
             blk[1] = U[1, 0] * a + U[1, 1] * b
-        return raw
+        return self._gates.to_outcomes(raw)
 
     def sync(self):
         self.slab.sync()

@@ -262,42 +262,49 @@ def test_explicit_group_schedules():
         describe(20, walsh=True, low=6, groups=[(7, 7, 0), (7, 6, 0)])   # 7 lane bits, 6 bits below the group
 
 
-def test_gate_factorisation_and_unit_rounding(golden_dir):
-    """u = D1 R D2 with R real: reconstructed from the library's outputs to
-    float32 accuracy.  Rounding cos and sin separately misses |(c, s)| = 1 by
-    up to 6e-8, which identical gates on every qubit would compound: the
-    library dithers the rounding so the running product of rotation norms
-    stays within one rounding of 1, and picks phase-table entries of smallest
-    norm error."""
+def test_gate_shear_factorisation(golden_dir):
+    """u = X^f D1 m M(t) D2 with M(t) = [[1, -t], [t, 1]], |t| <= 1 and
+    m = (1 + t^2)^(-1/2): reconstructed from the library's outputs to float32
+    accuracy for the 70-qubit instance's unitary, Haar unitaries, and the
+    diagonal and anti-diagonal corner cases; the flips are exactly the qubits
+    below gate_limit whose rotation is past 45 degrees; the common scale is
+    the product of the m's; phase-table entries have the smallest norm error
+    of their one-ulp neighbourhood."""
+    import ctypes
     d = np.load(f'{golden_dir}/rgs_graph.npz')
-    mats = np.concatenate([d['measurement_unitaries'][:4], instance(12, 5)[1]])
+    diag = np.diag([np.exp(0.3j), np.exp(-1.1j)])
+    anti = np.array([[0, -np.exp(0.7j)], [np.exp(0.2j), 0]])
+    mats = np.concatenate([d['measurement_unitaries'][:4], instance(12, 5)[1], [diag, anti]])
     n = len(mats)
     u = np.ascontiguousarray(mats.astype(np.complex128))
-    rot = np.zeros(2 * n, np.float32)
-    dph = np.zeros(((n + 7) // 8) * 256 * 2, np.float32)
-    d1 = np.zeros(4 * n, np.float64)
-    _lib.call('rgs_gate_zyz', _lib.ptr(u), n, n, _lib.ptr(rot), _lib.ptr(dph), _lib.ptr(d1))
-    d1 = d1.view(np.complex128).reshape(n, 2)
-    rot = rot.reshape(n, 2).astype(np.float64)
-    dph = dph.reshape(-1, 2).astype(np.float64)
-    eps = rot[:, 0] ** 2 + rot[:, 1] ** 2 - 1
-    assert np.max(np.abs(eps)) < 2.5e-7                        # within an ulp of the pair
-    assert np.max(np.abs(np.cumsum(eps))) < 1.2e-7             # no drift of the product
-    deps = dph[:, 0] ** 2 + dph[:, 1] ** 2 - 1
+    X = np.array([[0, 1], [1, 0]])
+    for limit in (n, n - 3):
+        tan = np.zeros(n, np.float32)
+        dph = np.zeros(((n + 7) // 8) * 256 * 2, np.float32)
+        d1 = np.zeros(4 * n, np.float64)
+        flip = ctypes.c_uint64()
+        _lib.call('rgs_gate_shear', _lib.ptr(u), n, limit, _lib.ptr(tan), _lib.ptr(dph), _lib.ptr(d1),
+                  ctypes.byref(flip))
+        d1c = d1.view(np.complex128).reshape(n, 2)
+        dphc = dph.reshape(-1, 2).astype(np.float64) @ [1, 1j]
+        assert np.all(np.abs(tan[:limit]) <= 1)
+        for q in range(n):
+            c, s_ = abs(u[q, 0, 0]), abs(u[q, 1, 0])
+            f = (flip.value >> q) & 1
+            assert f == (q < limit and s_ > c), q
+            if q >= limit:
+                continue
+            t = float(tan[q])
+            e_b = dphc[(q // 8) * 256 + (1 << (q % 8))]            # single-qubit entry of the D2 table
+            rec = np.diag(d1c[q]) @ (np.array([[1, -t], [t, 1]]) / np.sqrt(1 + t * t)) @ np.diag([1.0, e_b])
+            if f:
+                rec = X @ rec
+            assert np.max(np.abs(rec - u[q])) < 3e-7, q
+        assert flip.value >> limit == 0
+        scale = _lib.lib().rgs_gate_scale(_lib.ptr(tan), limit)
+        assert scale == pytest.approx(np.prod(1 / np.sqrt(1 + tan[:limit].astype(np.float64) ** 2)), rel=1e-14)
+    deps = np.abs(dphc) ** 2 - 1
     assert np.max(np.abs(deps)) < 1.2e-7 and abs(np.mean(deps)) < 1e-8
-    # 70 identical gates (the 70-qubit instance): the product of norms stays at 1
-    same = np.ascontiguousarray(np.repeat(d['measurement_unitaries'][:1], 63, axis=0).astype(np.complex128))
-    r2 = np.zeros(2 * 63, np.float32)
-    _lib.call('rgs_gate_zyz', _lib.ptr(same), 63, 63, _lib.ptr(r2), None, None)
-    r2 = r2.reshape(63, 2).astype(np.float64)
-    assert abs(np.prod(r2[:, 0] ** 2 + r2[:, 1] ** 2) - 1) < 1.2e-7
-    dphc = dph[:, 0] + 1j * dph[:, 1]
-    for q in range(n):
-        c, s = rot[q]
-        R = np.array([[c, -s], [s, c]])
-        e_b = dphc[(q // 8) * 256 + (1 << (q % 8))]            # single-qubit entry of the D2 table
-        rec = np.diag(d1[q]) @ R @ np.diag([1.0, e_b])
-        assert np.max(np.abs(rec - u[q])) < 3e-7, q
 
 
 def test_tuning_table_key_and_entries():

@@ -104,7 +104,7 @@ def test_headline_kernels_at_n27(pipeline):
     # amplitudes and p at drawn and uniform indices against the complex128 scan
     probe = np.concatenate([idx, rng.integers(0, 1 << n, 3000)]).astype(np.int64)
     ref = _oracle_amplitudes(g, u, probe, n)
-    got = raw[probe] * _d1_at(s._gates, probe)
+    got = raw[probe] * _d1_at(s._gates, probe ^ s.flip)     # D1 acts on the state index
     amp_err = np.abs(got - ref)
     assert np.all(amp_err <= 1e-5 * np.maximum(np.abs(ref), np.sqrt(0.1 * 2.0 ** -n))), amp_err.max()
     _check_p(od.probabilities(raw[probe]), np.abs(ref) ** 2, n)
@@ -141,10 +141,11 @@ def test_headline_n34_full_size():
         assert np.array_equal(levels[k], levels[k - 1][0::2] + levels[k - 1][1::2]), k
     mb = sl.mass_bits
     state = sl.state
+    flip = s.flip                     # outcome x is held at state index x ^ flip
 
     def fetch(blocks):
         b = torch.as_tensor(np.asarray(blocks, np.int64)).to(sl.device)
-        ix = (b[:, None] << mb) + torch.arange(1 << mb, device=sl.device)[None, :]
+        ix = ((b[:, None] << mb) + torch.arange(1 << mb, device=sl.device)[None, :]) ^ flip
         vals = state[ix.reshape(-1)].cpu().numpy()
         return od.probabilities(vals).reshape(len(blocks), 1 << mb)
 
@@ -160,11 +161,11 @@ def test_headline_n34_full_size():
     assert np.all((idx >= 0) & (idx < (1 << n)))
     host = od.split_draws(levels, fetch, mb, uni)
     assert np.array_equal(idx, host)
-    assert np.array_equal(p, s.slab.gather_prob(idx))
+    assert np.array_equal(p, s.slab.gather_prob(idx ^ flip))
     # p(x) at drawn and uniform indices against the complex128 rank scan
     probe = np.concatenate([idx[:512], rng.integers(0, 1 << n, 512)]).astype(np.int64)
     ref = _oracle_amplitudes(g, u, probe, n)
-    _check_p(s.slab.gather_prob(probe), np.abs(ref) ** 2, n)
+    _check_p(s.slab.gather_prob(probe ^ flip), np.abs(ref) ** 2, n)
     # size-independent draw statistic: E_draw[2^n p(x)] = 2^n sum p^2 =
     # E_uniform[(2^n p)^2], estimated from the draws and from 2^20 uniform
     # indices independently; they must agree within 5 standard errors

@@ -172,7 +172,7 @@ def test_walsh_and_gates_on_arbitrary_input(tun):
         sl.sync()
         got = sl.state.cpu().numpy().astype(np.complex128)
         if use_gates:
-            gates.apply_d1(got)
+            got = gates.to_outcomes(gates.apply_d1(got))
         ref = v.astype(np.complex128)
         if walsh:
             gs.walsh(ref)
