@@ -377,8 +377,44 @@ __global__ void __launch_bounds__(64 * W) k_scan_batch(const uint64_t* __restric
       // renormalise on every kRenormSteps-th step, on small steps and on the
       // last (see kRenormSteps)
       const bool renorm = r_out < 8 || j % kRenormSteps == kRenormSteps - 1 || j == n - 1;
-      S outv[P][K];
-      uint32_t mxh[P];
+      // The end of a step: the waves' maxima, the barrier that ends every
+      // read of the tables, then the outputs written back in place.  Each
+      // kind of step below calls it with its own outputs, so the compiler
+      // does not merge the output registers of the different kinds (a
+      // register copy per output at every step).  Lanes past 2^r_out (steps
+      // of rank below six) computed in-range values that are neither counted
+      // nor stored.
+      auto finish = [&](auto& outv, auto& mxh, auto track) {
+        constexpr bool T = decltype(track)::value;
+        if (T) {
+#pragma unroll
+          for (int q = 0; q < P; ++q) {
+            mxh[q] = wave_max_u32(lane_ok ? mxh[q] : 0u);
+            if (W > 1 && lane == 0) wmx[q * W + wave] = mxh[q];
+          }
+        }
+        __syncthreads();  // every read of the tables done; the waves' maxima visible
+#pragma unroll
+        for (int q = 0; q < P; ++q) {
+          int e = 0;
+          if (T) {
+            if (W > 1) {
+#pragma unroll
+              for (int w2 = 0; w2 < W; ++w2) mxh[q] = max(mxh[q], wmx[q * W + w2]);
+            }
+            e = scale_exponent(mxh[q]);
+          }
+          if (lane_ok) {
+#pragma unroll
+            for (int kk = 0; kk < K; ++kk) {
+              if (kk < nkw) E[q * TN + lane + 64 * (wave + W * (kk ^ (kk >> 1)))] = outv[q][kk];
+            }
+          }
+          esum[q] += e;
+          e_prev[q] = e;
+        }
+        __syncthreads();
+      };
       // the packed plan has nker <= 2, and split steps nker >= 1
       if ((hdr >> 33) & 1u) {
         uint32_t hstep[KB > 0 ? KB : 1];
@@ -387,31 +423,45 @@ __global__ void __launch_bounds__(64 * W) k_scan_batch(const uint64_t* __restric
         uint32_t lm8 = (((uint32_t)st[2 + lane] & a.mask) << SH) + lds_base;
         asm volatile("" : "+v"(lm8));   // keep the shifted word: terms then need one XOR each
         const uint32_t hw = wave < nk ? (uint32_t)st[66 + wave] : 0u;
-        auto run = [&](auto track) {
+        auto run = [&](auto nker_c, auto split_c, auto track) {
+          constexpr int NK = decltype(nker_c)::value;
+          constexpr bool SP = decltype(split_c)::value;
           constexpr bool T = decltype(track)::value;
+          S outv[P][K];
+          uint32_t mxh[P];
+          scan_outputs_u<S, K, NK, SP, P, TBYTES, T>(a, f0, f1, combos, lm8, hw, hstep, nkw, outv, mxh);
+          finish(outv, mxh, track);
+        };
+        using I0 = std::integral_constant<int, 0>;
+        using I1 = std::integral_constant<int, 1>;
+        using I2 = std::integral_constant<int, 2>;
+        auto by_kind = [&](auto track) {
           if (split) {
             if (a.nker == 1)
-              scan_outputs_u<S, K, 1, true, P, TBYTES, T>(a, f0, f1, combos, lm8, hw, hstep, nkw, outv, mxh);
+              run(I1(), std::true_type(), track);
             else
-              scan_outputs_u<S, K, 2, true, P, TBYTES, T>(a, f0, f1, combos, lm8, hw, hstep, nkw, outv, mxh);
+              run(I2(), std::true_type(), track);
           } else {
             if (a.nker == 0)
-              scan_outputs_u<S, K, 0, false, P, TBYTES, T>(a, f0, f1, combos, lm8, hw, hstep, nkw, outv, mxh);
+              run(I0(), std::false_type(), track);
             else if (a.nker == 1)
-              scan_outputs_u<S, K, 1, false, P, TBYTES, T>(a, f0, f1, combos, lm8, hw, hstep, nkw, outv, mxh);
+              run(I1(), std::false_type(), track);
             else
-              scan_outputs_u<S, K, 2, false, P, TBYTES, T>(a, f0, f1, combos, lm8, hw, hstep, nkw, outv, mxh);
+              run(I2(), std::false_type(), track);
           }
         };
         if (renorm)
-          run(std::true_type());
+          by_kind(std::true_type());
         else
-          run(std::false_type());
+          by_kind(std::false_type());
       } else {
-        // steps without the uniform flag have r_out < 8: at most two blocks,
-        // so at most one per wave (nkw <= 1); each proposal in turn
+        // steps without the uniform flag have r_out < 8 (always renormalised):
+        // at most two blocks, so at most one per wave (nkw <= 1); each
+        // proposal in turn
         const uint32_t lo_w = (uint32_t)st[2 + lane] ^ (wave < nk ? (uint32_t)st[66 + wave] : 0u);
         const uint32_t hstep1[1] = {0u};
+        S outv[P][K];
+        uint32_t mxh[P];
 #pragma unroll
         for (int q = 0; q < P; ++q) {
           ScanStepArgs aq = a;
@@ -428,37 +478,8 @@ __global__ void __launch_bounds__(64 * W) k_scan_batch(const uint64_t* __restric
                                    : scan_outputs<S, 1, 2, false>(Eq, aq, combos, lo_w, hstep1, nkw, o1);
           }
         }
+        finish(outv, mxh, std::true_type());
       }
-      // lanes past 2^r_out (steps of rank below six) computed in-range values
-      // that are neither counted nor stored
-      if (renorm) {
-#pragma unroll
-        for (int q = 0; q < P; ++q) {
-          mxh[q] = wave_max_u32(lane_ok ? mxh[q] : 0u);
-          if (W > 1 && lane == 0) wmx[q * W + wave] = mxh[q];
-        }
-      }
-      __syncthreads();  // every read of the tables done; the waves' maxima visible
-#pragma unroll
-      for (int q = 0; q < P; ++q) {
-        int e = 0;
-        if (renorm) {
-          if (W > 1) {
-#pragma unroll
-            for (int w2 = 0; w2 < W; ++w2) mxh[q] = max(mxh[q], wmx[q * W + w2]);
-          }
-          e = scale_exponent(mxh[q]);
-        }
-        if (lane_ok) {
-#pragma unroll
-          for (int kk = 0; kk < K; ++kk) {
-            if (kk < nkw) E[q * TN + lane + 64 * (wave + W * (kk ^ (kk >> 1)))] = outv[q][kk];
-          }
-        }
-        esum[q] += e;
-        e_prev[q] = e;
-      }
-      __syncthreads();
     }
     if (threadIdx.x < P && pb + (int)threadIdx.x < count) {
       const int q = threadIdx.x;
