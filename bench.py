@@ -20,6 +20,10 @@ Reported beside the headline value:
                 summary, when present.
   kernels       the same figures for every specialisation in the plan.
   passes        per-pass ms and GB/s; `walsh_gbps` over the Walsh passes.
+  deferred      the same step with the last pass deferred: it writes the
+                FP64 masses but not the state, and each draw recomputes the
+                tile holding its leaf block; samples/s, the last pass's ms and
+                read GB/s, and whether the draws equal the stored-pass draws.
   folded        the same state with H^n|0> folded into the init (no Walsh
                 passes), samples/s.
   scan          batched rank-scan p(x) of 65,536 device-resident uniform 70-bit
@@ -158,9 +162,65 @@ def run_headline(args, torch):
     drawn = idx[:16].cpu().numpy()
     assert np.all((drawn >= 0) & (drawn < (1 << n)))
     res = dict(s=s, t=t1 - t0, norm=norm, per_pass=per_pass, wht_gbps=wht_gbps, kernels=kernels,
+               last_uniforms=uni[total - 1], last_index=idx.cpu().numpy(), last_p=pdr.cpu().numpy(),
                dominant=dominant, passes_gbps=sum(nbytes) / sum(pass_ms[:npass]) / 1e6,
                pyramid_ms=pass_ms[npass], draw_ms=pass_ms[npass + 1])
     return res
+
+
+def run_deferred(s, res, args, torch):
+    """The headline step with its last pass deferred: that pass writes the
+    FP64 masses only and each draw recomputes the tile holding its leaf block
+    (rgs_draw_leaves).  The last timed step's draws must equal the stored-pass
+    draws of the same uniforms bit for bit."""
+    from rgs70_classical_challenge_amd import _lib
+    from rgs70_classical_challenge_amd.engine import ctypes_ref
+    import ctypes
+    sl = s.slab
+    s.defer_leaves = True
+    d = s._make_desc()
+    passes, text = sl.describe(d)
+    sp = sl.stream.cuda_stream
+    nb = ctypes.c_size_t()
+    _lib.call('rgs_draw_leaves_workspace', ctypes_ref(d), args.samples, ctypes.byref(nb))
+    work = torch.empty(nb.value, dtype=torch.uint8, device=sl.device)
+    uni = res['last_uniforms']
+    idx = torch.empty(args.samples, dtype=torch.int64, device=sl.device)
+    pdr = torch.empty(args.samples, dtype=torch.float64, device=sl.device)
+    ev = [(torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)) for _ in range(3)]
+    times = []
+
+    def step():
+        for p in range(passes - 1):
+            _lib.call('rgs_slab_pass', ctypes_ref(d), p, sp)
+        ev[0][0].record(sl.stream)
+        _lib.call('rgs_slab_pass', ctypes_ref(d), passes - 1, sp)
+        ev[0][1].record(sl.stream)
+        ev[1][0].record(sl.stream)
+        _lib.call('rgs_mass_pyramid', _lib.ptr(sl.pyramid), sl.base_bits, sp)
+        ev[1][1].record(sl.stream)
+        ev[2][0].record(sl.stream)
+        _lib.call('rgs_draw_leaves', ctypes_ref(d), _lib.ptr(sl.pyramid), _lib.ptr(uni), args.samples,
+                  _lib.ptr(work), nb.value, _lib.ptr(idx), _lib.ptr(pdr), sp)
+        ev[2][1].record(sl.stream)
+
+    step()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        step()
+        torch.cuda.synchronize()
+        times.append([a.elapsed_time(b) for a, b in ev])
+    dt = (time.perf_counter() - t0) / args.steps
+    same = bool(np.array_equal(idx.cpu().numpy(), res['last_index'])
+                and np.array_equal(pdr.cpu().numpy(), res['last_p']))
+    ms = np.mean(times, axis=0)
+    s.defer_leaves = False
+    s._desc = s._make_desc()
+    return {'passes': passes, 'plan': text, 'ms_per_step': round(dt * 1e3, 2),
+            'samples_per_s': round(args.samples / dt, 1), 'last_pass_ms': round(float(ms[0]), 3),
+            'last_pass_gbps': round((8 << s.n) / ms[0] / 1e6, 1), 'pyramid_ms': round(float(ms[1]), 3),
+            'draw_ms': round(float(ms[2]), 3), 'draws_equal_stored': same, 'norm': sl.root()}
 
 
 def run_folded(s, args, torch):
@@ -451,6 +511,11 @@ def main():
         if t.get('plan') == s.plan_text and dom in t.get('kernels', {}):
             out['roofline']['traffic'] = t['kernels'][dom]['hbm_bytes_per_launch']
     if not args.no_extras:
+        try:
+            out['deferred'] = run_deferred(s, res, args, torch)
+        except Exception as e:  # report, do not hide the headline
+            out['deferred'] = {'error': repr(e)}
+        log('[bench] deferred done')
         try:
             out['folded'] = run_folded(s, args, torch)
         except Exception as e:  # report, do not hide the headline

@@ -21,6 +21,7 @@
 #define RGS_AMD_H
 
 #include <stdint.h>
+#include <stddef.h>
 
 #ifdef __cplusplus
 extern "C" {
@@ -32,7 +33,7 @@ extern "C" {
 #define RGS_ENOMEM 3 /* MemoryError                          */
 #define RGS_EHIP 4   /* RuntimeError (HIP runtime failure)   */
 
-#define RGS_ABI_VERSION 2
+#define RGS_ABI_VERSION 3
 
 /* Thread-local message of the last failure ("" if none). */
 const char *rgs_last_error(void);
@@ -187,6 +188,14 @@ typedef struct rgs_slab_desc {
                                   (init_wbits); 1: it holds the (nbits+1)^2
                                   popcount table of rgs_cov_weight_table,
                                   read at [popc(b)][popc(b & init_omask)]    */
+  int defer_leaves;            /* 1: the last pass (contiguous, loading the
+                                  state, with mass != NULL) writes the mass
+                                  level but not the state, which keeps the
+                                  values before that pass; draws and reads
+                                  recompute the tiles they touch
+                                  (rgs_slab_leaf_tiles, rgs_draw_leaves) and
+                                  rgs_slab_finish writes the whole state.
+                                  0: every pass writes the state             */
 } rgs_slab_desc;
 
 /* Factor 2x2 unitaries u (complex128 [n][2][2]) as
@@ -224,6 +233,27 @@ int rgs_slab_pipeline(const rgs_slab_desc *d, void *stream);
  * No reference counterpart beyond the per-stage timers of compute
  * (work/rgs_attack/distributed_sampler.py:127-143). */
 int rgs_slab_pass(const rgs_slab_desc *d, int pass, void *stream);
+
+/* Deferred leaves (defer_leaves = 1).  The reference defers its terminal
+ * gate to the draw (DeferredGlobalMixin, work/rgs_attack/deferred_global_sampler.py:41-78:
+ * draw from the pre-gate masses, transform the two amplitudes drawn);
+ * here the whole last contiguous pass is deferred: its kernel computes the
+ * masses from the transformed values without storing them, and the values
+ * of a tile are recomputed when asked for by the same kernel code, so they
+ * are the bits the pass would have stored.
+ *
+ * rgs_slab_leaf_bits: the tile bits T of the deferred pass (tile t holds
+ * state indices t * 2^T .. (t + 1) * 2^T - 1).
+ * rgs_slab_leaf_tiles: the final values of `count` tiles (dev int64 tile
+ * indices; an index >= 2^(nbits - T) leaves its slot unwritten) into out,
+ * dev complex64 [count][2^T] in state order; count <= 2^24.
+ * rgs_slab_finish: write the final values of every tile into the state
+ * (after which the state equals the one rgs_slab_pipeline writes with
+ * defer_leaves = 0; run it once per pipeline run).
+ * Each needs the descriptor the pipeline ran with and the state it left. */
+int rgs_slab_leaf_bits(const rgs_slab_desc *d, int *bits);
+int rgs_slab_leaf_tiles(const rgs_slab_desc *d, const int64_t *tiles, int count, void *out, void *stream);
+int rgs_slab_finish(const rgs_slab_desc *d, void *stream);
 
 /* Number of HBM passes and their text description ("<kind>:<first bit>+<count>
  * ..."), for logs and the autotuner. */
@@ -279,6 +309,17 @@ int rgs_mass_pyramid(double *pyr, int base_bits, void *stream);
 int rgs_draw(const void *state, int nbits, const double *pyr, int mass_bits,
              const double *uniforms, int count, uint64_t flip, int64_t *out_index, double *out_p,
              void *stream);
+
+/* rgs_draw on a state left by a pipeline with defer_leaves = 1: the pyramid
+ * descent (one thread per draw), the recomputation of the tile holding each
+ * draw's leaf block (rgs_slab_leaf_tiles), then the leaf-block draw; the
+ * outcomes and p are those rgs_draw gives on the finished state, bit for
+ * bit.  nbits, mass_bits and flip come from the descriptor.  work: dev
+ * memory of at least rgs_draw_leaves_workspace(d, count) bytes (256-byte
+ * aligned). */
+int rgs_draw_leaves_workspace(const rgs_slab_desc *d, int count, size_t *bytes);
+int rgs_draw_leaves(const rgs_slab_desc *d, const double *pyr, const double *uniforms, int count, void *work,
+                    size_t work_bytes, int64_t *out_index, double *out_p, void *stream);
 
 /* float64 p = re^2 + im^2 at `count` indices (probability_at,
  * distributed_sampler.py:158-161). */

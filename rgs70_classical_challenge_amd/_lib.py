@@ -57,6 +57,7 @@ class SlabDesc(ctypes.Structure):
         ('flip', ctypes.c_uint64),
         ('init_omask', ctypes.c_uint64),
         ('init_wpop', c_int),
+        ('defer_leaves', c_int),
     ]
 
 
@@ -91,6 +92,12 @@ SIGNATURES = {
     'rgs_draw': (c_int, [c_void_p, c_int, c_void_p, c_int, c_void_p, c_int, ctypes.c_uint64, c_void_p, c_void_p,
                          c_void_p]),
     'rgs_gather_prob': (c_int, [c_void_p, c_void_p, c_int, c_void_p, c_void_p]),
+    'rgs_slab_leaf_bits': (c_int, [P(SlabDesc), P(c_int)]),
+    'rgs_slab_leaf_tiles': (c_int, [P(SlabDesc), c_void_p, c_int, c_void_p, c_void_p]),
+    'rgs_slab_finish': (c_int, [P(SlabDesc), c_void_p]),
+    'rgs_draw_leaves_workspace': (c_int, [P(SlabDesc), c_int, P(ctypes.c_size_t)]),
+    'rgs_draw_leaves': (c_int, [P(SlabDesc), c_void_p, c_void_p, c_int, c_void_p, ctypes.c_size_t, c_void_p,
+                                c_void_p, c_void_p]),
     'rgs_slab_batch_create': (c_int, [c_void_p, c_int, P(c_void_p)]),
     'rgs_slab_batch_run': (c_int, [c_void_p, c_void_p]),
     'rgs_slab_batch_info': (c_int, [c_void_p, P(c_int), P(c_int)]),

@@ -89,6 +89,7 @@ extern "C" int rgs_slab_batch_create(const rgs_slab_batch_item* items, int count
     if (!items[i].uniforms || !items[i].out_index || !items[i].out_p || items[i].samples < 0)
       return fail(kInvalid, "slab_batch: draw buffers missing");
     if (d->mass_bits < 2 || d->mass_bits > 8) return fail(kInvalid, "slab_batch: draws need 2 <= mass_bits <= 8");
+    if (d->defer_leaves) return fail(kInvalid, "slab_batch: items draw from finished states (defer_leaves = 0)");
     std::string err;
     int rc = plan_passes(d, plans[i], err);
     if (!rc) rc = check_desc(d, err);

@@ -2,6 +2,7 @@
 #include "rgs_common.h"
 #include "rgs_amd.h"
 #include "multi.h"
+#include "slab_plan.h"
 
 #include <algorithm>
 
@@ -66,11 +67,11 @@ __device__ __forceinline__ uint64_t level_off(int B, int l) {
   return (1ull << (B + 1)) - (1ull << (B + 1 - l));
 }
 
-__device__ __forceinline__ void draw_one(const float2* state, int nbits, const double* pyr, int mass_bits, double u,
-                                         uint64_t flip, int64_t* out_index, double* out_p) {
-  const int lane = threadIdx.x;
-  const int B = nbits - mass_bits;
-  double t = u * pyr[level_off(B, B)];
+// Pyramid descent from the root to a leaf block: t = u * root, then at each
+// level go to child 1 iff t >= mass(child 0) and mass(child 1) > 0,
+// subtracting mass(child 0).  Returns the block; t is what is left of it.
+__device__ __forceinline__ uint64_t descend(const double* pyr, int B, double u, double& t) {
+  t = __dmul_rn(u, pyr[level_off(B, B)]);
   uint64_t idx = 0;
   for (int l = B - 1; l >= 0; --l) {
     const double* lev = pyr + level_off(B, l);
@@ -82,6 +83,16 @@ __device__ __forceinline__ void draw_one(const float2* state, int nbits, const d
       idx = 2 * idx;
     }
   }
+  return idx;
+}
+
+// The draw inside leaf block idx (outcomes bbase .. bbase + 2^mass_bits - 1,
+// bbase = idx * 2^mass_bits) with the remainder t, one wavefront: outcome x
+// is read from src[(x ^ flip) & smask] (the state itself, smask = ~0, or a
+// recomputed tile holding it).
+__device__ __forceinline__ void draw_leaf(const float2* src, uint64_t smask, int mass_bits, uint64_t idx, double t,
+                                          uint64_t flip, int64_t* out_index, double* out_p) {
+  const int lane = threadIdx.x;
   // Leaf block: recompute the pairwise tree with the same association.
   const int bsize = 1 << mass_bits;
   const uint64_t bbase = idx << mass_bits;
@@ -89,7 +100,7 @@ __device__ __forceinline__ void draw_one(const float2* state, int nbits, const d
 #pragma unroll
   for (int j = 0; j < 4; ++j) {
     int l = lane * 4 + j;
-    p[j] = l < bsize ? prob64(state[(bbase + l) ^ flip]) : 0.0;
+    p[j] = l < bsize ? prob64(src[((bbase + l) ^ flip) & smask]) : 0.0;
   }
   double lv[7];
   lv[0] = __dadd_rn(__dadd_rn(p[0], p[1]), __dadd_rn(p[2], p[3]));
@@ -127,6 +138,13 @@ __device__ __forceinline__ void draw_one(const float2* state, int nbits, const d
   }
 }
 
+__device__ __forceinline__ void draw_one(const float2* state, int nbits, const double* pyr, int mass_bits, double u,
+                                         uint64_t flip, int64_t* out_index, double* out_p) {
+  double t;
+  const uint64_t idx = descend(pyr, nbits - mass_bits, u, t);
+  draw_leaf(state, ~0ull, mass_bits, idx, t, flip, out_index, out_p);
+}
+
 __global__ void __launch_bounds__(64) k_draw(const float2* state, int nbits, const double* pyr, int mass_bits,
                                              const double* uniforms, int count, uint64_t flip, int64_t* out_index,
                                              double* out_p) {
@@ -152,6 +170,31 @@ hipError_t launch_pyramid_multi(const PyrJob* jobs, const uint32_t* first, int c
 hipError_t launch_draw_multi(const DrawJob* jobs, const uint32_t* first, int count, uint32_t draws, hipStream_t st) {
   hipLaunchKernelGGL(k_draw_multi, dim3(draws), dim3(64), 0, st, jobs, first, count);
   return hipGetLastError();
+}
+
+// Deferred leaves: the descent of every draw (one thread each), recording
+// the block, the remainder and the leaf-bits tile holding the block...
+__global__ void __launch_bounds__(256) k_draw_descend(const double* pyr, int nbits, int mass_bits, int leaf_bits,
+                                                      const double* uniforms, int count, uint64_t flip,
+                                                      int64_t* tiles, int64_t* blocks, double* rest) {
+  const int d = blockIdx.x * 256 + threadIdx.x;
+  if (d >= count) return;
+  double t;
+  const uint64_t idx = descend(pyr, nbits - mass_bits, uniforms[d], t);
+  blocks[d] = (int64_t)idx;
+  rest[d] = t;
+  tiles[d] = (int64_t)(((idx << mass_bits) ^ flip) >> leaf_bits);
+}
+
+// ... then, after the tiles are recomputed (draw d's into slot d), the
+// leaf-block draws.
+__global__ void __launch_bounds__(64) k_draw_leaf(const float2* leaves, int leaf_bits, int mass_bits,
+                                                  const int64_t* blocks, const double* rest, int count,
+                                                  uint64_t flip, int64_t* out_index, double* out_p) {
+  const int d = blockIdx.x;
+  if (d >= count) return;
+  draw_leaf(leaves + ((uint64_t)d << leaf_bits), (1ull << leaf_bits) - 1ull, mass_bits, (uint64_t)blocks[d],
+            rest[d], flip, out_index + d, out_p + d);
 }
 
 __global__ void k_gather_prob(const float2* state, const int64_t* index, int count, double* out) {
@@ -203,5 +246,48 @@ extern "C" int rgs_gather_prob(const void* state, const int64_t* index, int coun
   hipLaunchKernelGGL(k_gather_prob, dim3((count + 255) / 256), dim3(256), 0, st,
                      reinterpret_cast<const float2*>(state), index, count, out_p);
   RGS_LAUNCH_CHECK("k_gather_prob");
+  return kOk;
+}
+
+// Workspace of rgs_draw_leaves: tile, block and remainder per draw, then the
+// recomputed tiles from a 256-byte boundary.
+static size_t leaves_head(int count) { return (((size_t)count * 24) + 255) & ~(size_t)255; }
+
+extern "C" int rgs_draw_leaves_workspace(const rgs_slab_desc* d, int count, size_t* bytes) {
+  if (!bytes || count < 0) return fail(kInvalid, "bad workspace query");
+  int lb = 0;
+  int rc = leaf_bits(d, &lb);
+  if (rc) return rc;
+  *bytes = leaves_head(count) + ((size_t)count << lb) * sizeof(float2);
+  return kOk;
+}
+
+extern "C" int rgs_draw_leaves(const rgs_slab_desc* d, const double* pyr, const double* uniforms, int count,
+                               void* work, size_t work_bytes, int64_t* out_index, double* out_p, void* stream) {
+  if (!pyr || !uniforms || !work || !out_index || !out_p) return fail(kInvalid, "null pointer");
+  if (count < 0 || count > (1 << 24)) return fail(kInvalid, "draw count must be in 0..2^24");
+  int lb = 0;
+  int rc = leaf_bits(d, &lb);
+  if (rc) return rc;
+  const int mb = d->mass_bits;
+  if (mb < 2 || mb > 8 || mb > lb) return fail(kInvalid, "draw needs 2 <= mass_bits <= min(8, leaf bits)");
+  if (count == 0) return kOk;
+  if ((reinterpret_cast<uintptr_t>(work) & 255) != 0) return fail(kInvalid, "workspace must be 256-byte aligned");
+  if (work_bytes < leaves_head(count) + ((size_t)count << lb) * sizeof(float2))
+    return fail(kInvalid, "workspace smaller than rgs_draw_leaves_workspace");
+  hipStream_t st = reinterpret_cast<hipStream_t>(stream);
+  char* w = reinterpret_cast<char*>(work);
+  int64_t* tiles = reinterpret_cast<int64_t*>(w);
+  int64_t* blocks = tiles + count;
+  double* rest = reinterpret_cast<double*>(blocks + count);
+  float2* leaves = reinterpret_cast<float2*>(w + leaves_head(count));
+  hipLaunchKernelGGL(k_draw_descend, dim3((count + 255) / 256), dim3(256), 0, st, pyr, d->nbits, mb, lb, uniforms,
+                     count, d->flip, tiles, blocks, rest);
+  RGS_LAUNCH_CHECK("k_draw_descend");
+  rc = leaf_tiles(d, tiles, count, leaves, st);
+  if (rc) return rc;
+  hipLaunchKernelGGL(k_draw_leaf, dim3(count), dim3(64), 0, st, leaves, lb, mb, blocks, rest, count, d->flip,
+                     out_index, out_p);
+  RGS_LAUNCH_CHECK("k_draw_leaf");
   return kOk;
 }

@@ -162,6 +162,16 @@ int plan_passes(const rgs_slab_desc* d, std::vector<PassPlan>& out, std::string&
     if ((p.flags & ~F_LOAD) || !(p.flags & F_LOAD)) kept.push_back(p);
   out.swap(kept);
   for (size_t i = 0; i < out.size(); ++i) out[i].first = i == 0;
+  if (d->defer_leaves) {
+    // the last pass writes its masses only; it must load the state (the
+    // earlier passes' values stay in place for its tiles to be recomputed)
+    PassPlan& lp = out.back();
+    if (out.size() < 2 || !lp.low || !(lp.flags & F_LOAD) || !(lp.flags & F_MASS)) {
+      err = "defer_leaves needs a mass level and a last contiguous pass that loads the state";
+      return kInvalid;
+    }
+    lp.flags |= F_NOSTORE;
+  }
   // the strided pass that applies the phase tables holds them in LDS besides
   // the tile: it may take its own number of exchange rounds
   if (d->phase_round_bits >= 0)
@@ -349,9 +359,76 @@ static int run_passes(const rgs_slab_desc* d, int first, int last, void* stream)
   return kOk;
 }
 
+// The deferred last pass of d's schedule, its arguments with the masses off
+// (the tile values only), and its LDS bytes.
+static int leaf_pass(const rgs_slab_desc* d, PassPlan& p, PassArgs& a, size_t& shmem) {
+  if (!d || !d->state) return fail(kInvalid, "null descriptor or state");
+  if (!d->defer_leaves) return fail(kInvalid, "the descriptor does not defer its leaves");
+  std::vector<PassPlan> plan;
+  std::string err;
+  int rc = plan_passes(d, plan, err);
+  if (rc) return fail(rc, err);
+  rc = check_desc(d, err);
+  if (rc) return fail(rc, err);
+  p = plan.back();
+  build_args(d, p, a);
+  a.flags &= ~(F_MASS | F_NOSTORE);
+  shmem = lds_bytes(p, d->nbits, d);
+  return kOk;
+}
+
+int leaf_tiles(const rgs_slab_desc* d, const int64_t* tiles, int count, void* out, hipStream_t st) {
+  if (!tiles || !out) return fail(kInvalid, "null tile list or output");
+  if (count < 0 || count > (1 << 24)) return fail(kInvalid, "leaf tile count must be in 0..2^24");
+  PassPlan p;
+  PassArgs a;
+  size_t shmem;
+  int rc = leaf_pass(d, p, a, shmem);
+  if (rc) return rc;
+  if (count == 0) return kOk;
+  hipError_t e = launch_c0_leaf(p.tb, p.rb, a, tiles, count, reinterpret_cast<float2*>(out), shmem, st);
+  if (e != hipSuccess) return fail(kHip, std::string("k_tile_leaf launch: ") + hipGetErrorString(e));
+  return kOk;
+}
+
+// (the schedule only: the descriptor's tables are not read)
+int leaf_bits(const rgs_slab_desc* d, int* bits) {
+  if (!d) return fail(kInvalid, "null descriptor");
+  if (!d->defer_leaves) return fail(kInvalid, "the descriptor does not defer its leaves");
+  std::vector<PassPlan> plan;
+  std::string err;
+  int rc = plan_passes(d, plan, err);
+  if (rc) return fail(rc, err);
+  *bits = plan.back().tb;
+  return kOk;
+}
+
 }  // namespace rgs
 
 using namespace rgs;
+
+extern "C" int rgs_slab_leaf_bits(const rgs_slab_desc* d, int* bits) {
+  if (!bits) return fail(kInvalid, "null output");
+  return leaf_bits(d, bits);
+}
+
+extern "C" int rgs_slab_leaf_tiles(const rgs_slab_desc* d, const int64_t* tiles, int count, void* out,
+                                   void* stream) {
+  return leaf_tiles(d, tiles, count, out, reinterpret_cast<hipStream_t>(stream));
+}
+
+extern "C" int rgs_slab_finish(const rgs_slab_desc* d, void* stream) {
+  PassPlan p;
+  PassArgs a;
+  size_t shmem;
+  int rc = leaf_pass(d, p, a, shmem);
+  if (rc) return rc;
+  // a mass-only last pass left the state final already
+  if (!(a.flags & ~F_LOAD)) return kOk;
+  hipError_t e = launch_pass(a, p, shmem, reinterpret_cast<hipStream_t>(stream));
+  if (e != hipSuccess) return fail(kHip, std::string("k_pass launch: ") + hipGetErrorString(e));
+  return kOk;
+}
 
 extern "C" int rgs_slab_describe(const rgs_slab_desc* d, int* passes, char* text, int text_len) {
   if (!d) return fail(kInvalid, "null descriptor");
@@ -364,10 +441,11 @@ extern "C" int rgs_slab_describe(const rgs_slab_desc* d, int* passes, char* text
     std::string s;
     for (auto& p : plan) {
       char buf[160];
-      std::snprintf(buf, sizeof buf, "%s%s%s%s%s%s:%d+%d/tb%d/c%d/rb%d ", p.low ? "low" : "strided",
+      std::snprintf(buf, sizeof buf, "%s%s%s%s%s%s%s:%d+%d/tb%d/c%d/rb%d ", p.low ? "low" : "strided",
                     (p.flags & F_LOAD) ? "" : "[init]", (p.flags & F_WHT) ? "[wht]" : "",
                     (p.flags & F_PHASE) ? "[phase]" : "", (p.flags & F_GATES) ? "[gates]" : "",
-                    (p.flags & F_MASS) ? "[mass]" : "", p.qstart, p.count, p.tb, p.c, p.rb);
+                    (p.flags & F_MASS) ? "[mass]" : "", (p.flags & F_NOSTORE) ? "[leaf]" : "", p.qstart,
+                    p.count, p.tb, p.c, p.rb);
       s += buf;
     }
     std::snprintf(text, text_len, "%s", s.c_str());

@@ -34,7 +34,9 @@ namespace rgs {
 // F_INIT_UP: a contiguous pass that makes the state and only applies the
 // Walsh butterflies builds its values in the lowest layout and walks up to
 // the store layout (one LDS exchange fewer than walking down and back).
-enum : int { F_LOAD = 1, F_WHT = 2, F_PHASE = 4, F_GATES = 8, F_MASS = 16, F_INIT_UP = 32 };
+// F_NOSTORE: the contiguous mass pass writes the masses only (the state
+// keeps the values it loaded; its tiles are recomputed on demand).
+enum : int { F_LOAD = 1, F_WHT = 2, F_PHASE = 4, F_GATES = 8, F_MASS = 16, F_INIT_UP = 32, F_NOSTORE = 64 };
 
 constexpr size_t kLdsBudget = 160 * 1024;
 
@@ -420,9 +422,12 @@ __device__ __forceinline__ void walk_rot_up(float2 (&v)[16], float2* lds, uint32
   }
 }
 
-// One tile of a pass: tile index blk of the state described by a.
-template <int TB, int C, int RBW>
-__device__ __forceinline__ void tile_run(const PassArgs& a, uint64_t blk) {
+// One tile of a pass: tile index blk of the state described by a.  With
+// LEAF (contiguous passes only) the tile's values go to leaf[0 .. 2^TB)
+// instead of back into the state.
+template <int TB, int C, int RBW, bool LEAF = false>
+__device__ __forceinline__ void tile_run(const PassArgs& a, uint64_t blk, float2* leaf = nullptr) {
+  static_assert(!LEAF || C == 0, "leaf tiles are contiguous");
   using G = Geo<TB, C, RBW>;
   constexpr int S1 = G::S1;
   constexpr int SL = G::at(G::count() - 1);  // == C
@@ -585,6 +590,7 @@ __device__ __forceinline__ void tile_run(const PassArgs& a, uint64_t blk) {
       // a block's pairwise sum is the same in state and outcome order (a
       // XOR relabelling keeps every pair), so only its slot moves
       if ((tid & ((1u << sb) - 1u)) == 0) a.mass[((base >> a.mass_bits) + (tid >> sb)) ^ a.mflip] = m;
+      if (a.flags & F_NOSTORE) return;
     }
   }
   if (!at_top) {
@@ -600,8 +606,38 @@ __device__ __forceinline__ void tile_run(const PassArgs& a, uint64_t blk) {
       exchange<TB, C, RBW, SL, S1>(v, lds, tl, tr);
     }
   }
+  if constexpr (LEAF) {
+    // g0 - base: the thread's offset inside the tile
 #pragma unroll
-  for (int j = 0; j < 16; ++j) store_stream(a.state + g0 + ((uint64_t)j << gsh), v[j]);
+    for (int j = 0; j < 16; ++j) leaf[(g0 - base) + ((uint64_t)j << gsh)] = v[j];
+  } else {
+#pragma unroll
+    for (int j = 0; j < 16; ++j) store_stream(a.state + g0 + ((uint64_t)j << gsh), v[j]);
+  }
+}
+
+// The contiguous pass's values of listed tiles: workgroup b recomputes tile
+// tiles[b] into out[b * 2^TB ..] (an index past the state's tiles is skipped).
+template <int TB, int RBW>
+__global__ void __launch_bounds__(1 << (TB - 4)) k_tile_leaf(PassArgs a, const int64_t* __restrict__ tiles,
+                                                            float2* __restrict__ out) {
+  const uint64_t t = (uint64_t)tiles[blockIdx.x];
+  if (t >> (a.nbits - TB)) return;
+  tile_run<TB, 0, RBW, true>(a, t, out + ((uint64_t)blockIdx.x << TB));
+}
+
+template <int TB, int RBW>
+hipError_t launch_tile_leaf(const PassArgs& a, const int64_t* tiles, int count, float2* out, size_t shmem,
+                            hipStream_t st) {
+  static bool attr_set = false;
+  if (!attr_set) {
+    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_tile_leaf<TB, RBW>),
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)kLdsBudget);
+    if (e != hipSuccess) return e;
+    attr_set = true;
+  }
+  hipLaunchKernelGGL((k_tile_leaf<TB, RBW>), dim3((unsigned)count), dim3(1 << (TB - 4)), shmem, st, a, tiles, out);
+  return hipGetLastError();
 }
 
 template <int TB, int C, int RBW>
@@ -662,6 +698,9 @@ hipError_t launch_c4(int tb, int rbw, const PassArgs& a, size_t shmem, hipStream
 hipError_t launch_c5(int tb, int rbw, const PassArgs& a, size_t shmem, hipStream_t st);
 hipError_t launch_c6(int tb, int rbw, const PassArgs& a, size_t shmem, hipStream_t st);
 hipError_t launch_c7(int tb, int rbw, const PassArgs& a, size_t shmem, hipStream_t st);
+// The leaf-tile launches of the contiguous specialisations (slab_k0.hip).
+hipError_t launch_c0_leaf(int tb, int rbw, const PassArgs& a, const int64_t* tiles, int count, float2* out,
+                          size_t shmem, hipStream_t st);
 // The multi-state launches of the same specialisations (count states of
 // 2^shift tiles each).
 #define RGS_MULTI_DECL(NAME) \

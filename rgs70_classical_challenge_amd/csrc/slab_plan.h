@@ -28,6 +28,10 @@ size_t lds_bytes(const PassPlan& p, int nbits, const rgs_slab_desc* d);
 int check_desc(const rgs_slab_desc* d, std::string& err);
 void build_args(const rgs_slab_desc* d, const PassPlan& p, PassArgs& a);
 hipError_t launch_pass(const PassArgs& a, const PassPlan& p, size_t shmem, hipStream_t st);
+// The deferred last pass's values of listed tiles (rgs_slab_leaf_tiles) and
+// its tile bits.
+int leaf_tiles(const rgs_slab_desc* d, const int64_t* tiles, int count, void* out, hipStream_t st);
+int leaf_bits(const rgs_slab_desc* d, int* bits);
 hipError_t launch_pass_multi(const PassArgs* args, int shift, int count, const PassPlan& p, size_t shmem,
                              hipStream_t st);
 

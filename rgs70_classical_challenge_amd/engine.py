@@ -108,7 +108,7 @@ class _Slab:
 
     def desc(self, init_kind, init_scale=0.0, init_lut=None, init_wlut=None, walsh=False, phase_lut=None,
              gates=None, gate_limit=None, mass=True, init_rows=None, phase_rows=None, init_cols=None,
-             init_wbits=8):
+             init_wbits=8, defer_leaves=False):
         """gates: a Gates object (rotations + diagonal phase) or None;
         init_rows / phase_rows: host form rows (form_rows, cov_form) of the
         sign tables init_lut (GRAPH and COV init) and phase_lut; init_cols:
@@ -140,6 +140,7 @@ class _Slab:
         d.top_bits = int(self.tuning.get('top_bits', 0))
         d.group_order = int(self.tuning.get('group_order', 0))
         d.flip = gates.flip if gates is not None else 0
+        d.defer_leaves = int(defer_leaves)
         # explicit strided groups, low bits up: (width, lane bits, LDS round bits)
         groups = self.tuning.get('groups') or ()
         if len(groups) > 8:
@@ -187,6 +188,52 @@ class _Slab:
         p = torch.empty(len(u), dtype=torch.float64, device=self.device)
         call('rgs_draw', ptr(self.state), self.nbits, ptr(self.pyramid), self.mass_bits, ptr(u), len(u), int(flip),
              ptr(idx), ptr(p), self.stream.cuda_stream)
+        return idx.cpu().numpy(), p.cpu().numpy()
+
+    # ---- deferred leaves: the last pass wrote masses only (defer_leaves)
+    def leaf_bits(self, d):
+        import ctypes
+        b = ctypes.c_int()
+        call('rgs_slab_leaf_bits', ctypes_ref(d), ctypes.byref(b))
+        return b.value
+
+    def leaf_tiles(self, d, tiles):
+        """Final values of the listed tiles (state order), a device
+        complex64 tensor [len(tiles), 2^leaf_bits]."""
+        torch = self.torch
+        lb = self.leaf_bits(d)
+        t = torch.as_tensor(np.ascontiguousarray(tiles, np.int64)).to(self.device)
+        out = torch.empty((len(t), 1 << lb), dtype=torch.complex64, device=self.device)
+        call('rgs_slab_leaf_tiles', ctypes_ref(d), ptr(t), len(t), ptr(out), self.stream.cuda_stream)
+        return out
+
+    def leaf_values(self, d, indices):
+        """Final complex64 values at state indices (host array)."""
+        ix = np.asarray(indices, np.int64)
+        lb = self.leaf_bits(d)
+        tiles, inv = np.unique(ix >> lb, return_inverse=True)
+        vals = self.leaf_tiles(d, tiles).cpu().numpy()
+        return vals[inv.reshape(ix.shape), ix & ((1 << lb) - 1)]
+
+    def leaf_block(self, d, start, stop):
+        """Final complex64 values of state indices start..stop-1 (host array)."""
+        lb = self.leaf_bits(d)
+        t0, t1 = start >> lb, ((stop - 1) >> lb) + 1
+        vals = self.leaf_tiles(d, np.arange(t0, t1, dtype=np.int64)).reshape(-1).cpu().numpy()
+        return vals[start - (t0 << lb):stop - (t0 << lb)]
+
+    def draw_leaves(self, d, uniforms):
+        """rgs_draw on a deferred state (rgs_draw_leaves): (outcomes, p)."""
+        import ctypes
+        torch = self.torch
+        u = torch.as_tensor(np.ascontiguousarray(uniforms, np.float64)).to(self.device)
+        nb = ctypes.c_size_t()
+        call('rgs_draw_leaves_workspace', ctypes_ref(d), len(u), ctypes.byref(nb))
+        work = torch.empty(max(nb.value, 1), dtype=torch.uint8, device=self.device)
+        idx = torch.empty(len(u), dtype=torch.int64, device=self.device)
+        p = torch.empty(len(u), dtype=torch.float64, device=self.device)
+        call('rgs_draw_leaves', ctypes_ref(d), ptr(self.pyramid), ptr(u), len(u), ptr(work), nb.value, ptr(idx),
+             ptr(p), self.stream.cuda_stream)
         return idx.cpu().numpy(), p.cpu().numpy()
 
     def gather_prob(self, indices):
@@ -313,7 +360,7 @@ class GraphStateSampler:
 
     PIPELINES = ('slab', 'literal', 'folded')
 
-    def __init__(self, g, u, device=0, pipeline='slab', mass_bits=8, tuning=None, arena=None):
+    def __init__(self, g, u, device=0, pipeline='slab', mass_bits=8, tuning=None, arena=None, defer_leaves=False):
         g = np.asarray(g, np.uint8)
         u = np.asarray(u, np.complex128)
         n = len(g)
@@ -324,6 +371,10 @@ class GraphStateSampler:
         if pipeline not in self.PIPELINES:
             raise ValueError(f'pipeline must be one of {self.PIPELINES}')
         self.g, self.u, self.n, self.pipeline = g, u, n, pipeline
+        # defer_leaves: the last pass writes the masses only; draws and reads
+        # recompute the tiles they touch, host() finishes the state
+        self.defer_leaves = bool(defer_leaves)
+        self._pending = False
         self.slab = _Slab(n, device, mass_bits, tuning, arena=arena)
         sl = self.slab
         self._lut = sl.to_device(graph_phase_lut(g))
@@ -336,15 +387,16 @@ class GraphStateSampler:
     def _make_desc(self):
         """Descriptor of this pipeline with the slab's current tuning."""
         sl, n = self.slab, self.n
+        dl = self.defer_leaves
         if self.pipeline == 'slab':
             return sl.desc(_lib.INIT_GRAPH, 2.0 ** (-n), init_lut=self._lut, init_rows=self._rows, walsh=True,
-                           gates=self._gates)
+                           gates=self._gates, defer_leaves=dl)
         if self.pipeline == 'literal':
             # H^n = WHT / 2^{n/2}: the delta carries the normalisation
             return sl.desc(_lib.INIT_DELTA, 2.0 ** (-n / 2), walsh=True, phase_lut=self._lut,
-                           phase_rows=self._rows, gates=self._gates)
+                           phase_rows=self._rows, gates=self._gates, defer_leaves=dl)
         return sl.desc(_lib.INIT_GRAPH, 2.0 ** (-n / 2), init_lut=self._lut, init_rows=self._rows,
-                       gates=self._gates)
+                       gates=self._gates, defer_leaves=dl)
 
     def compute(self):
         """Build the state and its mass pyramid; returns stage seconds."""
@@ -352,7 +404,15 @@ class GraphStateSampler:
         self.slab.run(self._desc)
         self.slab.sync()
         self._norm = None
+        self._pending = self.defer_leaves
         return {'state_and_masses': time.perf_counter() - t}
+
+    def finish(self):
+        """Write the deferred last pass's values into the state (once per compute)."""
+        if self._pending:
+            call('rgs_slab_finish', ctypes_ref(self._desc), self.slab.stream.cuda_stream)
+            self.slab.sync()
+            self._pending = False
 
     def masses(self, tilebits=None):
         """(mass float64[nmass], norm, tilebits, nmass) as DistributedSampler.masses
@@ -378,28 +438,35 @@ class GraphStateSampler:
 
     def sample(self, uniforms):
         """Device qubit-by-qubit draws, one uniform each: (outcomes, p)."""
+        if self._pending:
+            return self.slab.draw_leaves(self._desc, uniforms)
         return self.slab.draw_many(uniforms, self.flip)
 
     def draw(self, rng):
         """One draw consuming one uniform: (outcome, p, norm)."""
         norm = self.norm()
-        idx, p = self.slab.draw_many([rng.random()], self.flip)
+        idx, p = self.sample([rng.random()])
         return int(idx[0]), float(p[0]), norm
 
     def probability_at(self, index):
         if not 0 <= index < (1 << self.n):
             raise ValueError('index outside the state')
+        if self._pending:
+            v = self.slab.leaf_values(self._desc, [index ^ self.flip])[0]
+            return float(np.float64(v.real) ** 2 + np.float64(v.imag) ** 2)
         return float(self.slab.gather_prob([index ^ self.flip])[0])
 
     def device_amplitudes(self):
         """The device complex64 values before the D1 phases, as complex128, in
         outcome order: the numbers the masses, the pyramid and the draws are
         computed from."""
+        self.finish()
         return self._gates.to_outcomes(self.slab.state.cpu().numpy().astype(np.complex128))
 
     def host(self):
         """The state as complex128 in outcome order (device complex64 values
         times the D1 phases)."""
+        self.finish()
         return self._gates.to_outcomes(self._gates.apply_d1(self.slab.state.cpu().numpy().astype(np.complex128)))
 
     def sync(self):
@@ -431,7 +498,8 @@ class ConditionalSampler:
     """
 
     def __init__(self, g, u, A, B, devices=(0,), chunk_mib=1, fusion_bits=5, lane_bits=5, lookup_bits=8,
-                 defer_top=True, mass_bits=8, tuning=None, weight_bits=None, weight_table=None):
+                 defer_top=True, mass_bits=8, tuning=None, weight_bits=None, weight_table=None,
+                 defer_leaves=True):
         g = np.ascontiguousarray(np.asarray(g, np.uint8))
         u = np.ascontiguousarray(np.asarray(u, np.complex128))
         A, B = [int(x) for x in A], [int(x) for x in B]
@@ -445,6 +513,10 @@ class ConditionalSampler:
         self.g, self.u, self.A, self.B, self.m = g, u, A, B, m
         self.ng = 1
         self.defer_top = defer_top
+        # the last pass writes the masses only; the draw recomputes the tiles
+        # it reads (as the terminal gate is applied at draw time)
+        self.defer_leaves = bool(defer_leaves)
+        self._pending = False
         self.slab = _Slab(m, list(devices)[0], mass_bits, tuning, kind='c')
         sl = self.slab
         chunks = (m + 7) // 8
@@ -491,7 +563,8 @@ class ConditionalSampler:
         d = self.slab.desc(_lib.INIT_COV, 0.0, init_lut=self._map, init_wlut=self._wlut, walsh=True,
                            init_rows=self._cov_rows, init_cols=self._cov_cols, phase_lut=self._aphase,
                            phase_rows=self._arows, gates=self._gates,
-                           gate_limit=m - 1 if self.defer_top else m, init_wbits=wbits)
+                           gate_limit=m - 1 if self.defer_top else m, init_wbits=wbits,
+                           defer_leaves=self.defer_leaves)
         d.init_wpop = int(self._weight_table)
         return d
 
@@ -515,9 +588,17 @@ class ConditionalSampler:
         t = time.perf_counter()
         self.slab.run(self._desc)
         self.slab.sync()
+        self._pending = self.defer_leaves
         stage['state_and_masses'] = time.perf_counter() - t
         stage['linear_permutation_phase'] = 0.0
         return stage
+
+    def finish(self):
+        """Write the deferred last pass's values into the state (once per compute)."""
+        if self._pending:
+            call('rgs_slab_finish', ctypes_ref(self._desc), self.slab.stream.cuda_stream)
+            self.slab.sync()
+            self._pending = False
 
     def masses(self):
         """Tile masses (tiles of 2^min(20, m-1) inside each half), norm, tilebits, nmass per half."""
@@ -530,6 +611,8 @@ class ConditionalSampler:
         return mass, norm, tb, len(mass) // 2
 
     def _slice(self, start, stop):
+        if self._pending:
+            return self.slab.leaf_block(self._desc, start, stop)
         return self.slab.state[start:stop].cpu().numpy()
 
     def _outcome_slice(self, start, stop):
@@ -599,16 +682,24 @@ class ConditionalSampler:
         (DeferredGlobalMixin.draw, deferred_global_sampler.py:72-78).
         Returns (indices, unnormalised p)."""
         flip = self._gates.flip
-        idx, p = self.slab.draw_many(uniforms, flip)
+        if self._pending:
+            idx, p = self.slab.draw_leaves(self._desc, uniforms)
+        else:
+            idx, p = self.slab.draw_many(uniforms, flip)
         if not self.defer_top:
             return idx, p
         if bit_uniforms is None or len(bit_uniforms) != len(idx):
             raise ValueError('one bit uniform per draw is needed with the deferred top gate')
-        torch = self.slab.torch
-        local = torch.as_tensor((idx % self.n) ^ flip).to(self.slab.device)     # state indices
-        st = self.slab.state
-        lo = st.index_select(0, local).cpu().numpy().astype(np.complex128)
-        hi = st.index_select(0, local + self.n).cpu().numpy().astype(np.complex128)
+        local = (idx % self.n) ^ flip     # state indices
+        if self._pending:
+            lo = self.slab.leaf_values(self._desc, local).astype(np.complex128)
+            hi = self.slab.leaf_values(self._desc, local + self.n).astype(np.complex128)
+        else:
+            torch = self.slab.torch
+            lt = torch.as_tensor(local).to(self.slab.device)
+            st = self.slab.state
+            lo = st.index_select(0, lt).cpu().numpy().astype(np.complex128)
+            hi = st.index_select(0, lt + self.n).cpu().numpy().astype(np.complex128)
         U = self.deferred_global_unitary
         a = U[0, 0] * lo + U[0, 1] * hi
         b = U[1, 0] * lo + U[1, 1] * hi
@@ -622,10 +713,14 @@ class ConditionalSampler:
             if bit not in (0, 1):
                 raise ValueError('Bad global index')
             return float(abs(self.transformed_pair(index % self.n)[bit]) ** 2)
+        if self._pending:
+            v = self.slab.leaf_values(self._desc, [index ^ self._gates.flip])[0]
+            return float(np.float64(v.real) ** 2 + np.float64(v.imag) ** 2)
         return float(self.slab.gather_prob([index ^ self._gates.flip])[0])
 
     def host(self):
         """The conditional state as complex128 in outcome order."""
+        self.finish()
         raw = self._gates.apply_d1(self.slab.state.cpu().numpy().astype(np.complex128))
         if self.defer_top:
             blk = raw.reshape(2, -1)

@@ -33,7 +33,7 @@ def test_library_exports_every_declared_symbol():
     assert len(names) >= 10
     missing = [n for n in sorted(names) if not hasattr(lib, n)]
     assert not missing, missing
-    assert lib.rgs_abi_version() == 2
+    assert lib.rgs_abi_version() == 3
 
 
 def test_binding_covers_declarations():
@@ -162,7 +162,7 @@ def test_cov_luts_reject_singular_cut():
 
 
 def describe(nbits, walsh, gates=True, phase=False, mass=True, init=_lib.INIT_GRAPH, low=14, tile=14, lane=4,
-             rounds=0, phase_rounds=-1, top=0, order=0, groups=()):
+             rounds=0, phase_rounds=-1, top=0, order=0, groups=(), defer=False, desc_out=None):
     d = _lib.SlabDesc()
     d.state = 1
     d.nbits = nbits
@@ -180,6 +180,9 @@ def describe(nbits, walsh, gates=True, phase=False, mass=True, init=_lib.INIT_GR
     d.group_order = order
     for i, (bits, glane, grounds) in enumerate(groups):
         d.group_bits[i], d.group_lanes[i], d.group_rounds[i] = bits, glane, grounds
+    d.defer_leaves = int(defer)
+    if desc_out is not None:
+        desc_out.append(d)
     n = ctypes.c_int()
     buf = ctypes.create_string_buffer(4096)
     _lib.call('rgs_slab_describe', ctypes.byref(d), ctypes.byref(n), buf, 4096)
@@ -260,6 +263,48 @@ def test_explicit_group_schedules():
             describe(22, walsh=True, low=8, groups=bad)
     with pytest.raises(ValueError):
         describe(20, walsh=True, low=6, groups=[(7, 7, 0), (7, 6, 0)])   # 7 lane bits, 6 bits below the group
+
+
+def test_deferred_leaf_schedules():
+    """defer_leaves marks the last contiguous pass [leaf] (masses only) and
+    is refused where that pass would make the state or has no mass level;
+    the leaf entry points check their arguments before any launch."""
+    keep = []
+    n, text = describe(34, walsh=True, phase=True, low=13, tile=13, lane=6, order=1, defer=True, desc_out=keep)
+    full = describe(34, walsh=True, phase=True, low=13, tile=13, lane=6, order=1)[1]
+    assert text.split()[:-1] == full.split()[:-1], text
+    assert text.split()[-1] == 'low[gates][mass][leaf]:0+13/tb13/c0/rb0', text
+    d = keep[0]
+    bits = ctypes.c_int()
+    _lib.call('rgs_slab_leaf_bits', ctypes.byref(d), ctypes.byref(bits))
+    assert bits.value == 13
+    nb = ctypes.c_size_t()
+    _lib.call('rgs_draw_leaves_workspace', ctypes.byref(d), 10000, ctypes.byref(nb))
+    assert nb.value == ((10000 * 24 + 255) // 256) * 256 + (10000 << 13) * 8
+    with pytest.raises(ValueError, match='2\\^24'):
+        _lib.call('rgs_slab_leaf_tiles', ctypes.byref(d), 1, (1 << 24) + 1, 1, None)
+    with pytest.raises(ValueError, match='null'):
+        _lib.call('rgs_slab_leaf_tiles', ctypes.byref(d), None, 4, 1, None)
+    with pytest.raises(ValueError, match='2\\^24'):
+        _lib.call('rgs_draw_leaves', ctypes.byref(d), 1, 1, (1 << 24) + 1, 1, 1 << 40, 1, 1, None)
+    with pytest.raises(ValueError, match='aligned'):
+        _lib.call('rgs_draw_leaves', ctypes.byref(d), 1, 1, 16, 8, 1 << 40, 1, 1, None)
+    with pytest.raises(ValueError, match='smaller'):
+        _lib.call('rgs_draw_leaves', ctypes.byref(d), 1, 1, 16, 256, 1024, 1, 1, None)
+    n, text = describe(35, walsh=True, phase=True, low=12, groups=[(8, 5, 0), (8, 6, 1), (7, 7, 1)], defer=True)
+    assert text.split()[-1] == 'low[gates][mass][leaf]:0+12/tb12/c0/rb0', text
+    # one pass makes the state: nothing to defer; no mass level: nothing to draw
+    with pytest.raises(ValueError, match='defer_leaves'):
+        describe(12, walsh=True, defer=True)
+    with pytest.raises(ValueError, match='defer_leaves'):
+        describe(34, walsh=True, mass=False, defer=True)
+    # a descriptor that does not defer has no leaf tiles
+    keep = []
+    describe(34, walsh=True, desc_out=keep)
+    with pytest.raises(ValueError, match='does not defer'):
+        _lib.call('rgs_slab_leaf_bits', ctypes.byref(keep[0]), ctypes.byref(bits))
+    with pytest.raises(ValueError, match='does not defer'):
+        _lib.call('rgs_slab_finish', ctypes.byref(keep[0]), None)
 
 
 def test_gate_shear_factorisation(golden_dir):
@@ -379,6 +424,12 @@ def test_slab_batch_rejects_bad_items_before_touching_the_device():
         with pytest.raises(ValueError, match=msg):
             _lib.call('rgs_slab_batch_create', arr, 2, ctypes.byref(h))
         assert not h.value
+    deferred = desc()
+    deferred.defer_leaves = 1
+    arr = (_lib.SlabBatchItem * 1)(item(deferred))
+    with pytest.raises(ValueError, match='defer_leaves'):
+        _lib.call('rgs_slab_batch_create', arr, 1, ctypes.byref(h))
+    assert not h.value
     arr = (_lib.SlabBatchItem * 1)(item(good, samples=-1))
     with pytest.raises(ValueError, match='draw buffers'):
         _lib.call('rgs_slab_batch_create', arr, 1, ctypes.byref(h))

@@ -175,6 +175,20 @@ def test_headline_n34_full_size():
     se = np.sqrt(z_draw.var() / len(z_draw) + (z_unif ** 2).var() / len(z_unif))
     print(f'  E_draw[2^n p] = {a:.4f}, E_unif[(2^n p)^2] = {b:.4f}, s.e. {se:.4f}', flush=True)
     assert abs(a - b) < 5 * se
+    # the same step with the last pass deferred (masses only; draws recompute
+    # the tiles they read): the same pyramid, draws and p bit for bit, and the
+    # finished state equal to the stored one
+    p_probe = s.slab.gather_prob(probe ^ flip)
+    s.defer_leaves = True
+    s._desc = s._make_desc()
+    assert s.plan_text.split()[:-1] == s.slab.describe(s._desc)[1].split()[:-1]
+    s.compute()
+    assert np.array_equal(sl.pyramid.cpu().numpy(), pyr)
+    idx2, p2 = s.sample(uni)
+    assert np.array_equal(idx2, idx) and np.array_equal(p2, p)
+    assert np.array_equal([s.probability_at(int(x)) for x in probe[:64]], p_probe[:64])
+    s.finish()
+    assert np.array_equal(fetch(blocks), leaves)
     del s, state
     gc.collect()
     torch.cuda.empty_cache()
