@@ -247,6 +247,15 @@ class _Slab:
         self.torch.cuda.synchronize(self.device)
 
 
+def _deferred(slab, d, defer):
+    """d with its last pass deferred (defer_leaves) when asked for and the
+    schedule has one to defer (a single pass makes the state itself)."""
+    if defer:
+        passes, _ = slab.describe(d)
+        d.defer_leaves = int(passes >= 2)
+    return d
+
+
 def ctypes_ref(d):
     import ctypes
     return ctypes.byref(d)
@@ -387,16 +396,17 @@ class GraphStateSampler:
     def _make_desc(self):
         """Descriptor of this pipeline with the slab's current tuning."""
         sl, n = self.slab, self.n
-        dl = self.defer_leaves
         if self.pipeline == 'slab':
-            return sl.desc(_lib.INIT_GRAPH, 2.0 ** (-n), init_lut=self._lut, init_rows=self._rows, walsh=True,
-                           gates=self._gates, defer_leaves=dl)
-        if self.pipeline == 'literal':
+            d = sl.desc(_lib.INIT_GRAPH, 2.0 ** (-n), init_lut=self._lut, init_rows=self._rows, walsh=True,
+                        gates=self._gates)
+        elif self.pipeline == 'literal':
             # H^n = WHT / 2^{n/2}: the delta carries the normalisation
-            return sl.desc(_lib.INIT_DELTA, 2.0 ** (-n / 2), walsh=True, phase_lut=self._lut,
-                           phase_rows=self._rows, gates=self._gates, defer_leaves=dl)
-        return sl.desc(_lib.INIT_GRAPH, 2.0 ** (-n / 2), init_lut=self._lut, init_rows=self._rows,
-                       gates=self._gates, defer_leaves=dl)
+            d = sl.desc(_lib.INIT_DELTA, 2.0 ** (-n / 2), walsh=True, phase_lut=self._lut,
+                        phase_rows=self._rows, gates=self._gates)
+        else:
+            d = sl.desc(_lib.INIT_GRAPH, 2.0 ** (-n / 2), init_lut=self._lut, init_rows=self._rows,
+                        gates=self._gates)
+        return _deferred(sl, d, self.defer_leaves)
 
     def compute(self):
         """Build the state and its mass pyramid; returns stage seconds."""
@@ -404,7 +414,7 @@ class GraphStateSampler:
         self.slab.run(self._desc)
         self.slab.sync()
         self._norm = None
-        self._pending = self.defer_leaves
+        self._pending = bool(self._desc.defer_leaves)
         return {'state_and_masses': time.perf_counter() - t}
 
     def finish(self):
@@ -563,10 +573,9 @@ class ConditionalSampler:
         d = self.slab.desc(_lib.INIT_COV, 0.0, init_lut=self._map, init_wlut=self._wlut, walsh=True,
                            init_rows=self._cov_rows, init_cols=self._cov_cols, phase_lut=self._aphase,
                            phase_rows=self._arows, gates=self._gates,
-                           gate_limit=m - 1 if self.defer_top else m, init_wbits=wbits,
-                           defer_leaves=self.defer_leaves)
+                           gate_limit=m - 1 if self.defer_top else m, init_wbits=wbits)
         d.init_wpop = int(self._weight_table)
-        return d
+        return _deferred(self.slab, d, self.defer_leaves)
 
     def compute(self, outB):
         if len(outB) != self.m:
@@ -588,7 +597,7 @@ class ConditionalSampler:
         t = time.perf_counter()
         self.slab.run(self._desc)
         self.slab.sync()
-        self._pending = self.defer_leaves
+        self._pending = bool(self._desc.defer_leaves)
         stage['state_and_masses'] = time.perf_counter() - t
         stage['linear_permutation_phase'] = 0.0
         return stage

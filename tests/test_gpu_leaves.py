@@ -93,7 +93,8 @@ def test_conditional_deferred_leaves(golden_dir, name, tuning):
     g, u, A, B = d['graph'], d['measurement_unitaries'], d['A'].tolist(), d['B'].tolist()
     ref = E.ConditionalSampler(g, u, A, B, [0], tuning=tuning, defer_leaves=False)
     dfr = E.ConditionalSampler(g, u, A, B, [0], tuning=tuning)
-    assert dfr.defer_leaves and dfr.plan_text.split()[-1].split(':')[0] == 'low[gates][mass][leaf]'
+    # a one-pass schedule makes the state itself: nothing to defer
+    assert dfr.defer_leaves and ('[leaf]' in dfr.plan_text) == (dfr.passes > 1), dfr.plan_text
     rng = np.random.default_rng(2424)
     for trial in range(2):
         ob = rng.integers(0, 2, len(B)).tolist()
