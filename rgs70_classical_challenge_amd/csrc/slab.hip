@@ -36,7 +36,9 @@ size_t lds_bytes(const PassPlan& p, int nbits, const rgs_slab_desc* d) {
   if ((p.flags & F_PHASE) && d->dphase_lut && !d->dphase_global) bytes += (size_t)chunks_of(nbits) * 256 * 8;
   if (!(p.flags & F_LOAD) && d->init_kind == RGS_INIT_COV) {
     const int wb = d->init_wbits ? d->init_wbits : 8;
-    bytes += d->init_wpop ? (size_t)(nbits + 1) * (nbits + 1) * 8
+    // the popcount table as staged: (|o| + 1) (m - |o| + 1) entries, at most
+    // (floor(m/2) + 1) (ceil(m/2) + 1) whatever the outcome bits
+    bytes += d->init_wpop ? (size_t)(nbits / 2 + 1) * ((nbits + 1) / 2 + 1) * 8
                           : (size_t)((nbits + wb - 1) / wb) * ((size_t)1 << wb) * 8;
   }
   return bytes;
@@ -320,8 +322,11 @@ void build_args(const rgs_slab_desc* d, const PassPlan& p, PassArgs& a) {
   a.wbits = d->init_wbits ? d->init_wbits : 8;
   a.wchunks = (n + a.wbits - 1) / a.wbits;
   a.wpop = d->init_wpop ? n + 1 : 0;
-  a.wentries = a.wpop ? (n + 1) * (n + 1) : a.wchunks << a.wbits;
+  const int opop = __builtin_popcountll(d->init_omask);
+  a.wstride = n - opop + 1;
+  a.wentries = a.wpop ? (opop + 1) * a.wstride : a.wchunks << a.wbits;
   a.womask = d->init_omask;
+  a.wnmask = ~d->init_omask & ((n < 64 ? 1ull << n : 0ull) - 1ull);
 }
 
 hipError_t launch_pass_multi(const PassArgs* args, int shift, int count, const PassPlan& p, size_t shmem,

@@ -87,7 +87,9 @@ struct PassArgs {
   int wchunks;        // COV weight chunks
   int wpop;           // COV popcount table: its row stride m + 1 (0: chunk products)
   int wentries;       // COV weight entries staged in LDS
-  uint64_t womask;    // COV popcount table: the B outcome bits
+  int wstride;        // COV popcount table as staged: row stride m - |o| + 1
+  uint64_t womask;    // COV popcount table: the B outcome bits o
+  uint64_t wnmask;    // COV popcount table: the other B bits (~o, m bits)
   uint64_t mflip;     // mass blocks: block b of the state is outcome block b ^ mflip
 };
 
@@ -220,9 +222,11 @@ __device__ __forceinline__ void cov_init_fixed(float2 (&v)[16], uint64_t x0, con
   }
 }
 
-// The covariant init with identical B unitaries: the weight of b is
-// table[popc(b)][popc(b & o)] (o: the B outcome bits), one table read per
-// amplitude; map, sign and register offsets as in cov_init_fixed.
+// The covariant init with identical B unitaries: the weight of b depends on
+// h = popc(b & o) <= |o| and l = popc(b & ~o) <= m - |o| only (o: the B
+// outcome bits), so the table is staged in LDS as [h][l] - at most
+// (m/2 + 1)^2 entries instead of (m + 1)^2 - and read once per amplitude;
+// map, sign and register offsets as in cov_init_fixed.
 __device__ __forceinline__ void cov_init_popc(float2 (&v)[16], uint64_t x0, const PassArgs& a,
                                               const uint64_t* lut, const float2* w) {
   uint64_t b0 = 0, acc = 0;
@@ -232,8 +236,8 @@ __device__ __forceinline__ void cov_init_popc(float2 (&v)[16], uint64_t x0, cons
     acc ^= lut[2 * e + 1];
   }
   const uint32_t sg = register_signs((uint32_t)__popcll(acc & (x0 | (1ull << 63))) & 1u, x0, a.iq, a.ifreg);
-  const uint64_t o = a.womask;
-  const uint32_t st = (uint32_t)a.wpop;
+  const uint64_t o = a.womask, no = a.wnmask;
+  const uint32_t st = (uint32_t)a.wstride;
 #pragma unroll
   for (int j = 0; j < 16; ++j) {
     uint64_t dj = 0;   // uniform: M j
@@ -241,8 +245,8 @@ __device__ __forceinline__ void cov_init_popc(float2 (&v)[16], uint64_t x0, cons
     for (int i = 0; i < 4; ++i)
       if (j & (1 << i)) dj ^= a.icol[i];
     const uint64_t b = b0 ^ dj;
-    const uint32_t k = (uint32_t)__popcll(b), h = (uint32_t)__popcll(b & o);
-    v[j] = cneg_if(w[k * st + h], (sg >> j) & 1u);
+    const uint32_t h = (uint32_t)__popcll(b & o), l = (uint32_t)__popcll(b & no);
+    v[j] = cneg_if(w[h * st + l], (sg >> j) & 1u);
   }
 }
 
@@ -454,8 +458,14 @@ __device__ __forceinline__ void tile_run(const PassArgs& a, uint64_t blk, float2
     if (cov)
       for (int i = tid; i < a.wentries; i += G::NT) {
         // the rotations' scale enters every weight once: the popcount table's
-        // entries, or the first chunk table of a product of chunk tables
-        const float2 w = a.init_w[i];
+        // entries, or the first chunk table of a product of chunk tables.
+        // Popcount entry (h, l) is the table's [h + l][h].
+        uint32_t src = (uint32_t)i;
+        if (a.wpop) {
+          const uint32_t h = (uint32_t)i / (uint32_t)a.wstride, l = (uint32_t)i - h * (uint32_t)a.wstride;
+          src = (h + l) * (uint32_t)a.wpop + h;
+        }
+        const float2 w = a.init_w[src];
         const float sc = (a.wpop || i < (1 << a.wbits)) ? a.wscale : 1.f;
         lut_w[i] = make_float2(__fmul_rn(w.x, sc), __fmul_rn(w.y, sc));
       }
