@@ -26,6 +26,9 @@ Reported beside the headline value:
                 read GB/s, and whether the draws equal the stored-pass draws.
   folded        the same state with H^n|0> folded into the init (no Walsh
                 passes), samples/s.
+  stream_reference  torch copy_ (read + write) and fill_ (write only) of
+                32 GiB on the same device: the library's own streaming rates,
+                beside which the pass rates can be read.
   scan          batched rank-scan p(x) of 65,536 device-resident uniform 70-bit
                 proposals of a 70-vertex 3-regular instance per launch (FP64
                 sums), proposals/s with complex64 tables (the scorer) and
@@ -433,6 +436,36 @@ def cpu_baseline(seed):
             'pyramid_seconds': round(t_pyr, 3), 'draw_seconds': round(t_draw, 3)}
 
 
+def stream_reference(torch, gib=32):
+    """Library streaming rates on the same device, as context for the pass
+    rates: torch copy_ (read + write) and fill_ (write only) of gib GiB,
+    best of five each (HIP events)."""
+    n = (gib << 30) // 4
+    a = torch.ones(n, dtype=torch.float32, device='cuda')
+    b = torch.empty_like(a)
+
+    def best(fn):
+        fn()
+        torch.cuda.synchronize()
+        t = []
+        for _ in range(5):
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record()
+            fn()
+            e1.record()
+            torch.cuda.synchronize()
+            t.append(e0.elapsed_time(e1))
+        return min(t)
+
+    copy_ms = best(lambda: b.copy_(a))
+    fill_ms = best(lambda: b.fill_(0.0))
+    del a, b
+    torch.cuda.empty_cache()
+    return {'bytes': gib << 30, 'torch_copy_ms': round(copy_ms, 3),
+            'torch_copy_gbps': round(2 * (gib << 30) / copy_ms / 1e6, 1),
+            'torch_fill_ms': round(fill_ms, 3), 'torch_fill_gbps': round((gib << 30) / fill_ms / 1e6, 1)}
+
+
 def calibrate_copy(torch):
     """One plain device-to-device copy of 2^32 bytes for counter calibration."""
     a = torch.ones(1 << 30, dtype=torch.float32, device='cuda')
@@ -523,6 +556,11 @@ def main():
         log('[bench] folded done')
         del s, res
         torch.cuda.empty_cache()
+        try:
+            out['stream_reference'] = stream_reference(torch)
+        except Exception as e:
+            out['stream_reference'] = {'error': repr(e)}
+        log('[bench] stream reference done')
         try:
             out['scan'] = run_scan(torch)
         except Exception as e:
