@@ -272,11 +272,12 @@ void build_args(const rgs_slab_desc* d, const PassPlan& p, PassArgs& a) {
   a.init_w = reinterpret_cast<const float2*>(d->init_wlut);
   a.phase_lut = d->phase_lut;
   a.dphase = reinterpret_cast<const float2*>(d->dphase_lut);
-  // rotations of this pass's target bits travel in the kernel arguments
+  // rotations of this pass's target bits travel in the kernel arguments;
+  // qubits at or above gate_limit get t = 0 (the kernel rotates every bit)
   const int t0 = p.low ? 0 : p.qstart;
   for (int i = 0; i < 16; ++i) {
     const int q = t0 + i;
-    a.tng[i] = (d->rot && i < p.count && q < n) ? d->rot[q] : 0.f;
+    a.tng[i] = (d->rot && i < p.count && q < n && q < d->gate_limit) ? d->rot[q] : 0.f;
   }
   // the rotations' common scale, applied once: in the values the state is
   // made from, or as the first pass loads a given state

@@ -370,14 +370,18 @@ __device__ __forceinline__ void wht_layout(float2 (&v)[16]) {
   if constexpr (s + 3 < h) wht_reg<3>(v);
 }
 
+// Every target bit of the layout is rotated: a qubit at or above gate_limit
+// has t = 0, the identity (x - 0 y = x and y + 0 x = y exactly, up to the sign
+// of a zero).  A runtime skip per bit would make the compiler copy all 32
+// rotated registers back at the branch's merge.
 template <typename G, int C, int K>
 __device__ __forceinline__ void rot_layout(float2 (&v)[16], const PassArgs& a) {
   constexpr int s = G::at(K), h = G::hi(K);
   // target bit index of register bit B: s + B - C
-  if constexpr (s + 0 < h) if (a.qstart + s + 0 - C < a.gate_limit) rot_reg<0>(v, a.tng[s + 0 - C]);
-  if constexpr (s + 1 < h) if (a.qstart + s + 1 - C < a.gate_limit) rot_reg<1>(v, a.tng[s + 1 - C]);
-  if constexpr (s + 2 < h) if (a.qstart + s + 2 - C < a.gate_limit) rot_reg<2>(v, a.tng[s + 2 - C]);
-  if constexpr (s + 3 < h) if (a.qstart + s + 3 - C < a.gate_limit) rot_reg<3>(v, a.tng[s + 3 - C]);
+  if constexpr (s + 0 < h) rot_reg<0>(v, a.tng[s + 0 - C]);
+  if constexpr (s + 1 < h) rot_reg<1>(v, a.tng[s + 1 - C]);
+  if constexpr (s + 2 < h) rot_reg<2>(v, a.tng[s + 2 - C]);
+  if constexpr (s + 3 < h) rot_reg<3>(v, a.tng[s + 3 - C]);
 }
 
 // Walk down the layouts from K applying the Walsh butterflies (ends at layout count-1).
