@@ -27,6 +27,7 @@
 #include "rgs_common.h"
 #include "rgs_amd.h"
 
+#include <algorithm>
 #include <type_traits>
 
 namespace rgs {
@@ -430,12 +431,36 @@ __device__ __forceinline__ void walk_rot_up(float2 (&v)[16], float2* lds, uint32
   }
 }
 
+// FP64 masses of the 2^mass_bits blocks of a contiguous tile at state index
+// base, from registers in layout 0 (local bits 0..3): the pairwise tree over
+// the thread's 16 values, then across the lanes.  A block's pairwise sum is
+// the same in state and outcome order (a XOR relabelling keeps every pair),
+// so only its slot moves.
+__device__ __forceinline__ void write_masses(const float2 (&v)[16], const PassArgs& a, uint64_t base,
+                                             uint32_t tid) {
+  double p[16];
+#pragma unroll
+  for (int j = 0; j < 16; ++j) p[j] = prob64(v[j]);
+#pragma unroll
+  for (int w = 1; w < 16; w <<= 1) {
+#pragma unroll
+    for (int j = 0; j < 16; j += 2 * w) p[j] = __dadd_rn(p[j], p[j + w]);
+  }
+  double m = p[0];
+  const int sb = a.mass_bits - 4;
+  for (int b = 0; b < sb; ++b) m = __dadd_rn(m, __shfl_xor(m, 1 << b));
+  if ((tid & ((1u << sb) - 1u)) == 0) a.mass[((base >> a.mass_bits) + (tid >> sb)) ^ a.mflip] = m;
+}
+
 // One tile of a pass: tile index blk of the state described by a.  With
 // LEAF (contiguous passes only) the tile's values go to leaf[0 .. 2^TB)
-// instead of back into the state.
-template <int TB, int C, int RBW, bool LEAF = false>
+// instead of back into the state.  FL != 0 fixes the pass's flags at compile
+// time (they must equal a.flags): the branches of the other pass kinds drop
+// out of the kernel.
+template <int TB, int C, int RBW, bool LEAF = false, int FL = 0>
 __device__ __forceinline__ void tile_run(const PassArgs& a, uint64_t blk, float2* leaf = nullptr) {
   static_assert(!LEAF || C == 0, "leaf tiles are contiguous");
+  const int flags = FL ? FL : a.flags;
   using G = Geo<TB, C, RBW>;
   constexpr int S1 = G::S1;
   constexpr int SL = G::at(G::count() - 1);  // == C
@@ -452,7 +477,7 @@ __device__ __forceinline__ void tile_run(const PassArgs& a, uint64_t blk, float2
   const uint32_t tl = tid & ((1u << G::TLB) - 1u);
   const uint32_t tr = RBW ? __builtin_amdgcn_readfirstlane(tid >> G::TLB) : 0u;
 
-  const bool cov = !(a.flags & F_LOAD) && a.init_kind == RGS_INIT_COV;
+  const bool cov = !(flags & F_LOAD) && a.init_kind == RGS_INIT_COV;
   // the diagonal-phase table is read through LDS or straight from global
   // memory (a per-plan choice: which is faster depends on the pass's other
   // work); each has its own code path, so LDS reads stay LDS instructions
@@ -487,12 +512,12 @@ __device__ __forceinline__ void tile_run(const PassArgs& a, uint64_t blk, float2
   const int gsh = a.qstart + S1 - C;
 
   // a state made in place starts in layout SL when it walks up, else in S1
-  const bool up = C == 0 && (a.flags & F_INIT_UP);
+  const bool up = C == 0 && (flags & F_INIT_UP);
   const uint64_t gi = up ? gidx(local_base<SL, C, RBW>(tl, tr)) : g0;
   const int gish = up ? a.qstart + SL - C : gsh;
 
   float2 v[16];
-  if (a.flags & F_LOAD) {
+  if (flags & F_LOAD) {
 #pragma unroll
     for (int j = 0; j < 16; ++j) v[j] = load_stream(a.state + g0 + ((uint64_t)j << gsh));
     if (a.lscale != 1.f) {
@@ -567,18 +592,18 @@ __device__ __forceinline__ void tile_run(const PassArgs& a, uint64_t blk, float2
   if (up) {
     if constexpr (C == 0) walk_wht_up<TB, C, RBW, G::count() - 1>(v, lds, tl, tr);
     at_top = true;
-  } else if (a.flags & F_WHT) {
+  } else if (flags & F_WHT) {
     walk_wht<TB, C, RBW, 0>(v, lds, tl, tr);
-    if (a.flags & F_PHASE) apply_phase(std::integral_constant<int, SL>());
-    if (a.flags & F_GATES) {
+    if (flags & F_PHASE) apply_phase(std::integral_constant<int, SL>());
+    if (flags & F_GATES) {
       walk_rot_up<TB, C, RBW, G::count() - 1>(v, lds, tl, tr, a);
       at_top = true;
     } else {
       at_top = (SL == S1);
     }
   } else {
-    if (a.flags & F_PHASE) apply_phase(std::integral_constant<int, S1>());
-    if (a.flags & F_GATES) {
+    if (flags & F_PHASE) apply_phase(std::integral_constant<int, S1>());
+    if (flags & F_GATES) {
       walk_rot_down<TB, C, RBW, 0>(v, lds, tl, tr, a);
       at_top = (SL == S1);
     } else {
@@ -586,25 +611,12 @@ __device__ __forceinline__ void tile_run(const PassArgs& a, uint64_t blk, float2
     }
   }
   if constexpr (C == 0) {
-    if (a.flags & F_MASS) {
+    if (flags & F_MASS) {
       // registers must hold local bits 0..3 (layout SL = 0)
       if (at_top) exchange<TB, C, RBW, S1, SL>(v, lds, tl, tr);
       at_top = (SL == S1);
-      double p[16];
-#pragma unroll
-      for (int j = 0; j < 16; ++j) p[j] = prob64(v[j]);
-#pragma unroll
-      for (int w = 1; w < 16; w <<= 1) {
-#pragma unroll
-        for (int j = 0; j < 16; j += 2 * w) p[j] = __dadd_rn(p[j], p[j + w]);
-      }
-      double m = p[0];
-      const int sb = a.mass_bits - 4;
-      for (int b = 0; b < sb; ++b) m = __dadd_rn(m, __shfl_xor(m, 1 << b));
-      // a block's pairwise sum is the same in state and outcome order (a
-      // XOR relabelling keeps every pair), so only its slot moves
-      if ((tid & ((1u << sb) - 1u)) == 0) a.mass[((base >> a.mass_bits) + (tid >> sb)) ^ a.mflip] = m;
-      if (a.flags & F_NOSTORE) return;
+      write_masses(v, a, base, tid);
+      if (flags & F_NOSTORE) return;
     }
   }
   if (!at_top) {
@@ -659,6 +671,63 @@ __global__ void __launch_bounds__(1 << (TB - 4)) k_tile(PassArgs a) {
   tile_run<TB, C, RBW>(a, (uint64_t)blockIdx.x + ((uint64_t)blockIdx.y << 24));
 }
 
+// The contiguous mass pass with its flags fixed (the gates and masses of the
+// last pass, storing the state: 49.1 -> 46.9 ms at n = 34), and the flags of
+// the masses-only pass (k_tile_mass_stream below).
+constexpr int kMassPass = F_LOAD | F_GATES | F_MASS;
+constexpr int kMassOnlyPass = F_LOAD | F_GATES | F_MASS | F_NOSTORE;
+template <int TB, int RBW, int FL>
+__global__ void __launch_bounds__(1 << (TB - 4)) k_tile_mass(PassArgs a) {
+  tile_run<TB, 0, RBW, false, FL>(a, (uint64_t)blockIdx.x + ((uint64_t)blockIdx.y << 24));
+}
+
+// The masses-only pass as a stream: workgroup g takes the tiles g, g + G,
+// g + 2G, ... (G = gridDim.x, about the workgroups a device holds at once)
+// and loads each next tile's amplitudes while it rotates the current one,
+// in two register sets (the pass stores nothing, so it keeps few registers).
+// Loads, rotations and masses are tile_run's, so the masses are the same
+// bits.
+template <int TB>
+__global__ void __launch_bounds__(1 << (TB - 4)) k_tile_mass_stream(PassArgs a) {
+  using G = Geo<TB, 0, 0>;
+  constexpr int S1 = G::S1;
+  extern __shared__ __align__(16) unsigned char smem[];
+  float2* lds = reinterpret_cast<float2*>(smem);
+  const uint32_t tid = threadIdx.x;
+  const uint64_t tiles = 1ull << (a.nbits - TB);
+  const uint64_t step = gridDim.x;
+  const uint32_t off = local_base<S1, 0, 0>(tid, 0u);   // the thread's offset in a tile
+  // (the masses-only pass is never a schedule's first, so its loads carry no
+  // scale: the launcher checks lscale == 1)
+  auto load = [&](float2 (&v)[16], uint64_t blk) {
+    const float2* src = a.state + (blk << TB) + off;
+#pragma unroll
+    for (int j = 0; j < 16; ++j) v[j] = load_stream(src + ((uint64_t)j << S1));
+  };
+  auto tile = [&](float2 (&v)[16], uint64_t blk) {
+    walk_rot_down<TB, 0, 0, 0>(v, lds, tid, 0u, a);
+    write_masses(v, a, blk << TB, tid);
+  };
+  uint64_t blk = blockIdx.x;
+  if (blk >= tiles) return;   // the whole workgroup
+  // The next tile is loaded unconditionally (the last one again past the
+  // end): a load behind a branch makes the compiler wait for every load in
+  // flight before the current tile's first use, which would undo the overlap.
+  auto next = [&](uint64_t b) { return b + step < tiles ? b + step : b; };
+  float2 va[16], vb[16];
+  load(va, blk);
+  for (;;) {
+    load(vb, next(blk));
+    tile(va, blk);
+    blk += step;
+    if (blk >= tiles) break;
+    load(va, next(blk));
+    tile(vb, blk);
+    blk += step;
+    if (blk >= tiles) break;
+  }
+}
+
 // The same pass over many states of one plan in one launch.  The states of
 // a plan group have the same size, so each owns 2^shift consecutive tiles:
 // workgroup b runs tile b & (2^shift - 1) of state b >> shift, with that
@@ -687,20 +756,53 @@ hipError_t launch_tile_multi(const PassArgs* args, int shift, int count, size_t 
   return hipGetLastError();
 }
 
-// One workgroup per tile.
-template <int TB, int C, int RBW>
-hipError_t launch_tile(const PassArgs& a, size_t shmem, hipStream_t st) {
-  static bool attr_set = false;
+// One workgroup per tile: kernel K over the state's 2^(nbits - TB) tiles.
+template <int TB>
+hipError_t launch_tiles(const void* kernel, const PassArgs& a, size_t shmem, hipStream_t st, bool& attr_set) {
   if (!attr_set) {
-    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_tile<TB, C, RBW>),
-                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)kLdsBudget);
+    hipError_t e = hipFuncSetAttribute(kernel, hipFuncAttributeMaxDynamicSharedMemorySize, (int)kLdsBudget);
     if (e != hipSuccess) return e;
     attr_set = true;
   }
   const uint64_t tiles = 1ull << (a.nbits - TB);
   dim3 grid((unsigned)(tiles < (1ull << 24) ? tiles : (1ull << 24)), (unsigned)(tiles >> 24 ? tiles >> 24 : 1));
-  hipLaunchKernelGGL((k_tile<TB, C, RBW>), grid, dim3(1 << (TB - 4)), shmem, st, a);
+  void* args[] = {const_cast<PassArgs*>(&a)};
+  return hipLaunchKernel(kernel, grid, dim3(1 << (TB - 4)), args, shmem, st);
+}
+
+// The masses-only stream: as many workgroups as the device's CUs hold at
+// once (LDS bound, at most four per CU), at most one per tile.
+template <int TB>
+hipError_t launch_tile_mass_stream(const PassArgs& a, size_t shmem, hipStream_t st) {
+  static bool attr_set = false;
+  if (!attr_set) {
+    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_tile_mass_stream<TB>),
+                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)kLdsBudget);
+    if (e != hipSuccess) return e;
+    attr_set = true;
+  }
+  int dev = 0, cus = 0;
+  hipError_t e = hipGetDevice(&dev);
+  if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+  if (e != hipSuccess) return e;
+  const uint64_t per_cu = std::max<uint64_t>(1, std::min<uint64_t>(4, kLdsBudget / std::max<size_t>(shmem, 1)));
+  const uint64_t tiles = 1ull << (a.nbits - TB);
+  const uint64_t grid = std::min<uint64_t>(tiles, per_cu * (uint64_t)std::max(cus, 1));
+  hipLaunchKernelGGL((k_tile_mass_stream<TB>), dim3((unsigned)grid), dim3(1 << (TB - 4)), shmem, st, a);
   return hipGetLastError();
+}
+
+template <int TB, int C, int RBW>
+hipError_t launch_tile(const PassArgs& a, size_t shmem, hipStream_t st) {
+  if constexpr (C == 0) {
+    static bool mass_attr = false;
+    if (a.flags == kMassPass)
+      return launch_tiles<TB>(reinterpret_cast<const void*>(&k_tile_mass<TB, RBW, kMassPass>), a, shmem, st,
+                              mass_attr);
+    if (a.flags == kMassOnlyPass && a.lscale == 1.f) return launch_tile_mass_stream<TB>(a, shmem, st);
+  }
+  static bool attr_set = false;
+  return launch_tiles<TB>(reinterpret_cast<const void*>(&k_tile<TB, C, RBW>), a, shmem, st, attr_set);
 }
 
 // Dispatch over the instantiated specialisations of one lane-bit count
