@@ -27,7 +27,6 @@
 #include "rgs_common.h"
 #include "rgs_amd.h"
 
-#include <algorithm>
 #include <type_traits>
 
 namespace rgs {
@@ -672,60 +671,15 @@ __global__ void __launch_bounds__(1 << (TB - 4)) k_tile(PassArgs a) {
 }
 
 // The contiguous mass pass with its flags fixed (the gates and masses of the
-// last pass, storing the state: 49.1 -> 46.9 ms at n = 34), and the flags of
-// the masses-only pass (k_tile_mass_stream below).
+// last pass, storing the state or not): 49.0 -> 46.3 ms for the storing pass
+// at n = 34.  (A masses-only variant that loads each next tile while it
+// rotates the current one measured the same, 29.6-29.8 ms at n = 34: the
+// pass is bound by instruction issue, not by load latency.)
 constexpr int kMassPass = F_LOAD | F_GATES | F_MASS;
 constexpr int kMassOnlyPass = F_LOAD | F_GATES | F_MASS | F_NOSTORE;
 template <int TB, int RBW, int FL>
 __global__ void __launch_bounds__(1 << (TB - 4)) k_tile_mass(PassArgs a) {
   tile_run<TB, 0, RBW, false, FL>(a, (uint64_t)blockIdx.x + ((uint64_t)blockIdx.y << 24));
-}
-
-// The masses-only pass as a stream: workgroup g takes the tiles g, g + G,
-// g + 2G, ... (G = gridDim.x, about the workgroups a device holds at once)
-// and loads each next tile's amplitudes while it rotates the current one,
-// in two register sets (the pass stores nothing, so it keeps few registers).
-// Loads, rotations and masses are tile_run's, so the masses are the same
-// bits.
-template <int TB>
-__global__ void __launch_bounds__(1 << (TB - 4)) k_tile_mass_stream(PassArgs a) {
-  using G = Geo<TB, 0, 0>;
-  constexpr int S1 = G::S1;
-  extern __shared__ __align__(16) unsigned char smem[];
-  float2* lds = reinterpret_cast<float2*>(smem);
-  const uint32_t tid = threadIdx.x;
-  const uint64_t tiles = 1ull << (a.nbits - TB);
-  const uint64_t step = gridDim.x;
-  const uint32_t off = local_base<S1, 0, 0>(tid, 0u);   // the thread's offset in a tile
-  // (the masses-only pass is never a schedule's first, so its loads carry no
-  // scale: the launcher checks lscale == 1)
-  auto load = [&](float2 (&v)[16], uint64_t blk) {
-    const float2* src = a.state + (blk << TB) + off;
-#pragma unroll
-    for (int j = 0; j < 16; ++j) v[j] = load_stream(src + ((uint64_t)j << S1));
-  };
-  auto tile = [&](float2 (&v)[16], uint64_t blk) {
-    walk_rot_down<TB, 0, 0, 0>(v, lds, tid, 0u, a);
-    write_masses(v, a, blk << TB, tid);
-  };
-  uint64_t blk = blockIdx.x;
-  if (blk >= tiles) return;   // the whole workgroup
-  // The next tile is loaded unconditionally (the last one again past the
-  // end): a load behind a branch makes the compiler wait for every load in
-  // flight before the current tile's first use, which would undo the overlap.
-  auto next = [&](uint64_t b) { return b + step < tiles ? b + step : b; };
-  float2 va[16], vb[16];
-  load(va, blk);
-  for (;;) {
-    load(vb, next(blk));
-    tile(va, blk);
-    blk += step;
-    if (blk >= tiles) break;
-    load(va, next(blk));
-    tile(vb, blk);
-    blk += step;
-    if (blk >= tiles) break;
-  }
 }
 
 // The same pass over many states of one plan in one launch.  The states of
@@ -770,36 +724,16 @@ hipError_t launch_tiles(const void* kernel, const PassArgs& a, size_t shmem, hip
   return hipLaunchKernel(kernel, grid, dim3(1 << (TB - 4)), args, shmem, st);
 }
 
-// The masses-only stream: as many workgroups as the device's CUs hold at
-// once (LDS bound, at most four per CU), at most one per tile.
-template <int TB>
-hipError_t launch_tile_mass_stream(const PassArgs& a, size_t shmem, hipStream_t st) {
-  static bool attr_set = false;
-  if (!attr_set) {
-    hipError_t e = hipFuncSetAttribute(reinterpret_cast<const void*>(&k_tile_mass_stream<TB>),
-                                       hipFuncAttributeMaxDynamicSharedMemorySize, (int)kLdsBudget);
-    if (e != hipSuccess) return e;
-    attr_set = true;
-  }
-  int dev = 0, cus = 0;
-  hipError_t e = hipGetDevice(&dev);
-  if (e == hipSuccess) e = hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-  if (e != hipSuccess) return e;
-  const uint64_t per_cu = std::max<uint64_t>(1, std::min<uint64_t>(4, kLdsBudget / std::max<size_t>(shmem, 1)));
-  const uint64_t tiles = 1ull << (a.nbits - TB);
-  const uint64_t grid = std::min<uint64_t>(tiles, per_cu * (uint64_t)std::max(cus, 1));
-  hipLaunchKernelGGL((k_tile_mass_stream<TB>), dim3((unsigned)grid), dim3(1 << (TB - 4)), shmem, st, a);
-  return hipGetLastError();
-}
-
 template <int TB, int C, int RBW>
 hipError_t launch_tile(const PassArgs& a, size_t shmem, hipStream_t st) {
   if constexpr (C == 0) {
-    static bool mass_attr = false;
+    static bool mass_attr = false, mass_only_attr = false;
     if (a.flags == kMassPass)
       return launch_tiles<TB>(reinterpret_cast<const void*>(&k_tile_mass<TB, RBW, kMassPass>), a, shmem, st,
                               mass_attr);
-    if (a.flags == kMassOnlyPass && a.lscale == 1.f) return launch_tile_mass_stream<TB>(a, shmem, st);
+    if (a.flags == kMassOnlyPass)
+      return launch_tiles<TB>(reinterpret_cast<const void*>(&k_tile_mass<TB, RBW, kMassOnlyPass>), a, shmem, st,
+                              mass_only_attr);
   }
   static bool attr_set = false;
   return launch_tiles<TB>(reinterpret_cast<const void*>(&k_tile<TB, C, RBW>), a, shmem, st, attr_set);
