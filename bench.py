@@ -89,8 +89,14 @@ def pass_bytes(plan_text, n):
 
 
 def kernel_of(tok):
-    """k_tile specialisation of a plan token such as 'strided[wht]:14+8/tb14/c6/rb2'."""
-    _, tb, c, rb = tok.split('/')
+    """Kernel of a plan token such as 'strided[wht]:14+8/tb14/c6/rb2': the
+    contiguous gates + masses pass has its flags fixed at compile time
+    (k_tile_mass<TB, RBW, flags>: 25 storing the state, 89 masses only), every
+    other pass is the k_tile specialisation of its shape."""
+    kind, tb, c, rb = tok.split('/')
+    head = kind.split(':')[0]
+    if head in ('low[gates][mass]', 'low[gates][mass][leaf]'):
+        return f'k_tile_mass<{int(tb[2:])}, {int(rb[2:])}, {89 if "[leaf]" in head else 25}>'
     return f'k_tile<{int(tb[2:])}, {int(c[1:])}, {int(rb[2:])}>'
 
 

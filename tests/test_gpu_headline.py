@@ -1,9 +1,10 @@
 """The n = 34 plan's kernels against the oracle, at a size the oracle holds in
 full and at the full n = 34 size.
 
-The bench's plan runs two k_tile specialisations: k_tile<13, 0, 0> (13-bit
-contiguous tiles) and k_tile<13, 6, 0> (13-bit strided tiles, 64-amplitude
-rows, no LDS rounds), with the strided groups walked from the high bits down.
+The bench's plan runs two tile shapes: 13-bit contiguous tiles (k_tile<13, 0, 0>
+for the init pass, k_tile_mass<13, 0, 25> for the gates + masses pass) and
+13-bit strided tiles with 64-amplitude rows and no LDS rounds (k_tile<13, 6,
+0>), with the strided groups walked from the high bits down.
 At n = 27 the same tuning gives the same pass kinds (two 7-bit strided groups:
 Walsh-only, Walsh + gates, gates-only), and the whole state fits the oracle.
 At n = 34 no oracle holds 2^34 amplitudes, so the checks are the FP64 norm,

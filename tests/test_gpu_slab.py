@@ -91,7 +91,8 @@ TUNINGS = [None, dict(low_bits=8, tile_bits=8, lane_bits=3), dict(low_bits=10, t
            dict(low_bits=8, tile_bits=10, lane_bits=3, group_order=1),
            # the n = 34 plan's shape: 13-bit contiguous tiles, 13-bit strided
            # tiles with 64-amplitude rows and no LDS rounds, groups walked from
-           # the high bits down (k_tile<13, 0, 0> and k_tile<13, 6, 0>)
+           # the high bits down (k_tile<13, 0, 0>, k_tile_mass<13, 0, 25> and
+           # k_tile<13, 6, 0>)
            HEADLINE]
 
 
