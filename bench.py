@@ -90,13 +90,20 @@ def pass_bytes(plan_text, n):
 
 def kernel_of(tok):
     """Kernel of a plan token such as 'strided[wht]:14+8/tb14/c6/rb2': the
-    contiguous gates + masses pass has its flags fixed at compile time
-    (k_tile_mass<TB, RBW, flags>: 25 storing the state, 89 masses only), every
-    other pass is the k_tile specialisation of its shape."""
+    contiguous mass passes have their flags fixed at compile time
+    (k_tile_mass<TB, RBW, flags>: gates + masses 25 storing the state, 89
+    masses only; Walsh + phase + gates + masses 31 and 95), and so do the
+    strided passes that load the state and apply Walsh, phase and gates
+    (k_tile_fixed<TB, C, RBW, 15>); every other pass is the k_tile
+    specialisation of its shape."""
     kind, tb, c, rb = tok.split('/')
     head = kind.split(':')[0]
-    if head in ('low[gates][mass]', 'low[gates][mass][leaf]'):
-        return f'k_tile_mass<{int(tb[2:])}, {int(rb[2:])}, {89 if "[leaf]" in head else 25}>'
+    mass = {'low[gates][mass]': 25, 'low[gates][mass][leaf]': 89, 'low[wht][phase][gates][mass]': 31,
+            'low[wht][phase][gates][mass][leaf]': 95}
+    if head in mass:
+        return f'k_tile_mass<{int(tb[2:])}, {int(rb[2:])}, {mass[head]}>'
+    if head == 'strided[wht][phase][gates]':
+        return f'k_tile_fixed<{int(tb[2:])}, {int(c[1:])}, {int(rb[2:])}, 15>'
     return f'k_tile<{int(tb[2:])}, {int(c[1:])}, {int(rb[2:])}>'
 
 

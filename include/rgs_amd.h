@@ -33,7 +33,7 @@ extern "C" {
 #define RGS_ENOMEM 3 /* MemoryError                          */
 #define RGS_EHIP 4   /* RuntimeError (HIP runtime failure)   */
 
-#define RGS_ABI_VERSION 3
+#define RGS_ABI_VERSION 4
 
 /* Thread-local message of the last failure ("" if none). */
 const char *rgs_last_error(void);
@@ -196,6 +196,15 @@ typedef struct rgs_slab_desc {
                                   (rgs_slab_leaf_tiles, rgs_draw_leaves) and
                                   rgs_slab_finish writes the whole state.
                                   0: every pass writes the state             */
+  int fuse_walsh;              /* with do_walsh and no phase_lut: 1 gives each
+                                  pass the Walsh butterflies, the diagonal
+                                  phases (dphase_lut restricted to the pass's
+                                  bits) and the rotations of its own target
+                                  bits, so the first strided group makes the
+                                  state and one pass per group follows (four
+                                  at n = 34 instead of seven); the per-qubit
+                                  order Walsh -> phase -> rotation is kept.
+                                  0: the Walsh transform on all bits first   */
 } rgs_slab_desc;
 
 /* Factor 2x2 unitaries u (complex128 [n][2][2]) as

@@ -58,6 +58,7 @@ class SlabDesc(ctypes.Structure):
         ('init_omask', ctypes.c_uint64),
         ('init_wpop', c_int),
         ('defer_leaves', c_int),
+        ('fuse_walsh', c_int),
     ]
 
 

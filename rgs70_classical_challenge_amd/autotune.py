@@ -3,7 +3,9 @@
 For a state size n it times every admissible (low_bits, tile_bits,
 lane_bits, round_bits) choice of the fused pass schedule on the device - each
 candidate runs the whole pipeline a few times and keeps its best time - and
-then, for the fastest, a narrow phase group (top_bits, with the contiguous
+then, for the fastest (each shape also with fuse_walsh for the plain
+pipeline: every pass applies the Walsh butterflies, phases and rotations of
+its own bits), a narrow phase group (top_bits, with the contiguous
 pass one bit wider or narrower), the LDS rounds of the pass that applies the
 phase tables (phase_round_bits), where that pass reads its diagonal-phase
 table (dphase_global), and the order in which the strided groups are walked
@@ -149,7 +151,14 @@ def tune_engine(eng, label, reps=3, extra_stage=True, start=None):
         if best is None or ms < best[1]:
             best = (dict(cand), ms)
 
-    for cand in (candidates(n) if start is None else [dict(start)]):
+    # a pipeline with no sign table after its Walsh transform (the plain
+    # 'slab' pipeline) may also give each pass the Walsh butterflies, phases
+    # and rotations of its own bits (fuse_walsh): every shape is timed both ways
+    fusable = getattr(eng, 'pipeline', None) == 'slab'
+    cands = candidates(n) if start is None else [dict(start)]
+    if fusable and start is None:
+        cands = cands + [dict(c, fuse_walsh=1) for c in cands]
+    for cand in cands:
         print(f'[autotune] {label} timing {cand}', file=sys.stderr, flush=True)
         run(cand)
     if start is None and extra_stage and best is not None and n > best[0]['low_bits']:

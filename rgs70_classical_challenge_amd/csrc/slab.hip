@@ -27,13 +27,31 @@ namespace rgs {
 
 static int chunks_of(int bits) { return (bits + 7) / 8; }
 
+// The diagonal phases a pass applies: all bits, or with fuse_walsh the pass's
+// own target bits.  Returns the number of 8-bit chunks of dphase_lut that
+// cover them, with the first chunk in *c0 and the bits in *mask.
+static int dphase_span(const rgs_slab_desc* d, const PassPlan& p, int* c0, uint64_t* mask) {
+  int lo = 0, hi = d->nbits;
+  if (d->fuse_walsh) {
+    lo = p.low ? 0 : p.qstart;
+    hi = lo + p.count;
+  }
+  *c0 = lo / 8;
+  *mask = (hi >= 64 ? ~0ull : (1ull << hi) - 1ull) & ~((1ull << lo) - 1ull);
+  return chunks_of(hi) - lo / 8;
+}
+
 size_t lds_bytes(const PassPlan& p, int nbits, const rgs_slab_desc* d) {
   size_t slots = ((size_t)1 << (p.tb - p.rb)) + ((size_t)1 << (p.tb - p.rb - 4));
   size_t bytes = slots * 8;
   // LDS tables: the phase pass's diagonal-phase table (unless it is read from
   // global memory) and the covariant init's weights; sign and map tables are
   // read from global memory
-  if ((p.flags & F_PHASE) && d->dphase_lut && !d->dphase_global) bytes += (size_t)chunks_of(nbits) * 256 * 8;
+  if ((p.flags & F_PHASE) && d->dphase_lut && !d->dphase_global) {
+    int c0;
+    uint64_t mask;
+    bytes += (size_t)dphase_span(d, p, &c0, &mask) * 256 * 8;
+  }
   if (!(p.flags & F_LOAD) && d->init_kind == RGS_INIT_COV) {
     const int wb = d->init_wbits ? d->init_wbits : 8;
     // the popcount table as staged: (|o| + 1) (m - |o| + 1) entries, at most
@@ -53,6 +71,11 @@ int plan_passes(const rgs_slab_desc* d, std::vector<PassPlan>& out, std::string&
   const int L = d->lane_bits;
   if (d->round_bits < 0 || d->round_bits > 2) { err = "round_bits must be in 0..2"; return kInvalid; }
   if (d->phase_round_bits < -1 || d->phase_round_bits > 2) { err = "phase_round_bits must be in -1..2"; return kInvalid; }
+  if (d->fuse_walsh != 0 && d->fuse_walsh != 1) { err = "fuse_walsh must be 0 or 1"; return kInvalid; }
+  if (d->fuse_walsh && (!d->do_walsh || d->phase_lut)) {
+    err = "fuse_walsh needs the Walsh transform and no sign table after it (phase_lut)";
+    return kInvalid;
+  }
   std::vector<PassPlan> groups;
   if (d->group_bits[0] > 0) {
     // explicit groups: each with its own width, row width and LDS rounds
@@ -141,6 +164,13 @@ int plan_passes(const rgs_slab_desc* d, std::vector<PassPlan>& out, std::string&
   out.clear();
   if (k == 0) {
     out.push_back(with(low, src | (walsh ? F_WHT : 0) | (phase ? F_PHASE : 0) | (gates ? F_GATES : 0) | (mass ? F_MASS : 0)));
+  } else if (walsh && d->fuse_walsh) {
+    // every pass takes the Walsh butterflies, the diagonal phases and the
+    // rotations of its own bits (operations on different qubits commute):
+    // the first group makes the state, the contiguous pass ends with the masses
+    const int own = F_WHT | (phase ? F_PHASE : 0) | (gates ? F_GATES : 0);
+    for (int i = 0; i < k; ++i) out.push_back(with(groups[d->group_order ? k - 1 - i : i], (i ? F_LOAD : src) | own));
+    out.push_back(with(low, F_LOAD | own | (mass ? F_MASS : 0)));
   } else if (walsh) {
     // the Walsh groups may run in either order (the transform factors over
     // bits); the last one also applies the phase and its gates, and the
@@ -272,6 +302,8 @@ void build_args(const rgs_slab_desc* d, const PassPlan& p, PassArgs& a) {
   a.init_w = reinterpret_cast<const float2*>(d->init_wlut);
   a.phase_lut = d->phase_lut;
   a.dphase = reinterpret_cast<const float2*>(d->dphase_lut);
+  a.dshift = 0;
+  a.dmask = ~0ull;
   // rotations of this pass's target bits travel in the kernel arguments;
   // qubits at or above gate_limit get t = 0 (the kernel rotates every bit)
   const int t0 = p.low ? 0 : p.qstart;
@@ -297,7 +329,13 @@ void build_args(const rgs_slab_desc* d, const PassPlan& p, PassArgs& a) {
   const bool gen = !(p.flags & F_LOAD);
   a.init_chunks = (gen && (d->init_kind == RGS_INIT_GRAPH || d->init_kind == RGS_INIT_COV)) ? chunks_of(n) : 0;
   a.phase_chunks = ((p.flags & F_PHASE) && d->phase_lut) ? chunks_of(n) : 0;
-  a.dphase_chunks = ((p.flags & F_PHASE) && d->dphase_lut) ? chunks_of(n) : 0;
+  a.dphase_chunks = 0;
+  if ((p.flags & F_PHASE) && d->dphase_lut) {
+    int c0;
+    a.dphase_chunks = dphase_span(d, p, &c0, &a.dmask);
+    a.dphase += (size_t)c0 * 256;
+    a.dshift = 8 * c0;
+  }
   a.gate_limit = d->rot ? std::min(d->gate_limit, n) : 0;
   {
     const int s1 = std::max(p.tb - 4, p.c);

@@ -83,6 +83,8 @@ struct PassArgs {
   int gate_limit;
   int mass_bits;
   int dphase_global;  // read dphase from global memory instead of staging it
+  int dshift;         // dphase starts at chunk dshift / 8 of the global index
+  uint64_t dmask;     // global bits whose diagonal phases this pass applies
   int wbits;          // COV weight chunk bits (8 or 9)
   int wchunks;        // COV weight chunks
   int wpop;           // COV popcount table: its row stride m + 1 (0: chunk products)
@@ -559,11 +561,13 @@ __device__ __forceinline__ void tile_run(const PassArgs& a, uint64_t blk, float2
     const uint64_t gb = gidx(local_base<S, C, RBW>(tl, tr));
     const uint32_t sh = a.qstart + S - C;
     auto apply_d = [&](auto tab) {
+      // the phases of the bits in dmask, from a table whose first chunk
+      // covers global bits dshift.. (the register bits are in dmask)
       float2 ph[16];
-      ph[0] = dphase_of(gb, tab, a.dphase_chunks);
+      ph[0] = dphase_of((gb & a.dmask) >> a.dshift, tab, a.dphase_chunks);
 #pragma unroll
       for (int i = 0; i < 4; ++i) {
-        const uint32_t p = sh + i;
+        const uint32_t p = sh + i - a.dshift;
         const float2 f = tab_at(tab, ((p >> 3) << 8) | (1u << (p & 7)));
 #pragma unroll
         for (int j = 1 << i; j < (2 << i); ++j) ph[j] = cmul(ph[j - (1 << i)], f);
@@ -677,9 +681,19 @@ __global__ void __launch_bounds__(1 << (TB - 4)) k_tile(PassArgs a) {
 // pass is bound by instruction issue, not by load latency.)
 constexpr int kMassPass = F_LOAD | F_GATES | F_MASS;
 constexpr int kMassOnlyPass = F_LOAD | F_GATES | F_MASS | F_NOSTORE;
+// The passes of a fuse_walsh schedule that load the state: strided
+// [wht][phase][gates] and the contiguous [wht][phase][gates][mass] (storing
+// the state or not).
+constexpr int kFusedPass = F_LOAD | F_WHT | F_PHASE | F_GATES;
+constexpr int kFusedMassPass = kFusedPass | F_MASS;
+constexpr int kFusedMassOnlyPass = kFusedPass | F_MASS | F_NOSTORE;
 template <int TB, int RBW, int FL>
 __global__ void __launch_bounds__(1 << (TB - 4)) k_tile_mass(PassArgs a) {
   tile_run<TB, 0, RBW, false, FL>(a, (uint64_t)blockIdx.x + ((uint64_t)blockIdx.y << 24));
+}
+template <int TB, int C, int RBW, int FL>
+__global__ void __launch_bounds__(1 << (TB - 4)) k_tile_fixed(PassArgs a) {
+  tile_run<TB, C, RBW, false, FL>(a, (uint64_t)blockIdx.x + ((uint64_t)blockIdx.y << 24));
 }
 
 // The same pass over many states of one plan in one launch.  The states of
@@ -727,13 +741,24 @@ hipError_t launch_tiles(const void* kernel, const PassArgs& a, size_t shmem, hip
 template <int TB, int C, int RBW>
 hipError_t launch_tile(const PassArgs& a, size_t shmem, hipStream_t st) {
   if constexpr (C == 0) {
-    static bool mass_attr = false, mass_only_attr = false;
+    static bool mass_attr = false, mass_only_attr = false, fmass_attr = false, fmass_only_attr = false;
     if (a.flags == kMassPass)
       return launch_tiles<TB>(reinterpret_cast<const void*>(&k_tile_mass<TB, RBW, kMassPass>), a, shmem, st,
                               mass_attr);
     if (a.flags == kMassOnlyPass)
       return launch_tiles<TB>(reinterpret_cast<const void*>(&k_tile_mass<TB, RBW, kMassOnlyPass>), a, shmem, st,
                               mass_only_attr);
+    if (a.flags == kFusedMassPass)
+      return launch_tiles<TB>(reinterpret_cast<const void*>(&k_tile_mass<TB, RBW, kFusedMassPass>), a, shmem, st,
+                              fmass_attr);
+    if (a.flags == kFusedMassOnlyPass)
+      return launch_tiles<TB>(reinterpret_cast<const void*>(&k_tile_mass<TB, RBW, kFusedMassOnlyPass>), a, shmem,
+                              st, fmass_only_attr);
+  } else {
+    static bool fused_attr = false;
+    if (a.flags == kFusedPass)
+      return launch_tiles<TB>(reinterpret_cast<const void*>(&k_tile_fixed<TB, C, RBW, kFusedPass>), a, shmem, st,
+                              fused_attr);
   }
   static bool attr_set = false;
   return launch_tiles<TB>(reinterpret_cast<const void*>(&k_tile<TB, C, RBW>), a, shmem, st, attr_set);

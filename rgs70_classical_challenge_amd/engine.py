@@ -45,7 +45,7 @@ def default_tuning(nbits):
     pass, strided passes of 14-bit tiles with 64-amplitude rows and their LDS
     exchanges in four rounds)."""
     return dict(low_bits=min(12, nbits), tile_bits=14, lane_bits=6, round_bits=2, phase_round_bits=-1,
-                dphase_global=0, top_bits=0, group_order=0)
+                dphase_global=0, top_bits=0, group_order=0, fuse_walsh=0)
 
 
 def resolve_tuning(nbits, props, tuning=None, kind=''):
@@ -60,7 +60,8 @@ def resolve_tuning(nbits, props, tuning=None, kind=''):
             entry = lookup_props(nbits, props)
         if entry is not None:
             tuning = {k: entry[k] for k in ('low_bits', 'tile_bits', 'lane_bits', 'round_bits', 'phase_round_bits',
-                                            'dphase_global', 'top_bits', 'group_order', 'groups') if k in entry}
+                                            'dphase_global', 'top_bits', 'group_order', 'groups', 'fuse_walsh')
+                      if k in entry}
     return dict(default_tuning(nbits), **(tuning or {}))
 
 
@@ -399,6 +400,9 @@ class GraphStateSampler:
         if self.pipeline == 'slab':
             d = sl.desc(_lib.INIT_GRAPH, 2.0 ** (-n), init_lut=self._lut, init_rows=self._rows, walsh=True,
                         gates=self._gates)
+            # each pass: Walsh, D2 phase and rotations of its own bits (the
+            # tuning's choice; no sign table follows the Walsh transform here)
+            d.fuse_walsh = int(sl.tuning.get('fuse_walsh', 0))
         elif self.pipeline == 'literal':
             # H^n = WHT / 2^{n/2}: the delta carries the normalisation
             d = sl.desc(_lib.INIT_DELTA, 2.0 ** (-n / 2), walsh=True, phase_lut=self._lut,

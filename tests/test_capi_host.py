@@ -33,7 +33,7 @@ def test_library_exports_every_declared_symbol():
     assert len(names) >= 10
     missing = [n for n in sorted(names) if not hasattr(lib, n)]
     assert not missing, missing
-    assert lib.rgs_abi_version() == 3
+    assert lib.rgs_abi_version() == 4
 
 
 def test_binding_covers_declarations():
@@ -162,7 +162,8 @@ def test_cov_luts_reject_singular_cut():
 
 
 def describe(nbits, walsh, gates=True, phase=False, mass=True, init=_lib.INIT_GRAPH, low=14, tile=14, lane=4,
-             rounds=0, phase_rounds=-1, top=0, order=0, groups=(), defer=False, desc_out=None):
+             rounds=0, phase_rounds=-1, top=0, order=0, groups=(), defer=False, desc_out=None, dphase=False,
+             fuse=0):
     d = _lib.SlabDesc()
     d.state = 1
     d.nbits = nbits
@@ -181,6 +182,8 @@ def describe(nbits, walsh, gates=True, phase=False, mass=True, init=_lib.INIT_GR
     for i, (bits, glane, grounds) in enumerate(groups):
         d.group_bits[i], d.group_lanes[i], d.group_rounds[i] = bits, glane, grounds
     d.defer_leaves = int(defer)
+    d.dphase_lut = 1 if dphase else None
+    d.fuse_walsh = fuse
     if desc_out is not None:
         desc_out.append(d)
     n = ctypes.c_int()
@@ -263,6 +266,31 @@ def test_explicit_group_schedules():
             describe(22, walsh=True, low=8, groups=bad)
     with pytest.raises(ValueError):
         describe(20, walsh=True, low=6, groups=[(7, 7, 0), (7, 6, 0)])   # 7 lane bits, 6 bits below the group
+
+
+def test_fused_walsh_schedules():
+    """fuse_walsh: every pass takes the Walsh butterflies, the diagonal
+    phase and the rotations of its own bits - the first strided group makes
+    the state and the contiguous pass ends with the masses; refused with a
+    sign table after the Walsh transform or without the transform."""
+    n, text = describe(34, walsh=True, dphase=True, low=13, tile=13, lane=6, fuse=1)
+    assert [t.split('/')[0] for t in text.split()] == [
+        'strided[init][wht][phase][gates]:13+7', 'strided[wht][phase][gates]:20+7',
+        'strided[wht][phase][gates]:27+7', 'low[wht][phase][gates][mass]:0+13'], text
+    n, text = describe(34, walsh=True, dphase=True, low=13, tile=13, lane=6, order=1, fuse=1)
+    assert [t.split('/')[0] for t in text.split()][:3] == [
+        'strided[init][wht][phase][gates]:27+7', 'strided[wht][phase][gates]:20+7',
+        'strided[wht][phase][gates]:13+7'], text
+    n, text = describe(34, walsh=True, dphase=True, low=13, tile=13, lane=6, fuse=1, defer=True)
+    assert n == 4 and text.split()[-1] == 'low[wht][phase][gates][mass][leaf]:0+13/tb13/c0/rb0', text
+    n, text = describe(12, walsh=True, dphase=True, fuse=1)
+    assert n == 1 and text.startswith('low[init][wht][phase][gates][mass]'), text
+    with pytest.raises(ValueError, match='fuse_walsh'):
+        describe(34, walsh=True, phase=True, fuse=1)
+    with pytest.raises(ValueError, match='fuse_walsh'):
+        describe(34, walsh=False, dphase=True, fuse=1)
+    with pytest.raises(ValueError, match='fuse_walsh'):
+        describe(34, walsh=True, dphase=True, fuse=2)
 
 
 def test_deferred_leaf_schedules():

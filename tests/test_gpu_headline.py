@@ -69,18 +69,19 @@ def _check_p(p_dev, p_ref, n):
     assert np.all(err <= bar), (err / bar).max()
 
 
-def _sampler(n, seed, pipeline):
+def _sampler(n, seed, pipeline, fuse=0):
     from rgs70_classical_challenge_amd.engine import GraphStateSampler
     g, u = instance(n, seed)
-    s = GraphStateSampler(g, u, pipeline=pipeline, tuning=HEADLINE)
+    s = GraphStateSampler(g, u, pipeline=pipeline, tuning=dict(HEADLINE, fuse_walsh=fuse))
     return g, u, s
 
 
-@pytest.mark.parametrize('pipeline', ['slab', 'literal', 'folded'])
-def test_headline_kernels_at_n27(pipeline):
+@pytest.mark.parametrize('pipeline,fuse', [('slab', 0), ('slab', 1), ('literal', 0), ('folded', 0)])
+def test_headline_kernels_at_n27(pipeline, fuse):
     n = 27
-    g, u, s = _sampler(n, 2727, pipeline)
-    print(f'case n={n} pipeline={pipeline} plan={s.plan_text}', flush=True)
+    g, u, s = _sampler(n, 2727, pipeline, fuse)
+    print(f'case n={n} pipeline={pipeline} fuse_walsh={fuse} plan={s.plan_text}', flush=True)
+    assert s.passes == (3 if fuse else {'slab': 5, 'literal': 5, 'folded': 3}[pipeline]), s.plan_text
     assert _kernels(s.plan_text) <= {'tb13/c0/rb0', 'tb13/c6/rb0'}, s.plan_text
     assert 'tb13/c6/rb0' in _kernels(s.plan_text)
     s.compute()
@@ -114,8 +115,10 @@ def test_headline_kernels_at_n27(pipeline):
     torch.cuda.empty_cache()
 
 
-def test_headline_n34_full_size():
-    """The bench's instance and plan at full size: 2^34 complex64 amplitudes."""
+@pytest.mark.parametrize('fuse', [0, 1])
+def test_headline_n34_full_size(fuse):
+    """The bench's instance and plans at full size: 2^34 complex64 amplitudes
+    (the seven-pass plan, and the four-pass plan of fuse_walsh)."""
     n = 34
     gc.collect()
     torch.cuda.empty_cache()
@@ -123,9 +126,9 @@ def test_headline_n34_full_size():
     need = (8 << n) + (16 << (n - 8)) + (4 << 30)
     if free < need:
         pytest.skip(f'the 2^34 state needs {need} bytes of free device memory; {free} free')
-    g, u, s = _sampler(n, BENCH_SEED, 'slab')
-    print(f'case n=34 bench instance plan={s.plan_text}', flush=True)
-    assert s.passes == 7 and _kernels(s.plan_text) == {'tb13/c0/rb0', 'tb13/c6/rb0'}, s.plan_text
+    g, u, s = _sampler(n, BENCH_SEED, 'slab', fuse)
+    print(f'case n=34 bench instance fuse_walsh={fuse} plan={s.plan_text}', flush=True)
+    assert s.passes == (4 if fuse else 7) and _kernels(s.plan_text) == {'tb13/c0/rb0', 'tb13/c6/rb0'}, s.plan_text
     s.compute()
     sl = s.slab
     root = sl.root()

@@ -95,6 +95,14 @@ TUNINGS = [None, dict(low_bits=8, tile_bits=8, lane_bits=3), dict(low_bits=10, t
            # k_tile<13, 6, 0>)
            HEADLINE]
 
+# fuse_walsh (slab pipeline): each pass takes the Walsh butterflies, the D2
+# phases of its own bits and their rotations; the first strided group makes
+# the state
+FUSED = [dict(HEADLINE, fuse_walsh=1), dict(low_bits=8, tile_bits=10, lane_bits=3, fuse_walsh=1),
+         dict(low_bits=10, tile_bits=12, lane_bits=4, round_bits=2, phase_round_bits=0, dphase_global=1,
+              fuse_walsh=1),
+         dict(low_bits=8, tile_bits=12, lane_bits=4, group_order=1, fuse_walsh=1)]
+
 
 @pytest.mark.parametrize('pipeline', ['slab', 'literal', 'folded'])
 @pytest.mark.parametrize('n', [10, 16, 20])
@@ -103,11 +111,14 @@ def test_graph_state_pipelines(pipeline, n):
     g, u = instance(n, 1000 + n)
     ref = gs.full_state(g, u)
     rng = np.random.default_rng(n)
-    for tun in TUNINGS:
+    for tun in TUNINGS + (FUSED if pipeline == 'slab' else []):
         if tun and tun['low_bits'] > n:
             continue
         print(f'case pipeline={pipeline} n={n} tuning={tun}', flush=True)
         s = E.GraphStateSampler(g, u, pipeline=pipeline, tuning=tun)
+        if tun and tun.get('fuse_walsh') and n > tun['low_bits']:
+            assert s.plan_text.startswith('strided[init][wht][phase][gates]'), s.plan_text
+            assert s.plan_text.split()[-1].startswith('low[wht][phase][gates][mass]'), s.plan_text
         s.compute()
         check_state(s.host(), ref)
         assert abs(s.norm() - 1) < 1e-5
@@ -165,10 +176,12 @@ def test_walsh_and_gates_on_arbitrary_input(tun):
     u = haar_unitaries(n, rng)
     sl = _load_slab(n, tun)
     gates = E.Gates(sl, u, gate_limit=n - 1)
-    for walsh, use_gates in ((True, False), (False, True), (True, True)):
-        print(f'case walsh={walsh} gates={use_gates} tuning={tun}', flush=True)
+    for walsh, use_gates, fuse in ((True, False, 0), (False, True, 0), (True, True, 0), (True, False, 1),
+                                   (True, True, 1)):
+        print(f'case walsh={walsh} gates={use_gates} fuse_walsh={fuse} tuning={tun}', flush=True)
         sl.state.copy_(torch.as_tensor(v))
         d = sl.desc(_lib.INIT_LOAD, walsh=walsh, gates=gates if use_gates else None, gate_limit=n - 1, mass=False)
+        d.fuse_walsh = fuse
         sl.run(d, mass=False)
         sl.sync()
         got = sl.state.cpu().numpy().astype(np.complex128)
