@@ -88,22 +88,23 @@ def pass_bytes(plan_text, n):
     return out
 
 
-def kernel_of(tok):
+def kernel_of(tok, fused=False):
     """Kernel of a plan token such as 'strided[wht]:14+8/tb14/c6/rb2': the
     contiguous mass passes have their flags fixed at compile time
     (k_tile_mass<TB, RBW, flags>: gates + masses 25 storing the state, 89
-    masses only; Walsh + phase + gates + masses 31 and 95), and so do the
-    strided passes that load the state and apply Walsh, phase and gates
-    (k_tile_fixed<TB, C, RBW, 15>); every other pass is the k_tile
-    specialisation of its shape."""
+    masses only; the fused schedule's Walsh + phase + gates + masses, layout
+    by layout, 159 and 223), and so do the strided passes that load the state
+    and apply Walsh, phase and gates (k_tile_fixed<TB, C, RBW, flags>: 143 in
+    the fused schedule's single walk, 15 as the seven-pass schedule's phase
+    pass); every other pass is the k_tile specialisation of its shape."""
     kind, tb, c, rb = tok.split('/')
     head = kind.split(':')[0]
-    mass = {'low[gates][mass]': 25, 'low[gates][mass][leaf]': 89, 'low[wht][phase][gates][mass]': 31,
-            'low[wht][phase][gates][mass][leaf]': 95}
+    mass = {'low[gates][mass]': 25, 'low[gates][mass][leaf]': 89, 'low[wht][phase][gates][mass]': 159,
+            'low[wht][phase][gates][mass][leaf]': 223}
     if head in mass:
         return f'k_tile_mass<{int(tb[2:])}, {int(rb[2:])}, {mass[head]}>'
     if head == 'strided[wht][phase][gates]':
-        return f'k_tile_fixed<{int(tb[2:])}, {int(c[1:])}, {int(rb[2:])}, 15>'
+        return f'k_tile_fixed<{int(tb[2:])}, {int(c[1:])}, {int(rb[2:])}, {143 if fused else 15}>'
     return f'k_tile<{int(tb[2:])}, {int(c[1:])}, {int(rb[2:])}>'
 
 
@@ -164,7 +165,7 @@ def run_headline(args, torch):
     wht_gbps = sum(nbytes[p] for p in wht) / sum(pass_ms[p] for p in wht) / 1e6 if wht else None
     kernels = {}
     for p, tok in enumerate(s.plan_text.split()):
-        k = kernels.setdefault(kernel_of(tok), {'launches': 0, 'ms': 0.0, 'bytes': 0})
+        k = kernels.setdefault(kernel_of(tok, bool(s._desc.fuse_walsh)), {'launches': 0, 'ms': 0.0, 'bytes': 0})
         k['launches'] += 1
         k['ms'] += pass_ms[p]
         k['bytes'] += nbytes[p]

@@ -198,13 +198,17 @@ typedef struct rgs_slab_desc {
                                   0: every pass writes the state             */
   int fuse_walsh;              /* with do_walsh and no phase_lut: 1 gives each
                                   pass the Walsh butterflies, the diagonal
-                                  phases (dphase_lut restricted to the pass's
-                                  bits) and the rotations of its own target
+                                  phases and the rotations of its own target
                                   bits, so the first strided group makes the
                                   state and one pass per group follows (four
                                   at n = 34 instead of seven); the per-qubit
-                                  order Walsh -> phase -> rotation is kept.
+                                  order Walsh -> phase -> rotation is kept,
+                                  register layout by register layout.
                                   0: the Walsh transform on all bits first   */
+  const float *dphase_host;    /* HOST copy of dphase_lut (same layout),
+                                  required with fuse_walsh and dphase_lut: the
+                                  fused passes take each register layout's
+                                  phases from it as kernel arguments         */
 } rgs_slab_desc;
 
 /* Factor 2x2 unitaries u (complex128 [n][2][2]) as

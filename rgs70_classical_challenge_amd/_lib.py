@@ -59,6 +59,7 @@ class SlabDesc(ctypes.Structure):
         ('init_wpop', c_int),
         ('defer_leaves', c_int),
         ('fuse_walsh', c_int),
+        ('dphase_host', c_void_p),
     ]
 
 

@@ -27,31 +27,14 @@ namespace rgs {
 
 static int chunks_of(int bits) { return (bits + 7) / 8; }
 
-// The diagonal phases a pass applies: all bits, or with fuse_walsh the pass's
-// own target bits.  Returns the number of 8-bit chunks of dphase_lut that
-// cover them, with the first chunk in *c0 and the bits in *mask.
-static int dphase_span(const rgs_slab_desc* d, const PassPlan& p, int* c0, uint64_t* mask) {
-  int lo = 0, hi = d->nbits;
-  if (d->fuse_walsh) {
-    lo = p.low ? 0 : p.qstart;
-    hi = lo + p.count;
-  }
-  *c0 = lo / 8;
-  *mask = (hi >= 64 ? ~0ull : (1ull << hi) - 1ull) & ~((1ull << lo) - 1ull);
-  return chunks_of(hi) - lo / 8;
-}
-
 size_t lds_bytes(const PassPlan& p, int nbits, const rgs_slab_desc* d) {
   size_t slots = ((size_t)1 << (p.tb - p.rb)) + ((size_t)1 << (p.tb - p.rb - 4));
   size_t bytes = slots * 8;
   // LDS tables: the phase pass's diagonal-phase table (unless it is read from
   // global memory) and the covariant init's weights; sign and map tables are
   // read from global memory
-  if ((p.flags & F_PHASE) && d->dphase_lut && !d->dphase_global) {
-    int c0;
-    uint64_t mask;
-    bytes += (size_t)dphase_span(d, p, &c0, &mask) * 256 * 8;
-  }
+  if ((p.flags & F_PHASE) && !(p.flags & F_LAYER) && d->dphase_lut && !d->dphase_global)
+    bytes += (size_t)chunks_of(nbits) * 256 * 8;
   if (!(p.flags & F_LOAD) && d->init_kind == RGS_INIT_COV) {
     const int wb = d->init_wbits ? d->init_wbits : 8;
     // the popcount table as staged: (|o| + 1) (m - |o| + 1) entries, at most
@@ -74,6 +57,10 @@ int plan_passes(const rgs_slab_desc* d, std::vector<PassPlan>& out, std::string&
   if (d->fuse_walsh != 0 && d->fuse_walsh != 1) { err = "fuse_walsh must be 0 or 1"; return kInvalid; }
   if (d->fuse_walsh && (!d->do_walsh || d->phase_lut)) {
     err = "fuse_walsh needs the Walsh transform and no sign table after it (phase_lut)";
+    return kInvalid;
+  }
+  if (d->fuse_walsh && d->dphase_lut && !d->dphase_host) {
+    err = "fuse_walsh with dphase_lut needs its host copy (dphase_host)";
     return kInvalid;
   }
   std::vector<PassPlan> groups;
@@ -168,7 +155,7 @@ int plan_passes(const rgs_slab_desc* d, std::vector<PassPlan>& out, std::string&
     // every pass takes the Walsh butterflies, the diagonal phases and the
     // rotations of its own bits (operations on different qubits commute):
     // the first group makes the state, the contiguous pass ends with the masses
-    const int own = F_WHT | (phase ? F_PHASE : 0) | (gates ? F_GATES : 0);
+    const int own = F_LAYER | F_WHT | (phase ? F_PHASE : 0) | (gates ? F_GATES : 0);
     for (int i = 0; i < k; ++i) out.push_back(with(groups[d->group_order ? k - 1 - i : i], (i ? F_LOAD : src) | own));
     out.push_back(with(low, F_LOAD | own | (mass ? F_MASS : 0)));
   } else if (walsh) {
@@ -269,6 +256,39 @@ static void local_form(const PassPlan& p, const uint64_t* rows, LocalForm& F) {
   }
 }
 
+// The per-layout diagonal phases of an F_LAYER pass: the layouts walk down
+// from max(tb - 4, c) in steps of four bits to c, layout K owning local bits
+// [s_K, s_{K-1}) (s_{-1} = tb); lph[K][m] is the product, in double and
+// rounded once, of the single-bit phases e^{i b_q} (entry 1 << (q & 7) of
+// chunk q >> 3 of the host table) of the owned bits set in m.
+static void layer_phases(const rgs_slab_desc* d, const PassPlan& p, PassArgs& a) {
+  const int s1 = std::max(p.tb - 4, p.c);
+  int s = s1, h = p.tb;
+  for (int K = 0; K < 4; ++K) {
+    double re[16], im[16];
+    re[0] = 1.0;
+    im[0] = 0.0;
+    for (int i = 0; i < 4; ++i) {
+      double fr = 1.0, fi = 0.0;
+      if (s + i < h) {
+        const int q = global_bit(p, s + i);
+        const float* e = d->dphase_host + 2 * (((size_t)(q >> 3) << 8) | (1u << (q & 7)));
+        fr = e[0];
+        fi = e[1];
+      }
+      for (int j = 1 << i; j < (2 << i); ++j) {
+        const int b = j - (1 << i);
+        re[j] = re[b] * fr - im[b] * fi;
+        im[j] = re[b] * fi + im[b] * fr;
+      }
+    }
+    for (int j = 0; j < 16; ++j) a.lph[K][j] = make_float2((float)re[j], (float)im[j]);
+    if (s <= p.c) break;
+    h = s;
+    s = std::max(s - 4, p.c);
+  }
+}
+
 int check_desc(const rgs_slab_desc* d, std::string& err) {
   const int n = d->nbits;
   if (d->init_kind < 0 || d->init_kind > 3) { err = "bad init_kind"; return kInvalid; }
@@ -302,8 +322,6 @@ void build_args(const rgs_slab_desc* d, const PassPlan& p, PassArgs& a) {
   a.init_w = reinterpret_cast<const float2*>(d->init_wlut);
   a.phase_lut = d->phase_lut;
   a.dphase = reinterpret_cast<const float2*>(d->dphase_lut);
-  a.dshift = 0;
-  a.dmask = ~0ull;
   // rotations of this pass's target bits travel in the kernel arguments;
   // qubits at or above gate_limit get t = 0 (the kernel rotates every bit)
   const int t0 = p.low ? 0 : p.qstart;
@@ -329,13 +347,8 @@ void build_args(const rgs_slab_desc* d, const PassPlan& p, PassArgs& a) {
   const bool gen = !(p.flags & F_LOAD);
   a.init_chunks = (gen && (d->init_kind == RGS_INIT_GRAPH || d->init_kind == RGS_INIT_COV)) ? chunks_of(n) : 0;
   a.phase_chunks = ((p.flags & F_PHASE) && d->phase_lut) ? chunks_of(n) : 0;
-  a.dphase_chunks = 0;
-  if ((p.flags & F_PHASE) && d->dphase_lut) {
-    int c0;
-    a.dphase_chunks = dphase_span(d, p, &c0, &a.dmask);
-    a.dphase += (size_t)c0 * 256;
-    a.dshift = 8 * c0;
-  }
+  a.dphase_chunks = ((p.flags & F_PHASE) && !(p.flags & F_LAYER) && d->dphase_lut) ? chunks_of(n) : 0;
+  if ((p.flags & F_LAYER) && (p.flags & F_PHASE) && d->dphase_lut) layer_phases(d, p, a);
   a.gate_limit = d->rot ? std::min(d->gate_limit, n) : 0;
   {
     const int s1 = std::max(p.tb - 4, p.c);

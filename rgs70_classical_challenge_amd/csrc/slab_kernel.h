@@ -36,7 +36,13 @@ namespace rgs {
 // the store layout (one LDS exchange fewer than walking down and back).
 // F_NOSTORE: the contiguous mass pass writes the masses only (the state
 // keeps the values it loaded; its tiles are recomputed on demand).
-enum : int { F_LOAD = 1, F_WHT = 2, F_PHASE = 4, F_GATES = 8, F_MASS = 16, F_INIT_UP = 32, F_NOSTORE = 64 };
+// F_LAYER: one walk down the layouts, each applying the Walsh butterflies,
+// the diagonal phases (PassArgs::lph) and the rotations of the register bits
+// it owns (a pass of a fuse_walsh schedule: per qubit Walsh -> phase ->
+// rotation, and operations on different qubits commute).
+enum : int {
+  F_LOAD = 1, F_WHT = 2, F_PHASE = 4, F_GATES = 8, F_MASS = 16, F_INIT_UP = 32, F_NOSTORE = 64, F_LAYER = 128
+};
 
 constexpr size_t kLdsBudget = 160 * 1024;
 
@@ -83,8 +89,6 @@ struct PassArgs {
   int gate_limit;
   int mass_bits;
   int dphase_global;  // read dphase from global memory instead of staging it
-  int dshift;         // dphase starts at chunk dshift / 8 of the global index
-  uint64_t dmask;     // global bits whose diagonal phases this pass applies
   int wbits;          // COV weight chunk bits (8 or 9)
   int wchunks;        // COV weight chunks
   int wpop;           // COV popcount table: its row stride m + 1 (0: chunk products)
@@ -93,6 +97,9 @@ struct PassArgs {
   uint64_t womask;    // COV popcount table: the B outcome bits o
   uint64_t wnmask;    // COV popcount table: the other B bits (~o, m bits)
   uint64_t mflip;     // mass blocks: block b of the state is outcome block b ^ mflip
+  // F_LAYER with F_PHASE: lph[K][m] = the diagonal phase of register offset m
+  // over the bits that layout K owns (m a subset of them; lph[K][0] = 1)
+  float2 lph[4][16];
 };
 
 __device__ __forceinline__ uint32_t phase_parity(uint64_t x, const uint64_t* lut, int chunks) {
@@ -432,6 +439,34 @@ __device__ __forceinline__ void walk_rot_up(float2 (&v)[16], float2* lds, uint32
   }
 }
 
+// Diagonal phases of the register bits layout K owns (the low h - s bits of
+// the register index): register j takes lph[K][j & own].
+template <typename G, int K>
+__device__ __forceinline__ void phase_layout(float2 (&v)[16], const PassArgs& a) {
+  constexpr int s = G::at(K), h = G::hi(K);
+  constexpr int own = (1 << cmin(4, h - s)) - 1;
+#pragma unroll
+  for (int j = 1; j < 16; ++j)
+    if (j & own) v[j] = cmul(v[j], a.lph[K][j & own]);
+}
+
+// One walk down from layout K (ends at layout count-1): each layout applies
+// the Walsh butterflies, the diagonal phases and the rotations of the bits it
+// owns, in that order (F_LAYER).
+template <int TB, int C, int RBW, int K>
+__device__ __forceinline__ void walk_layer(float2 (&v)[16], float2* lds, uint32_t tl, uint32_t tr,
+                                           const PassArgs& a, int flags) {
+  using G = Geo<TB, C, RBW>;
+  static_assert(G::count() <= 4, "lph holds four layouts");
+  if (flags & F_WHT) wht_layout<G, K>(v);
+  if (flags & F_PHASE) phase_layout<G, K>(v, a);
+  if (flags & F_GATES) rot_layout<G, C, K>(v, a);
+  if constexpr (K + 1 < G::count()) {
+    exchange<TB, C, RBW, G::at(K), G::at(K + 1)>(v, lds, tl, tr);
+    walk_layer<TB, C, RBW, K + 1>(v, lds, tl, tr, a, flags);
+  }
+}
+
 // FP64 masses of the 2^mass_bits blocks of a contiguous tile at state index
 // base, from registers in layout 0 (local bits 0..3): the pairwise tree over
 // the thread's 16 values, then across the lanes.  A block's pairwise sum is
@@ -561,13 +596,11 @@ __device__ __forceinline__ void tile_run(const PassArgs& a, uint64_t blk, float2
     const uint64_t gb = gidx(local_base<S, C, RBW>(tl, tr));
     const uint32_t sh = a.qstart + S - C;
     auto apply_d = [&](auto tab) {
-      // the phases of the bits in dmask, from a table whose first chunk
-      // covers global bits dshift.. (the register bits are in dmask)
       float2 ph[16];
-      ph[0] = dphase_of((gb & a.dmask) >> a.dshift, tab, a.dphase_chunks);
+      ph[0] = dphase_of(gb, tab, a.dphase_chunks);
 #pragma unroll
       for (int i = 0; i < 4; ++i) {
-        const uint32_t p = sh + i - a.dshift;
+        const uint32_t p = sh + i;
         const float2 f = tab_at(tab, ((p >> 3) << 8) | (1u << (p & 7)));
 #pragma unroll
         for (int j = 1 << i; j < (2 << i); ++j) ph[j] = cmul(ph[j - (1 << i)], f);
@@ -595,6 +628,9 @@ __device__ __forceinline__ void tile_run(const PassArgs& a, uint64_t blk, float2
   if (up) {
     if constexpr (C == 0) walk_wht_up<TB, C, RBW, G::count() - 1>(v, lds, tl, tr);
     at_top = true;
+  } else if (flags & F_LAYER) {
+    walk_layer<TB, C, RBW, 0>(v, lds, tl, tr, a, flags);
+    at_top = (SL == S1);
   } else if (flags & F_WHT) {
     walk_wht<TB, C, RBW, 0>(v, lds, tl, tr);
     if (flags & F_PHASE) apply_phase(std::integral_constant<int, SL>());
@@ -684,7 +720,7 @@ constexpr int kMassOnlyPass = F_LOAD | F_GATES | F_MASS | F_NOSTORE;
 // The passes of a fuse_walsh schedule that load the state: strided
 // [wht][phase][gates] and the contiguous [wht][phase][gates][mass] (storing
 // the state or not).
-constexpr int kFusedPass = F_LOAD | F_WHT | F_PHASE | F_GATES;
+constexpr int kFusedPass = F_LOAD | F_WHT | F_PHASE | F_GATES | F_LAYER;
 constexpr int kFusedMassPass = kFusedPass | F_MASS;
 constexpr int kFusedMassOnlyPass = kFusedPass | F_MASS | F_NOSTORE;
 template <int TB, int RBW, int FL>

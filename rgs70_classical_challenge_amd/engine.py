@@ -128,6 +128,7 @@ class _Slab:
         d.phase_lut = ptr(phase_lut)
         d.phase_rows = ptr(phase_rows)
         d.dphase_lut = ptr(gates.dphase) if gates is not None else None
+        d.dphase_host = gates.dphase_host.ctypes.data if gates is not None else None
         d.rot = ptr(gates.rot) if gates is not None else None
         d.gate_limit = self.nbits if gate_limit is None else gate_limit
         d.mass = ptr(self.pyramid) if mass else None
@@ -282,6 +283,7 @@ class Gates:
         self.rot = tan                      # host: read when passes are enqueued
         self.flip = int(flip.value)
         self.dphase = slab.to_device(dph)
+        self.dphase_host = dph              # host copy: per-layout phases of fused passes
         self.d1 = d1.view(np.complex128).reshape(n, 2)
 
     def apply_d1(self, v):

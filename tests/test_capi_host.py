@@ -163,7 +163,7 @@ def test_cov_luts_reject_singular_cut():
 
 def describe(nbits, walsh, gates=True, phase=False, mass=True, init=_lib.INIT_GRAPH, low=14, tile=14, lane=4,
              rounds=0, phase_rounds=-1, top=0, order=0, groups=(), defer=False, desc_out=None, dphase=False,
-             fuse=0):
+             fuse=0, dphase_host=True):
     d = _lib.SlabDesc()
     d.state = 1
     d.nbits = nbits
@@ -183,6 +183,7 @@ def describe(nbits, walsh, gates=True, phase=False, mass=True, init=_lib.INIT_GR
         d.group_bits[i], d.group_lanes[i], d.group_rounds[i] = bits, glane, grounds
     d.defer_leaves = int(defer)
     d.dphase_lut = 1 if dphase else None
+    d.dphase_host = 1 if dphase and dphase_host else None     # planned only, never read here
     d.fuse_walsh = fuse
     if desc_out is not None:
         desc_out.append(d)
@@ -291,6 +292,8 @@ def test_fused_walsh_schedules():
         describe(34, walsh=False, dphase=True, fuse=1)
     with pytest.raises(ValueError, match='fuse_walsh'):
         describe(34, walsh=True, dphase=True, fuse=2)
+    with pytest.raises(ValueError, match='dphase_host'):
+        describe(34, walsh=True, dphase=True, fuse=1, dphase_host=False)
 
 
 def test_deferred_leaf_schedules():
