@@ -414,19 +414,21 @@ def test_plain_tuner_refuses_sizes_above_its_limit():
 
 def test_bench_plan_is_the_plan_the_gpu_tests_check():
     """The autotuned n = 34 entry the bench runs equals the tuning the GPU
-    tests run at n = 20, 27 and 34 (so its two k_tile specialisations are the
-    ones checked against the oracle)."""
+    tests run at n = 26 and 34 (so its three tile kernels are the ones
+    checked against the oracle)."""
     import json
     import os
-    from test_gpu_headline import HEADLINE
+    from test_gpu_headline import BENCH_PLAN, BENCH_PLAN_26, BENCH_KERNELS
     table = json.load(open(os.path.join(os.path.dirname(_lib.LIB_PATH), 'tuned.json')))
     for dev, entries in table.items():
         e = entries['34']
-        keys = ('low_bits', 'tile_bits', 'lane_bits', 'round_bits', 'group_order')
-        assert {k: e.get(k, 0) for k in keys} == {k: HEADLINE.get(k, 0) for k in keys}, dev
+        keys = ('low_bits', 'tile_bits', 'lane_bits', 'round_bits', 'group_order', 'fuse_walsh', 'groups')
+        assert {k: e.get(k, 0) for k in keys} == {k: BENCH_PLAN.get(k, 0) for k in keys}, dev
         assert e.get('top_bits', 0) == 0 and e.get('phase_round_bits', -1) in (-1, 0) and not e.get('dphase_global')
-    n, text = describe(34, walsh=True, low=13, tile=14, lane=6, rounds=0, order=1)
-    assert {tok.split('/', 1)[1] for tok in text.split()} == {'tb13/c0/rb0', 'tb13/c6/rb0'}, text
+    for n, plan in ((34, BENCH_PLAN), (26, BENCH_PLAN_26)):
+        _, text = describe(n, walsh=True, dphase=True, low=13, tile=13, lane=6, rounds=0, order=1, fuse=1,
+                           groups=plan['groups'])
+        assert {tok.split('/', 1)[1] for tok in text.split()} == BENCH_KERNELS, text
 
 
 def test_slab_batch_rejects_bad_items_before_touching_the_device():

@@ -30,8 +30,17 @@ pytestmark = pytest.mark.gpu
 
 torch = pytest.importorskip('torch')
 
-# the autotuned n = 34 entry (its 14-bit strided tiles narrow to 13 bits)
+# the seven-pass schedule's n = 34 tuning (its 14-bit strided tiles narrow to 13 bits)
 HEADLINE = dict(low_bits=13, tile_bits=14, lane_bits=6, round_bits=0, group_order=1)
+# the autotuned n = 34 entry the bench runs: the fused schedule with strided
+# groups of 8, 8 and 5 bits (32-amplitude rows), walked from the top:
+# k_tile<10, 5, 0> makes the state, k_tile_fixed<13, 5, 0, 143> runs the 8-bit
+# groups and k_tile_mass<13, 0, 159> the contiguous pass
+BENCH_PLAN = dict(low_bits=13, tile_bits=13, lane_bits=6, round_bits=0, group_order=1, fuse_walsh=1,
+                  groups=[[8, 5, 0], [8, 5, 0], [5, 5, 0]])
+# the same kernels at n = 26: one 8-bit and one 5-bit group
+BENCH_PLAN_26 = dict(BENCH_PLAN, groups=[[8, 5, 0], [5, 5, 0]])
+BENCH_KERNELS = {'tb10/c5/rb0', 'tb13/c5/rb0', 'tb13/c0/rb0'}
 BENCH_SEED = 20261203
 
 
@@ -69,21 +78,24 @@ def _check_p(p_dev, p_ref, n):
     assert np.all(err <= bar), (err / bar).max()
 
 
-def _sampler(n, seed, pipeline, fuse=0):
+def _sampler(n, seed, pipeline, tuning):
     from rgs70_classical_challenge_amd.engine import GraphStateSampler
     g, u = instance(n, seed)
-    s = GraphStateSampler(g, u, pipeline=pipeline, tuning=dict(HEADLINE, fuse_walsh=fuse))
+    s = GraphStateSampler(g, u, pipeline=pipeline, tuning=tuning)
     return g, u, s
 
 
-@pytest.mark.parametrize('pipeline,fuse', [('slab', 0), ('slab', 1), ('literal', 0), ('folded', 0)])
-def test_headline_kernels_at_n27(pipeline, fuse):
-    n = 27
-    g, u, s = _sampler(n, 2727, pipeline, fuse)
-    print(f'case n={n} pipeline={pipeline} fuse_walsh={fuse} plan={s.plan_text}', flush=True)
-    assert s.passes == (3 if fuse else {'slab': 5, 'literal': 5, 'folded': 3}[pipeline]), s.plan_text
-    assert _kernels(s.plan_text) <= {'tb13/c0/rb0', 'tb13/c6/rb0'}, s.plan_text
-    assert 'tb13/c6/rb0' in _kernels(s.plan_text)
+SEVEN = {'tb13/c0/rb0', 'tb13/c6/rb0'}
+
+
+@pytest.mark.parametrize('pipeline,tuning,n,passes,kernels', [
+    ('slab', HEADLINE, 27, 5, SEVEN), ('slab', dict(HEADLINE, fuse_walsh=1), 27, 3, SEVEN),
+    ('literal', HEADLINE, 27, 5, SEVEN), ('folded', HEADLINE, 27, 3, SEVEN),
+    ('slab', BENCH_PLAN_26, 26, 3, BENCH_KERNELS)], ids=['slab', 'slab-fused', 'literal', 'folded', 'slab-bench-plan'])
+def test_headline_kernels_at_n27(pipeline, tuning, n, passes, kernels):
+    g, u, s = _sampler(n, 2700 + n, pipeline, tuning)
+    print(f'case n={n} pipeline={pipeline} tuning={tuning} plan={s.plan_text}', flush=True)
+    assert s.passes == passes and _kernels(s.plan_text) == kernels, s.plan_text
     s.compute()
     root = s.slab.root()
     assert abs(root - 1) < 1e-6, root
@@ -115,7 +127,7 @@ def test_headline_kernels_at_n27(pipeline, fuse):
     torch.cuda.empty_cache()
 
 
-@pytest.mark.parametrize('fuse', [0, 1])
+@pytest.mark.parametrize('fuse', [0, 1], ids=['seven-pass', 'bench-plan'])
 def test_headline_n34_full_size(fuse):
     """The bench's instance and plans at full size: 2^34 complex64 amplitudes
     (the seven-pass plan, and the four-pass plan of fuse_walsh)."""
@@ -126,9 +138,9 @@ def test_headline_n34_full_size(fuse):
     need = (8 << n) + (16 << (n - 8)) + (4 << 30)
     if free < need:
         pytest.skip(f'the 2^34 state needs {need} bytes of free device memory; {free} free')
-    g, u, s = _sampler(n, BENCH_SEED, 'slab', fuse)
+    g, u, s = _sampler(n, BENCH_SEED, 'slab', BENCH_PLAN if fuse else HEADLINE)
     print(f'case n=34 bench instance fuse_walsh={fuse} plan={s.plan_text}', flush=True)
-    assert s.passes == (4 if fuse else 7) and _kernels(s.plan_text) == {'tb13/c0/rb0', 'tb13/c6/rb0'}, s.plan_text
+    assert (s.passes, _kernels(s.plan_text)) == ((4, BENCH_KERNELS) if fuse else (7, SEVEN)), s.plan_text
     s.compute()
     sl = s.slab
     root = sl.root()
