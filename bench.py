@@ -6,7 +6,10 @@ QR-drawn single-qubit unitaries; the state (x)_k u[k] |G> (2^34 complex64
 amplitudes, 128 GiB) is built by the slab pipeline - GF(2) phase init ->
 Walsh-Hadamard over all 34 bits -> per-qubit 2x2 gates -> FP64 masses - in
 a few fused HBM passes (tile plan from the autotuner table, else the default
-plan), the FP64 mass pyramid is built, and 10,000 exact samples are drawn
+plan; the table's n=34 entry is the fused Walsh schedule: four passes, each
+applying the Walsh butterflies, D2 phases and rotations of its own bits, the
+first making the state and the last writing the masses), the FP64 mass
+pyramid is built, and 10,000 exact samples are drawn
 qubit by qubit on the device.  The graph, tables and the 10,000 uniforms of
 every step are resident in HBM before the timed region.
 
